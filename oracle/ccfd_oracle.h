@@ -1,0 +1,99 @@
+/*
+ * ccfd_oracle.h - CPU restatement of ccfd's finite-volume hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker the HIP path is compared
+ * against; nothing in the product (ccfd_amd/, include/) links or calls it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it.
+ *
+ * Arrays are in the reference's element / slot order (ccfd_mesh), state as
+ * 4 doubles per element (rho, vx, vy, p) or (rho, mx, my, E).
+ */
+#ifndef CCFD_ORACLE_H
+#define CCFD_ORACLE_H
+
+#include "ccfd_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_cfg {
+	double gam, mu, Pr;        /* gamma, viscosity, Prandtl */
+	int navierstokes;          /* 1: -Dnavierstokes semantics */
+	int iFlux;                 /* 1..11, src/main.h:76-88 */
+	int spatialOrder;          /* 1 or 2 */
+	int limiter;               /* 1 BJ, 2 VK */
+	double venk_k;
+	/* exact functions (src/exactFunction.c) */
+	double refState[2][4];
+	double rp1Dinterface;
+	double dxRef;
+	/* time stepping (src/timeDiscretization.c) */
+	double cfl, dfl;
+	int timeStep1D;
+	int nRKstages, timeOrder;
+} oracle_cfg;
+
+typedef struct oracle_ctx oracle_ctx;
+
+oracle_ctx *oracle_create(const ccfd_mesh *m, const oracle_cfg *cfg);
+void oracle_destroy(oracle_ctx *o);
+
+/* element state */
+void oracle_set_prim(oracle_ctx *o, const double *prim);   /* also sets cons */
+void oracle_get_prim(const oracle_ctx *o, double *prim);
+void oracle_get_cons(const oracle_ctx *o, double *cons);
+
+/*
+ * fvTimeDerivative (src/finiteVolume.c:78-121) on the current primitive
+ * state. Outputs (any may be NULL): u_t (4/elem), u_x, u_y (4/elem, limited
+ * gradients), sideP (4/slot, side states), flux (4/slot, side fluxes).
+ */
+void oracle_fvtd(oracle_ctx *o, double time, double *u_t, double *u_x,
+		double *u_y, double *sideP, double *flux);
+
+/* single flux function on one rotated state pair, src/fluxCalculation.c */
+void oracle_riemann(const oracle_cfg *cfg, int iFlux, const double L[4],
+		const double R[4], double F[4]);
+
+/* calcTimeStep (src/timeDiscretization.c:249-329); returns dt */
+double oracle_calc_dt(oracle_ctx *o, double t, double pTime, double stopTime,
+		int *viscousDominates, int *nonfinite);
+
+/* one explicit step: Euler or low-storage RK (src/timeDiscretization.c:
+ * 337-401); resIter gets the global residual (src/analyze.c:521-539) */
+void oracle_explicit_step(oracle_ctx *o, double t, double dt, double res[4]);
+
+/* area-weighted L2 residual of the last fvTimeDerivative */
+void oracle_global_residual(const oracle_ctx *o, const double *u_t,
+		double res[4]);
+
+/* dot product of two 4-per-element vectors (src/linearSolver.c:115-128) */
+double oracle_dot(long nElems, const double *a, const double *b);
+
+/*
+ * Implicit backward-Euler step with JFNK-GMRES and optional LU-SGS
+ * (src/timeDiscretization.c:412-552, src/linearSolver.c:136-510).
+ * Returns 0, or -1 if Newton did not converge, -2 for a singular D block.
+ */
+typedef struct oracle_implicit_cfg {
+	int nKdim, nNewtonIter, precond;
+	double epsNewton, epsGMRES, gammaEW;
+} oracle_implicit_cfg;
+int oracle_implicit_step(oracle_ctx *o, const oracle_implicit_cfg *ic,
+		double t, double dt, double res[4], int *nNewton, int *nGMRES);
+
+/* RK coefficient table (src/timeDiscretization.c:94-178); returns -1 for an
+ * unsupported (stages, order) pair */
+int oracle_rk_coeffs(int nRKstages, int timeOrder, double coeff[6]);
+
+/* exact functions (src/exactFunction.c:35-183) and exact Riemann solver
+ * (src/exactRiemann.c:59-214) */
+void oracle_exact_func(const oracle_cfg *cfg, const ccfd_mesh *m, int f,
+		const double x[2], double time, double pVar[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
