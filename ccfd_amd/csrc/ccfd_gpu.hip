@@ -1,0 +1,961 @@
+// ccfd_gpu.hip - context management, mesh renumbering / upload, reductions,
+// explicit time stepping and the C ABI of include/ccfd_gpu.h.
+//
+// The operator kernels live in operator.hpp (instantiated per flux code in
+// op_flux.hip); the implicit solver in implicit.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ccfd_gpu.h"
+#include "ctx.hpp"
+
+using namespace ccfd;
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+int ccfd::fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+extern "C" const char* ccfd_gpu_last_error(void) { return g_err; }
+
+// ---------------------------------------------------- flux launch table
+#define DECL_FLUX(k)                                                         \
+  void ccfd_launch_flux_##k##_d(const OpArgs<double>&, int, hipStream_t);   \
+  void ccfd_launch_flux_##k##_f(const OpArgs<float>&, int, hipStream_t);
+DECL_FLUX(1) DECL_FLUX(2) DECL_FLUX(3) DECL_FLUX(4) DECL_FLUX(5) DECL_FLUX(6)
+DECL_FLUX(7) DECL_FLUX(8) DECL_FLUX(9) DECL_FLUX(10) DECL_FLUX(11)
+
+typedef void (*flux_fn_d)(const OpArgs<double>&, int, hipStream_t);
+typedef void (*flux_fn_f)(const OpArgs<float>&, int, hipStream_t);
+static const flux_fn_d kFluxD[12] = {
+    nullptr, ccfd_launch_flux_1_d, ccfd_launch_flux_2_d, ccfd_launch_flux_3_d,
+    ccfd_launch_flux_4_d, ccfd_launch_flux_5_d, ccfd_launch_flux_6_d,
+    ccfd_launch_flux_7_d, ccfd_launch_flux_8_d, ccfd_launch_flux_9_d,
+    ccfd_launch_flux_10_d, ccfd_launch_flux_11_d};
+static const flux_fn_f kFluxF[12] = {
+    nullptr, ccfd_launch_flux_1_f, ccfd_launch_flux_2_f, ccfd_launch_flux_3_f,
+    ccfd_launch_flux_4_f, ccfd_launch_flux_5_f, ccfd_launch_flux_6_f,
+    ccfd_launch_flux_7_f, ccfd_launch_flux_8_f, ccfd_launch_flux_9_f,
+    ccfd_launch_flux_10_f, ccfd_launch_flux_11_f};
+
+template <>
+void ccfd::launch_flux_code<double>(int code, const OpArgs<double>& a, int nb,
+                                    hipStream_t st) {
+  kFluxD[code](a, nb, st);
+}
+template <>
+void ccfd::launch_flux_code<float>(int code, const OpArgs<float>& a, int nb,
+                                   hipStream_t st) {
+  kFluxF[code](a, nb, st);
+}
+
+// --------------------------------------------------------------- kernels
+
+// prim (double, staging) -> prim_T and U = primCons(prim)
+template <typename T>
+__global__ void k_set_prim(int N, int Np, const double* __restrict__ pin,
+                           T* __restrict__ prim, double* __restrict__ U,
+                           Eq<double> q) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double p[4], c[4];
+  for (int v = 0; v < 4; ++v) p[v] = pin[v * Np + i];
+  prim_cons<double>(q, p, c);
+  for (int v = 0; v < 4; ++v) {
+    prim[v * Np + i] = T(p[v]);
+    U[v * Np + i] = c[v];
+  }
+}
+
+// U (double) -> prim_T = consPrim(U)
+template <typename T>
+__global__ void k_cons_to_prim(int N, int Np, const double* __restrict__ U,
+                               T* __restrict__ prim, Eq<double> q) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double c[4], p[4];
+  for (int v = 0; v < 4; ++v) c[v] = U[v * Np + i];
+  cons_prim<double>(q, c, p);
+  for (int v = 0; v < 4; ++v) prim[v * Np + i] = T(p[v]);
+}
+
+template <typename T>
+__global__ void k_to_double(long n, const T* __restrict__ src,
+                            double* __restrict__ dst) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = double(src[i]);
+}
+
+// calcTimeStep per-block minimum (src/timeDiscretization.c:249-307)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dt_partial(
+    int N, int Np, const T* __restrict__ prim, const double* __restrict__ area,
+    const double* __restrict__ sx, const double* __restrict__ sy, double gam,
+    double mu, double gamPrMax, double cfl, double dfl, int ts1d,
+    double* __restrict__ part /* [3][nb]: conv, visc, nonfinite */) {
+  __shared__ double sc[kBlock / 64], sv[kBlock / 64], sf[kBlock / 64];
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  double dc = 1e150, dv = 1e150, nf = 0.0;
+  if (i < N) {
+    double r = double(prim[i]), u = double(prim[Np + i]);
+    double v = double(prim[2 * Np + i]), p = double(prim[3 * Np + i]);
+    double a = sqrt(gam * p / r);
+    if (ts1d) {
+      dc = cfl * sy[i] / (fabs(u) + a);
+    } else {
+      double ssr = (fabs(u) + a) * sx[i] + (fabs(v) + a) * sy[i];
+      dc = cfl * area[i] / ssr;
+      if (mu > 1e-10) {
+        double ssv = gamPrMax * mu * sx[i] * sx[i] + gamPrMax * mu * sy[i] * sy[i];
+        dv = dfl * r * r * area[i] * area[i] / (4.0 * ssv);
+        if (!isfinite(dv)) nf = 1.0;
+      }
+    }
+    if (!isfinite(dc)) nf = 1.0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dc = fmin(dc, __shfl_down(dc, off, 64));
+    dv = fmin(dv, __shfl_down(dv, off, 64));
+    nf = fmax(nf, __shfl_down(nf, off, 64));
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sc[w] = dc; sv[w] = dv; sf[w] = nf; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; ++k) {
+      dc = fmin(dc, sc[k]); dv = fmin(dv, sv[k]); nf = fmax(nf, sf[k]);
+    }
+    dc = fmin(dc, sc[0]); dv = fmin(dv, sv[0]); nf = fmax(nf, sf[0]);
+    part[blockIdx.x] = dc;
+    part[gridDim.x + blockIdx.x] = dv;
+    part[2 * gridDim.x + blockIdx.x] = nf;
+  }
+}
+
+// final dt: global minimum, output / stop-time clamp; scal = {t, dt, viscDom,
+// nonfinite}. t is read from scal[0] when useDevT, else from tHost.
+__global__ void k_dt_final(int nb, const double* __restrict__ part,
+                           double* scal, int useDevT, double tHost,
+                           double pTime, double stopTime) {
+  __shared__ double sc[1024], sv[1024], sf[1024];
+  double dc = 1e150, dv = 1e150, nf = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    dc = fmin(dc, part[k]);
+    dv = fmin(dv, part[nb + k]);
+    nf = fmax(nf, part[2 * nb + k]);
+  }
+  sc[threadIdx.x] = dc; sv[threadIdx.x] = dv; sf[threadIdx.x] = nf;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sc[threadIdx.x] = fmin(sc[threadIdx.x], sc[threadIdx.x + s]);
+      sv[threadIdx.x] = fmin(sv[threadIdx.x], sv[threadIdx.x + s]);
+      sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double t = useDevT ? scal[0] : tHost;
+    double dt = fmin(sc[0], sv[0]);
+    double vd = sv[0] < sc[0] ? 1.0 : 0.0;
+    if ((t + dt > pTime) || (t + dt > stopTime))
+      dt = fmin(pTime, stopTime) - t;
+    else if ((t + 1.5 * dt > pTime) || (t + 1.5 * dt > stopTime))
+      dt = 0.5 * (fmin(pTime, stopTime) - t);
+    scal[0] = t;
+    scal[1] = dt;
+    scal[2] = vd;
+    scal[3] = fmax(scal[3], sf[0]);
+  }
+}
+
+// globalResidual final reduction (src/analyze.c:534-538); res -> scal[4..7]
+__global__ void k_res_final(int nb, const double* __restrict__ part,
+                            double totalArea_q, double* scal, int advanceT) {
+  __shared__ double s4[4][256];
+  for (int v = 0; v < 4; ++v) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) acc += part[v * nb + k];
+    s4[v][threadIdx.x] = acc;
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int v = 0; v < 4; ++v) s4[v][threadIdx.x] += s4[v][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int v = 0; v < 4; ++v) scal[4 + v] = sqrt(s4[v][0] * totalArea_q);
+    if (advanceT) scal[0] += scal[1];
+  }
+}
+
+__global__ void k_advance_t(double* scal) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[0] += scal[1];
+}
+
+template <typename T>
+__global__ void k_riemann(int code, long n, const double* __restrict__ L,
+                          const double* __restrict__ R, double* __restrict__ F,
+                          Eq<T> q) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T l[4], r[4], f[4];
+  for (int v = 0; v < 4; ++v) {
+    l[v] = T(L[4 * i + v]);
+    r[v] = T(R[4 * i + v]);
+  }
+  conv_flux<T>(q, code, l, r, f);
+  for (int v = 0; v < 4; ++v) F[4 * i + v] = double(f[v]);
+}
+
+// ------------------------------------------------------------ host helpers
+
+// Hilbert curve index of (x, y) on an n x n grid (n a power of two)
+static uint32_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
+  uint32_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s /= 2) {
+    uint32_t rx = (x & s) > 0;
+    uint32_t ry = (y & s) > 0;
+    d += s * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      uint32_t t = x; x = y; y = t;
+    }
+  }
+  return d;
+}
+
+void ccfd::sfc_order(const ccfd_mesh* m, std::vector<int>& perm) {
+  long N = m->nElems;
+  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+  for (long e = 0; e < N; ++e) {
+    x0 = std::min(x0, m->bary[2 * e]); x1 = std::max(x1, m->bary[2 * e]);
+    y0 = std::min(y0, m->bary[2 * e + 1]); y1 = std::max(y1, m->bary[2 * e + 1]);
+  }
+  double span = std::max(x1 - x0, y1 - y0);
+  if (!(span > 0)) span = 1.0;
+  std::vector<std::pair<uint32_t, int>> key(N);
+  for (long e = 0; e < N; ++e) {
+    uint32_t ix = (uint32_t)std::min(65535.0, (m->bary[2 * e] - x0) / span * 65535.0);
+    uint32_t iy = (uint32_t)std::min(65535.0, (m->bary[2 * e + 1] - y0) / span * 65535.0);
+    key[e] = {hilbert_index(65536u, ix, iy), (int)e};
+  }
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<uint32_t, int>& a,
+                      const std::pair<uint32_t, int>& b) { return a.first < b.first; });
+  perm.resize(N);
+  for (long k = 0; k < N; ++k) perm[k] = key[k].second;
+}
+
+// -------------------------------------------------------- context helpers
+
+void* ccfd::dmalloc(ccfd_gpu_ctx* c, size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
+  c->allocs.push_back(p);
+  hipMemsetAsync(p, 0, bytes ? bytes : 16, c->st);
+  return p;
+}
+
+static void free_all(ccfd_gpu_ctx* c) {
+  for (void* p : c->allocs) hipFree(p);
+  c->allocs.clear();
+}
+
+template <typename T>
+Geom<T>& geom(ccfd_gpu_ctx* c);
+template <>
+Geom<double>& geom<double>(ccfd_gpu_ctx* c) { return c->g64; }
+template <>
+Geom<float>& geom<float>(ccfd_gpu_ctx* c) { return c->g32; }
+
+template <typename T>
+static int upload_geom(ccfd_gpu_ctx* c, const std::vector<double>& w,
+                       const std::vector<double>& gp, const std::vector<double>& fn,
+                       const std::vector<double>& flen,
+                       const std::vector<double>& bbv,
+                       const std::vector<double>& bbd,
+                       const std::vector<double>& areaq,
+                       const std::vector<double>& venk) {
+  Geom<T>& g = geom<T>(c);
+  size_t sl = (size_t)c->S * c->Np;
+  auto up = [&](const std::vector<double>& src, T** dst) -> int {
+    std::vector<T> tmp(src.size());
+    for (size_t k = 0; k < src.size(); ++k) tmp[k] = T(src[k]);
+    *dst = (T*)dmalloc(c, tmp.size() * sizeof(T));
+    if (!*dst) return fail(CCFD_ENOMEM, "device allocation failed");
+    if (hipMemcpyAsync(*dst, tmp.data(), tmp.size() * sizeof(T),
+                       hipMemcpyHostToDevice, c->st) != hipSuccess)
+      return fail(CCFD_EHIP, "upload failed");
+    return hipStreamSynchronize(c->st) == hipSuccess ? 0 : CCFD_EHIP;
+  };
+  int rc;
+  if ((rc = up(w, &g.w)) || (rc = up(gp, &g.gp)) || (rc = up(fn, &g.fn)) ||
+      (rc = up(flen, &g.flen)) || (rc = up(bbv, &g.bbv)) ||
+      (rc = up(bbd, &g.bbd)) || (rc = up(areaq, &g.areaq)) ||
+      (rc = up(venk, &g.venk)))
+    return rc;
+  (void)sl;
+  return 0;
+}
+
+// ------------------------------------------------------------------ ABI
+
+extern "C" int ccfd_gpu_create(ccfd_gpu_ctx** out, int device) {
+  if (!out) return fail(CCFD_EINVAL, "null context pointer");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(CCFD_EHIP, "no HIP device available");
+  if (device < 0 || device >= n)
+    return fail(CCFD_EINVAL, "device %d out of range (%d devices)", device, n);
+  if (hipSetDevice(device) != hipSuccess)
+    return fail(CCFD_EHIP, "hipSetDevice(%d) failed", device);
+  ccfd_gpu_ctx* c = new ccfd_gpu_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(CCFD_EHIP, "stream creation failed");
+  }
+  if (hipHostMalloc((void**)&c->hscal, 64 * sizeof(double)) != hipSuccess) {
+    hipStreamDestroy(c->st);
+    delete c;
+    return fail(CCFD_EHIP, "pinned allocation failed");
+  }
+  *out = c;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_destroy(ccfd_gpu_ctx* c) {
+  if (!c) return CCFD_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->st);
+  ccfd::implicit_free(c);
+  free_all(c);
+  if (c->hscal) hipHostFree(c->hscal);
+  hipStreamDestroy(c->st);
+  delete c;
+  return CCFD_OK;
+}
+
+int ccfd::apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p) {
+  if (p->iFlux < 1 || p->iFlux > 11)
+    return fail(CCFD_EINVAL, "unknown flux function %d", p->iFlux);
+  if (p->spatialOrder != 1 && p->spatialOrder != 2)
+    return fail(CCFD_EINVAL, "spatial order must be 1 or 2");
+  if (p->spatialOrder == 2 && p->limiter != 1 && p->limiter != 2)
+    return fail(CCFD_EINVAL, "limiter must be 1 or 2");
+  if (p->precision != CCFD_FP64 && p->precision != CCFD_FP32_FLUX)
+    return fail(CCFD_EINVAL, "unknown precision %d", p->precision);
+  double rk[6] = {0};
+  if (p->nRKstages == 1 && p->timeOrder == 1) {
+    rk[1] = 1.0;
+  } else {
+    static const double t3[3][3] = {{0.1418, 0.4, 1.0},
+                                    {0.1918, 0.4929, 1.0},
+                                    {0.2884, 0.5010, 1.0}};
+    static const double t4[3][4] = {{0.0833, 0.2069, 0.4265, 1.0},
+                                    {0.1084, 0.2602, 0.5052, 1.0},
+                                    {0.1666, 0.3027, 0.5275, 1.0}};
+    static const double t5[3][5] = {{0.0533, 0.1263, 0.2375, 0.4414, 1.0},
+                                    {0.0695, 0.1602, 0.2898, 0.5060, 1.0},
+                                    {0.1067, 0.1979, 0.3232, 0.5201, 1.0}};
+    int o = p->timeOrder;
+    if (o < 1 || o > 3)
+      return fail(CCFD_EINVAL, "time order must be 1, 2 or 3");
+    if (p->nRKstages == 3) for (int k = 0; k < 3; ++k) rk[k + 1] = t3[o - 1][k];
+    else if (p->nRKstages == 4) for (int k = 0; k < 4; ++k) rk[k + 1] = t4[o - 1][k];
+    else if (p->nRKstages == 5) for (int k = 0; k < 5; ++k) rk[k + 1] = t5[o - 1][k];
+    else return fail(CCFD_EINVAL, "wrong number of RK stages %d", p->nRKstages);
+  }
+  c->prm = *p;
+  memcpy(c->rk, rk, sizeof(rk));
+  c->ex.refState[0][0] = p->refState[0][0];
+  for (int k = 0; k < 2; ++k)
+    for (int v = 0; v < 4; ++v) c->ex.refState[k][v] = p->refState[k][v];
+  c->ex.rp1Dinterface = p->rp1Dinterface;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_set_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p) {
+  if (!c || !p) return fail(CCFD_EINVAL, "null argument");
+  if (!c->hasMesh) return fail(CCFD_EINVAL, "no mesh uploaded");
+  int prevPrec = c->prm.precision;
+  int rc = apply_params(c, p);
+  if (rc) return rc;
+  if (prevPrec != p->precision) {
+    // re-derive the operator state in the new arithmetic from U
+    hipSetDevice(c->device);
+    return ccfd::refresh_prim(c);
+  }
+  // venk K may change: rebuild eps^2
+  return ccfd::upload_venk(c);
+}
+
+int ccfd::upload_venk(ccfd_gpu_ctx* c) {
+  std::vector<double> v(c->Np, 0.0);
+  for (int i = 0; i < c->N; ++i) {
+    double a = c->hArea[i];
+    double k = c->prm.venk_k;
+    v[i] = (k * sqrt(a)) * (k * sqrt(a)) * (k * sqrt(a));
+  }
+  std::vector<float> vf(v.begin(), v.end());
+  if (hipMemcpyAsync(c->g64.venk, v.data(), v.size() * 8, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipMemcpyAsync(c->g32.venk, vf.data(), vf.size() * 4, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "venk upload failed");
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
+                                 const ccfd_gpu_params* p) {
+  if (!c || !m || !p) return fail(CCFD_EINVAL, "null argument");
+  if (m->nElems <= 0) return fail(CCFD_EINVAL, "empty mesh");
+  if (m->nElems > (1L << 28))
+    return fail(CCFD_ENOTSUP, "mesh too large (%ld elements)", m->nElems);
+  if (m->nBC > kMaxBC)
+    return fail(CCFD_ENOTSUP, "at most %d boundary conditions", kMaxBC);
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->st);
+  ccfd::implicit_free(c);
+  free_all(c);
+  c->hasMesh = false;
+  int rc = apply_params(c, p);
+  if (rc) return rc;
+
+  const int N = (int)m->nElems;
+  int S = 0;
+  for (int e = 0; e < N; ++e) S = std::max(S, m->elemType[e]);
+  const int Np = ((N + kBlock - 1) / kBlock) * kBlock;
+  const int G = (int)m->nGhosts;
+  c->N = N; c->Np = Np; c->S = S; c->G = G;
+  c->nb = Np / kBlock;
+  c->totalArea_q = m->totalArea_q;
+  c->nSlotsHost = m->nSlots;
+
+  sfc_order(m, c->perm);
+  c->inv.assign(N, 0);
+  for (int k = 0; k < N; ++k) c->inv[c->perm[k]] = k;
+  c->slotStart.assign(m->slotStart, m->slotStart + N + 1);
+
+  // slot arrays [c][S][Np], element arrays [Np]
+  size_t sl = (size_t)S * Np;
+  std::vector<int> link(sl, kNoSide);
+  std::vector<double> w(2 * sl, 0.0), gp(2 * sl, 0.0), fn(2 * sl, 0.0),
+      flen(sl, 1.0), bbv(2 * sl, 0.0), bbd(sl, 1.0);
+  std::vector<double> areaq(Np, 0.0), area(Np, 0.0), sx(Np, 1.0), sy(Np, 1.0),
+      bary(2 * (size_t)Np, 0.0), venk(Np, 0.0);
+  c->hArea.assign(N, 0.0);
+  for (int i = 0; i < N; ++i) {
+    int e = c->perm[i];
+    areaq[i] = m->areaq[e];
+    area[i] = m->area[e];
+    sx[i] = m->sx[e];
+    sy[i] = m->sy[e];
+    bary[i] = m->bary[2 * e];
+    bary[Np + i] = m->bary[2 * e + 1];
+    c->hArea[i] = m->area[e];
+    long s0 = m->slotStart[e];
+    int ns = (int)(m->slotStart[e + 1] - s0);
+    for (int k = 0; k < ns; ++k) {
+      long s = s0 + k;
+      size_t o = (size_t)k * Np + i;
+      long nb = m->nbr[s];
+      if (nb >= 0) {
+        long ps = m->partner[s] - m->slotStart[nb];
+        link[o] = (c->inv[nb] << 2) | (int)ps;
+      } else {
+        link[o] = (int)nb;   // -(g + 1)
+      }
+      w[o] = m->w[2 * s];
+      w[sl + o] = m->w[2 * s + 1];
+      gp[o] = m->GP[2 * s];
+      gp[sl + o] = m->GP[2 * s + 1];
+      fn[o] = m->fn[2 * s];
+      fn[sl + o] = m->fn[2 * s + 1];
+      bool own = m->owner[s] || nb < 0;
+      flen[o] = own ? m->flen[s] : -m->flen[s];
+      bbv[o] = m->bbv[2 * s];
+      bbv[sl + o] = m->bbv[2 * s + 1];
+      bbd[o] = m->bbd[s];
+    }
+  }
+  // ghosts (host order)
+  std::vector<int> ghostBC(std::max(G, 1), 0);
+  std::vector<double> ghostBary(2 * (size_t)std::max(G, 1), 0.0);
+  for (int g = 0; g < G; ++g) {
+    ghostBC[g] = m->ghostBC[g];
+    ghostBary[g] = m->ghostBary[2 * g];
+    ghostBary[G + g] = m->ghostBary[2 * g + 1];
+  }
+  for (int b = 0; b < m->nBC; ++b) {
+    c->bct.bc[b].type = m->bc[b].type;
+    c->bct.bc[b].exactFunc = m->bc[b].exactFunc;
+    for (int v = 0; v < 4; ++v) c->bct.bc[b].pVar[v] = m->bc[b].pVar[v];
+  }
+  c->ex.xMin = m->xMin; c->ex.xMax = m->xMax;
+  c->ex.yMin = m->yMin; c->ex.yMax = m->yMax;
+  c->ex.dxRef = sqrt(1.0 / (m->totalArea_q * m->nElems));
+
+  // device allocations
+  auto upd = [&](const void* src, size_t bytes, void** dst) -> int {
+    *dst = dmalloc(c, bytes);
+    if (!*dst) return fail(CCFD_ENOMEM, "device allocation of %zu bytes failed", bytes);
+    if (hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->st) != hipSuccess)
+      return fail(CCFD_EHIP, "upload failed");
+    return hipStreamSynchronize(c->st) == hipSuccess ? 0
+                                                     : fail(CCFD_EHIP, "upload sync failed");
+  };
+  if ((rc = upd(link.data(), link.size() * 4, (void**)&c->link)) ||
+      (rc = upd(ghostBC.data(), ghostBC.size() * 4, (void**)&c->ghostBC)) ||
+      (rc = upd(ghostBary.data(), ghostBary.size() * 8, (void**)&c->ghostBary)) ||
+      (rc = upd(bary.data(), bary.size() * 8, (void**)&c->bary)) ||
+      (rc = upd(area.data(), area.size() * 8, (void**)&c->area)) ||
+      (rc = upd(sx.data(), sx.size() * 8, (void**)&c->sx)) ||
+      (rc = upd(sy.data(), sy.size() * 8, (void**)&c->sy)))
+    return rc;
+  if ((rc = upload_geom<double>(c, w, gp, fn, flen, bbv, bbd, areaq, venk)) ||
+      (rc = upload_geom<float>(c, w, gp, fn, flen, bbv, bbd, areaq, venk)))
+    return rc;
+
+  // state buffers (sized for double, used as T)
+  size_t e4 = 4 * (size_t)Np;
+  c->prim[0] = dmalloc(c, e4 * 8);
+  c->prim[1] = dmalloc(c, e4 * 8);
+  c->U = (double*)dmalloc(c, e4 * 8);
+  c->U0 = (double*)dmalloc(c, e4 * 8);
+  c->ut = dmalloc(c, e4 * 8);
+  c->ux = dmalloc(c, e4 * 8);
+  c->uy = dmalloc(c, e4 * 8);
+  c->side = dmalloc(c, 4 * sl * 8);
+  c->ghostP = dmalloc(c, 4 * (size_t)std::max(G, 1) * 8);
+  c->stage = (double*)dmalloc(c, std::max(e4, 4 * (size_t)S * Np) * 8);
+  c->part = (double*)dmalloc(c, 4 * (size_t)c->nb * 8 + 64);
+  c->scal = (double*)dmalloc(c, 64 * 8);
+  if (!c->prim[0] || !c->prim[1] || !c->U || !c->U0 || !c->ut || !c->ux ||
+      !c->uy || !c->side || !c->ghostP || !c->stage || !c->part || !c->scal)
+    return fail(CCFD_ENOMEM, "device state allocation failed");
+  c->cur = 0;
+  c->hasMesh = true;
+  if ((rc = upload_venk(c))) return rc;
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "set_mesh sync failed");
+  return CCFD_OK;
+}
+
+// host AoS (reference order) <-> device SoA (renumbered) through c->stage
+static int upload_aos(ccfd_gpu_ctx* c, const double* h) {
+  std::vector<double> tmp(4 * (size_t)c->Np, 0.0);
+  for (int i = 0; i < c->N; ++i) {
+    int e = c->perm[i];
+    for (int v = 0; v < 4; ++v) tmp[(size_t)v * c->Np + i] = h[4 * (size_t)e + v];
+  }
+  if (hipMemcpyAsync(c->stage, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "state upload failed");
+  return CCFD_OK;
+}
+
+static int download_aos(ccfd_gpu_ctx* c, const double* dsrc, double* h) {
+  std::vector<double> tmp(4 * (size_t)c->Np);
+  if (hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * 8, hipMemcpyDeviceToHost,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "state download failed: %s",
+                hipGetErrorString(hipGetLastError()));
+  for (int i = 0; i < c->N; ++i) {
+    int e = c->perm[i];
+    for (int v = 0; v < 4; ++v) h[4 * (size_t)e + v] = tmp[(size_t)v * c->Np + i];
+  }
+  return CCFD_OK;
+}
+
+// T-typed device array [4][Np] -> double staging
+static const double* as_double(ccfd_gpu_ctx* c, const void* src, size_t n) {
+  if (c->prm.precision == CCFD_FP64) return (const double*)src;
+  hipLaunchKernelGGL(k_to_double<float>, dim3((n + 255) / 256), dim3(256), 0,
+                     c->st, (long)n, (const float*)src, c->stage);
+  return c->stage;
+}
+
+Eq<double> ccfd::eqd(const ccfd_gpu_ctx* c) {
+  Eq<double> q;
+  q.gam = c->prm.gam;
+  q.gam1 = c->prm.gam - 1.0;
+  q.gam1q = 1.0 / q.gam1;
+  q.mu = c->prm.navierstokes ? c->prm.mu : 0.0;
+  q.Pr = c->prm.Pr;
+  return q;
+}
+
+int ccfd::refresh_prim(ccfd_gpu_ctx* c) {
+  dim3 g((c->N + 255) / 256), b(256);
+  if (c->prm.precision == CCFD_FP64)
+    hipLaunchKernelGGL(k_cons_to_prim<double>, g, b, 0, c->st, c->N, c->Np,
+                       c->U, (double*)c->prim[c->cur], eqd(c));
+  else
+    hipLaunchKernelGGL(k_cons_to_prim<float>, g, b, 0, c->st, c->N, c->Np,
+                       c->U, (float*)c->prim[c->cur], eqd(c));
+  return hipStreamSynchronize(c->st) == hipSuccess
+             ? CCFD_OK
+             : fail(CCFD_EHIP, "consPrim kernel failed");
+}
+
+extern "C" int ccfd_gpu_set_prim(ccfd_gpu_ctx* c, const double* prim) {
+  if (!c || !prim || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh / null state");
+  hipSetDevice(c->device);
+  int rc = upload_aos(c, prim);
+  if (rc) return rc;
+  dim3 g((c->N + 255) / 256), b(256);
+  if (c->prm.precision == CCFD_FP64)
+    hipLaunchKernelGGL(k_set_prim<double>, g, b, 0, c->st, c->N, c->Np,
+                       c->stage, (double*)c->prim[c->cur], c->U, eqd(c));
+  else
+    hipLaunchKernelGGL(k_set_prim<float>, g, b, 0, c->st, c->N, c->Np,
+                       c->stage, (float*)c->prim[c->cur], c->U, eqd(c));
+  return hipStreamSynchronize(c->st) == hipSuccess
+             ? CCFD_OK
+             : fail(CCFD_EHIP, "set_prim failed");
+}
+
+extern "C" int ccfd_gpu_set_cons(ccfd_gpu_ctx* c, const double* cons) {
+  if (!c || !cons || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh / null state");
+  hipSetDevice(c->device);
+  int rc = upload_aos(c, cons);
+  if (rc) return rc;
+  if (hipMemcpyAsync(c->U, c->stage, 4 * (size_t)c->Np * 8,
+                     hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "copy failed");
+  return refresh_prim(c);
+}
+
+extern "C" int ccfd_gpu_get_prim(ccfd_gpu_ctx* c, double* prim) {
+  if (!c || !prim || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh / null output");
+  hipSetDevice(c->device);
+  return download_aos(c, as_double(c, c->prim[c->cur], 4 * (size_t)c->Np), prim);
+}
+
+extern "C" int ccfd_gpu_get_cons(ccfd_gpu_ctx* c, double* cons) {
+  if (!c || !cons || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh / null output");
+  hipSetDevice(c->device);
+  return download_aos(c, c->U, cons);
+}
+
+extern "C" int ccfd_gpu_get_ut(ccfd_gpu_ctx* c, double* ut) {
+  if (!c || !ut || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh / null output");
+  hipSetDevice(c->device);
+  return download_aos(c, as_double(c, c->ut, 4 * (size_t)c->Np), ut);
+}
+
+extern "C" int ccfd_gpu_get_recon(ccfd_gpu_ctx* c, double* u_x, double* u_y,
+                                  double* sideP) {
+  if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
+  hipSetDevice(c->device);
+  int rc;
+  if (u_x && (rc = download_aos(c, as_double(c, c->ux, 4 * (size_t)c->Np), u_x))) return rc;
+  if (u_y && (rc = download_aos(c, as_double(c, c->uy, 4 * (size_t)c->Np), u_y))) return rc;
+  if (sideP) {
+    size_t sl = (size_t)c->S * c->Np;
+    std::vector<double> tmp(4 * sl);
+    const double* src = c->prm.spatialOrder == 2
+                            ? as_double(c, c->side, 4 * sl)
+                            : nullptr;
+    if (src) {
+      if (hipMemcpyAsync(tmp.data(), src, tmp.size() * 8, hipMemcpyDeviceToHost,
+                         c->st) != hipSuccess ||
+          hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(CCFD_EHIP, "side download failed");
+      for (int i = 0; i < c->N; ++i) {
+        int e = c->perm[i];
+        long s0 = c->slotStart[e], ns = c->slotStart[e + 1] - s0;
+        for (int k = 0; k < ns; ++k)
+          for (int v = 0; v < 4; ++v)
+            sideP[4 * (s0 + k) + v] = tmp[((size_t)v * c->S + k) * c->Np + i];
+      }
+    } else {
+      std::vector<double> pr(4 * (size_t)c->N);
+      if ((rc = ccfd_gpu_get_prim(c, pr.data()))) return rc;
+      for (int e = 0; e < c->N; ++e)
+        for (long s = c->slotStart[e]; s < c->slotStart[e + 1]; ++s)
+          for (int v = 0; v < 4; ++v) sideP[4 * s + v] = pr[4 * (size_t)e + v];
+    }
+  }
+  return CCFD_OK;
+}
+
+// ----------------------------------------------------------- operator
+
+template <typename T>
+OpArgs<T> ccfd::op_args(ccfd_gpu_ctx* c) {
+  OpArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  Geom<T>& g = geom<T>(c);
+  a.N = c->N; a.Np = c->Np; a.S = c->S; a.G = c->G;
+  a.order = c->prm.spatialOrder;
+  a.ns = c->prm.navierstokes ? 1 : 0;
+  a.link = c->link;
+  a.w = g.w; a.gp = g.gp; a.fn = g.fn; a.flen = g.flen;
+  a.bbv = g.bbv; a.bbd = g.bbd; a.areaq = g.areaq; a.venk = g.venk;
+  a.bary = c->bary; a.area = c->area;
+  a.ghostBC = c->ghostBC; a.ghostBary = c->ghostBary;
+  a.prim = (const T*)c->prim[c->cur];
+  a.side = (T*)c->side;
+  a.ux = (T*)c->ux; a.uy = (T*)c->uy;
+  a.ghostP = (T*)c->ghostP;
+  a.ut = (T*)c->ut;
+  a.write_ut = 1;
+  Eq<double> qd = eqd(c);
+  a.eq.gam = T(qd.gam); a.eq.gam1 = T(qd.gam1); a.eq.gam1q = T(qd.gam1q);
+  a.eq.mu = T(qd.mu); a.eq.Pr = T(qd.Pr);
+  a.ex = c->ex;
+  a.bct = c->bct;
+  return a;
+}
+template OpArgs<double> ccfd::op_args<double>(ccfd_gpu_ctx*);
+template OpArgs<float> ccfd::op_args<float>(ccfd_gpu_ctx*);
+
+template <typename T>
+void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
+  if (a.order == 2) {
+    int lim = c->prm.limiter;
+    // gradients are stored for the viscous flux and for host readback
+    if (lim == 1) {
+      if (a.ns) hipLaunchKernelGGL((recon_kernel<T, 1, 1>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
+      else hipLaunchKernelGGL((recon_kernel<T, 1, 0>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
+    } else {
+      if (a.ns) hipLaunchKernelGGL((recon_kernel<T, 2, 1>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
+      else hipLaunchKernelGGL((recon_kernel<T, 2, 0>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
+    }
+  }
+  launch_flux_code<T>(c->prm.iFlux, a, c->nb, c->st);
+}
+template void ccfd::launch_operator<double>(ccfd_gpu_ctx*, OpArgs<double>&);
+template void ccfd::launch_operator<float>(ccfd_gpu_ctx*, OpArgs<float>&);
+
+template <typename T>
+static int fvtd_t(ccfd_gpu_ctx* c, double time) {
+  OpArgs<T> a = op_args<T>(c);
+  a.time0 = time;
+  launch_operator<T>(c, a);
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_fvtd(ccfd_gpu_ctx* c, double time) {
+  if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
+  hipSetDevice(c->device);
+  int rc = c->prm.precision == CCFD_FP64 ? fvtd_t<double>(c, time)
+                                         : fvtd_t<float>(c, time);
+  if (rc) return rc;
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "operator failed: %s",
+                hipGetErrorString(hipGetLastError()));
+  return CCFD_OK;
+}
+
+// ------------------------------------------------------------ time step
+
+void ccfd::launch_dt(ccfd_gpu_ctx* c, int useDevT, double tHost, double pTime,
+                     double stopTime) {
+  double gam = c->prm.gam;
+  double mu = c->prm.navierstokes ? c->prm.mu : 0.0;
+  double gamPrMax = fmax(4.0 / 3.0, gam / c->prm.Pr);
+  if (c->prm.precision == CCFD_FP64)
+    hipLaunchKernelGGL(k_dt_partial<double>, dim3(c->nb), dim3(kBlock), 0, c->st,
+                       c->N, c->Np, (const double*)c->prim[c->cur], c->area,
+                       c->sx, c->sy, gam, mu, gamPrMax, c->prm.cfl, c->prm.dfl,
+                       c->prm.timeStep1D, c->part);
+  else
+    hipLaunchKernelGGL(k_dt_partial<float>, dim3(c->nb), dim3(kBlock), 0, c->st,
+                       c->N, c->Np, (const float*)c->prim[c->cur], c->area,
+                       c->sx, c->sy, gam, mu, gamPrMax, c->prm.cfl, c->prm.dfl,
+                       c->prm.timeStep1D, c->part);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, c->st, c->nb, c->part,
+                     c->scal, useDevT, tHost, pTime, stopTime);
+}
+
+extern "C" int ccfd_gpu_calc_dt(ccfd_gpu_ctx* c, double t, double pTime,
+                                double stopTime, double* dt, int* vd) {
+  if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
+  hipSetDevice(c->device);
+  hipMemsetAsync(c->scal + 3, 0, 8, c->st);
+  launch_dt(c, 0, t, pTime, stopTime);
+  if (hipMemcpyAsync(c->hscal, c->scal, 8 * 8, hipMemcpyDeviceToHost, c->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "dt reduction failed");
+  if (dt) *dt = c->hscal[1];
+  if (vd) *vd = c->hscal[2] != 0.0;
+  if (c->hscal[3] != 0.0)
+    return fail(CCFD_ENONFINITE, "convective or viscous time step NaN");
+  return CCFD_OK;
+}
+
+// one explicit step; dt from the host (dtdev == nullptr) or the device
+template <typename T>
+static void step_t(ccfd_gpu_ctx* c, double t, double dt, bool devDt, bool res) {
+  std::swap(c->U, c->U0);   // U0 <- current state
+  int nRK = c->prm.nRKstages;
+  bool euler = (nRK == 1 && c->prm.timeOrder == 1);
+  int nst = euler ? 1 : nRK;
+  for (int k = 1; k <= nst; ++k) {
+    OpArgs<T> a = op_args<T>(c);
+    if (devDt) {
+      a.tdev = c->scal;
+      a.tcoef = c->rk[k - 1];
+      a.dtdev = c->scal + 1;
+    } else {
+      a.time0 = t + c->rk[k - 1] * dt;
+      a.dt0 = dt;
+    }
+    a.rk = 1;
+    a.rkc = c->rk[k];
+    a.U0 = c->U0;
+    a.U = c->U;
+    a.prim_out = (T*)c->prim[1 - c->cur];
+    a.write_ut = 1;
+    a.res_part = (res && k == nst) ? c->part : nullptr;
+    if (!c->keepGradients && !a.ns) { a.ux = nullptr; a.uy = nullptr; }
+    launch_operator<T>(c, a);
+    c->cur = 1 - c->cur;
+  }
+  if (res)
+    hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->st, c->nb, c->part,
+                       c->totalArea_q, c->scal, devDt ? 1 : 0);
+  else if (devDt)
+    hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, c->st, c->scal);
+}
+
+extern "C" int ccfd_gpu_explicit_step(ccfd_gpu_ctx* c, double t, double dt,
+                                      double res[4]) {
+  if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
+  hipSetDevice(c->device);
+  if (c->prm.precision == CCFD_FP64) step_t<double>(c, t, dt, false, res != nullptr);
+  else step_t<float>(c, t, dt, false, res != nullptr);
+  if (res) {
+    if (hipMemcpyAsync(c->hscal, c->scal, 8 * 8, hipMemcpyDeviceToHost, c->st) !=
+        hipSuccess)
+      return fail(CCFD_EHIP, "residual copy failed");
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "explicit step failed: %s",
+                hipGetErrorString(hipGetLastError()));
+  if (res)
+    for (int v = 0; v < 4; ++v) res[v] = c->hscal[4 + v];
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_advance(ccfd_gpu_ctx* c, int nSteps, double* t,
+                                double pTime, double stopTime, double res[4]) {
+  if (!c || !c->hasMesh || !t) return fail(CCFD_EINVAL, "no mesh / null t");
+  if (nSteps < 0) return fail(CCFD_EINVAL, "negative step count");
+  hipSetDevice(c->device);
+  c->hscal[0] = *t;
+  hipMemcpyAsync(c->scal, c->hscal, 8, hipMemcpyHostToDevice, c->st);
+  hipMemsetAsync(c->scal + 3, 0, 8, c->st);
+  for (int s = 0; s < nSteps; ++s) {
+    launch_dt(c, 1, 0.0, pTime, stopTime);
+    bool r = res && s == nSteps - 1;
+    if (c->prm.precision == CCFD_FP64) step_t<double>(c, 0.0, 0.0, true, r);
+    else step_t<float>(c, 0.0, 0.0, true, r);
+  }
+  if (hipMemcpyAsync(c->hscal, c->scal, 8 * 8, hipMemcpyDeviceToHost, c->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "advance failed: %s",
+                hipGetErrorString(hipGetLastError()));
+  *t = c->hscal[0];
+  if (res && nSteps > 0)
+    for (int v = 0; v < 4; ++v) res[v] = c->hscal[4 + v];
+  if (c->hscal[3] != 0.0)
+    return fail(CCFD_ENONFINITE, "convective or viscous time step NaN");
+  return CCFD_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_res_partial(int N, int Np,
+                                                        const T* __restrict__ ut,
+                                                        const double* __restrict__ area,
+                                                        double* part) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < N) {
+    for (int v = 0; v < 4; ++v) {
+      double u = double(ut[v * Np + i]);
+      r[v] = area[i] * u * u;
+    }
+  }
+  block_sum4<T>(r, part, blockIdx.x, gridDim.x);
+}
+
+extern "C" int ccfd_gpu_residual(ccfd_gpu_ctx* c, double res[4]) {
+  if (!c || !c->hasMesh || !res) return fail(CCFD_EINVAL, "no mesh / null res");
+  hipSetDevice(c->device);
+  if (c->prm.precision == CCFD_FP64)
+    hipLaunchKernelGGL(k_res_partial<double>, dim3(c->nb), dim3(kBlock), 0, c->st,
+                       c->N, c->Np, (const double*)c->ut, c->area, c->part);
+  else
+    hipLaunchKernelGGL(k_res_partial<float>, dim3(c->nb), dim3(kBlock), 0, c->st,
+                       c->N, c->Np, (const float*)c->ut, c->area, c->part);
+  hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->st, c->nb, c->part,
+                     c->totalArea_q, c->scal, 0);
+  if (hipMemcpyAsync(c->hscal, c->scal, 8 * 8, hipMemcpyDeviceToHost, c->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "residual failed");
+  for (int v = 0; v < 4; ++v) res[v] = c->hscal[4 + v];
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_riemann_batch(int code, int precision, double gam,
+                                      long n, const double* L, const double* R,
+                                      double* F) {
+  if (code < 1 || code > 11) return fail(CCFD_EINVAL, "unknown flux %d", code);
+  if (n < 0 || (n > 0 && (!L || !R || !F))) return fail(CCFD_EINVAL, "bad arrays");
+  if (n == 0) return CCFD_OK;
+  double *dL = nullptr, *dR = nullptr, *dF = nullptr;
+  size_t b = 4 * (size_t)n * 8;
+  if (hipMalloc(&dL, b) != hipSuccess || hipMalloc(&dR, b) != hipSuccess ||
+      hipMalloc(&dF, b) != hipSuccess) {
+    hipFree(dL); hipFree(dR); hipFree(dF);
+    return fail(CCFD_ENOMEM, "riemann batch allocation failed");
+  }
+  hipMemcpy(dL, L, b, hipMemcpyHostToDevice);
+  hipMemcpy(dR, R, b, hipMemcpyHostToDevice);
+  dim3 g((n + 255) / 256), bl(256);
+  if (precision == CCFD_FP64) {
+    Eq<double> q = {gam, gam - 1.0, 1.0 / (gam - 1.0), 0.0, 0.72};
+    hipLaunchKernelGGL(k_riemann<double>, g, bl, 0, 0, code, n, dL, dR, dF, q);
+  } else {
+    Eq<float> q = {float(gam), float(gam - 1.0), float(1.0 / (gam - 1.0)), 0.f, 0.72f};
+    hipLaunchKernelGGL(k_riemann<float>, g, bl, 0, 0, code, n, dL, dR, dF, q);
+  }
+  hipError_t e = hipMemcpy(F, dF, b, hipMemcpyDeviceToHost);
+  hipFree(dL); hipFree(dR); hipFree(dF);
+  return e == hipSuccess ? CCFD_OK : fail(CCFD_EHIP, "riemann batch failed");
+}
+
+extern "C" int ccfd_gpu_synchronize(ccfd_gpu_ctx* c) {
+  if (!c) return fail(CCFD_EINVAL, "null context");
+  hipSetDevice(c->device);
+  return hipStreamSynchronize(c->st) == hipSuccess ? CCFD_OK
+                                                   : fail(CCFD_EHIP, "sync failed");
+}

@@ -1,0 +1,91 @@
+// ctx.hpp - the device context behind ccfd_gpu_ctx and the internal helpers
+// shared by ccfd_gpu.hip (explicit path) and implicit.hip (JFNK path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "ccfd_gpu.h"
+#include "operator.hpp"
+
+namespace ccfd {
+
+// geometry in the operator's arithmetic
+template <typename T>
+struct Geom {
+  T* w = nullptr;      // [2][S][Np] least-squares weights
+  T* gp = nullptr;     // [2][S][Np] side midpoint - barycentre
+  T* fn = nullptr;     // [2][S][Np] owner-side normal of the face
+  T* flen = nullptr;   // [S][Np] face length, negative if not the owner
+  T* bbv = nullptr;    // [2][S][Np] barycentre-to-barycentre vector
+  T* bbd = nullptr;    // [S][Np] its length
+  T* areaq = nullptr;  // [Np] 1 / area
+  T* venk = nullptr;   // [Np] Venkatakrishnan eps^2
+};
+
+struct ImplicitState;  // implicit.hip
+
+}  // namespace ccfd
+
+struct ccfd_gpu_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  bool hasMesh = false;
+  ccfd_gpu_params prm{};
+  double rk[6] = {0};
+  int N = 0, Np = 0, S = 0, G = 0, nb = 0;
+  long nSlotsHost = 0;
+  double totalArea_q = 0.0;
+  std::vector<int> perm, inv;       // new -> old, old -> new element
+  std::vector<long> slotStart;      // host slot CSR (reference order)
+  std::vector<double> hArea;        // renumbered element areas
+  // mesh
+  int* link = nullptr;
+  int* ghostBC = nullptr;
+  double* ghostBary = nullptr;
+  double* bary = nullptr;
+  double* area = nullptr;
+  double* sx = nullptr;
+  double* sy = nullptr;
+  ccfd::Geom<double> g64;
+  ccfd::Geom<float> g32;
+  ccfd::BCTable bct{};
+  ccfd::ExactCfg ex{};
+  // state (buffers sized for double, used as the operator type T)
+  void* prim[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* U = nullptr;
+  double* U0 = nullptr;
+  void* ut = nullptr;
+  void* ux = nullptr;
+  void* uy = nullptr;
+  void* side = nullptr;
+  void* ghostP = nullptr;
+  double* stage = nullptr;   // staging for host I/O
+  double* part = nullptr;    // per-block reduction partials
+  double* scal = nullptr;    // device scalars: t, dt, viscDom, nonfinite, res[4]
+  double* hscal = nullptr;   // pinned host mirror of scal
+  bool keepGradients = false;
+  std::vector<void*> allocs;
+  ccfd::ImplicitState* imp = nullptr;
+};
+
+namespace ccfd {
+
+int fail(int code, const char* fmt, ...);
+void* dmalloc(ccfd_gpu_ctx* c, size_t bytes);
+void sfc_order(const ccfd_mesh* m, std::vector<int>& perm);
+int apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p);
+int upload_venk(ccfd_gpu_ctx* c);
+int refresh_prim(ccfd_gpu_ctx* c);
+Eq<double> eqd(const ccfd_gpu_ctx* c);
+template <typename T>
+OpArgs<T> op_args(ccfd_gpu_ctx* c);
+template <typename T>
+void launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a);
+void launch_dt(ccfd_gpu_ctx* c, int useDevT, double tHost, double pTime,
+               double stopTime);
+void implicit_free(ccfd_gpu_ctx* c);
+
+}  // namespace ccfd

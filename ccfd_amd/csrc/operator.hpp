@@ -1,0 +1,371 @@
+// operator.hpp - the two kernels of one finite-volume operator evaluation
+// (ccfd fvTimeDerivative, src/finiteVolume.c:78-121) on the renumbered
+// structure-of-arrays mesh, plus their launch descriptors.
+//
+//   recon_kernel  (2nd order only): least-squares gradient of the primitive
+//                 state (src/reconstruction.c:252-275), ghost cell states
+//                 (setBCatBarys, src/boundary.c:320-331), Barth-Jespersen /
+//                 Venkatakrishnan limiting (src/reconstruction.c:25-192) and
+//                 side states p + GP.grad (src/reconstruction.c:293-311).
+//   flux_kernel   one thread per element evaluates the Riemann flux of each
+//                 of its sides in the owning side's orientation
+//                 (src/fluxCalculation.c:884-997, the partner takes -flux),
+//                 ghost side states (setBCatSides, src/boundary.c:300-314),
+//                 optional viscous flux, and sums them in the reference's
+//                 side order (src/finiteVolume.c:96-120): a deterministic
+//                 gather without atomics or a face-flux buffer. Optional
+//                 epilogue: the low-storage RK update
+//                 (src/timeDiscretization.c:380-397) and per-block partial
+//                 sums of the global residual (src/analyze.c:521-539).
+//
+// Layout: element arrays [var][Np]; side arrays [var][slot][Np] with slot in
+// the reference's per-element side order; "link" per slot encodes the
+// neighbour (element << 2 | partner slot), a ghost -(g + 1), or INT_MIN
+// (no side). flen < 0 marks a side that is not the face owner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "physics.hpp"
+
+namespace ccfd {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBC = 32;
+constexpr int kMaxSlots = 4;
+constexpr int kNoSide = INT_MIN;
+
+struct BCTable {
+  BCDev bc[kMaxBC];
+};
+
+template <typename T>
+struct OpArgs {
+  int N, Np, S, G;
+  int order, ns;
+  // mesh, slot arrays [c][S][Np] / [S][Np]
+  const int* __restrict__ link;
+  const T* __restrict__ w;
+  const T* __restrict__ gp;
+  const T* __restrict__ fn;
+  const T* __restrict__ flen;
+  const T* __restrict__ bbv;
+  const T* __restrict__ bbd;
+  // element arrays [Np]
+  const T* __restrict__ areaq;
+  const T* __restrict__ venk;
+  const double* __restrict__ bary;   // [2][Np]
+  const double* __restrict__ area;   // residual weight
+  // ghosts [G]
+  const int* __restrict__ ghostBC;
+  const double* __restrict__ ghostBary;  // [2][G]
+  // state
+  const T* __restrict__ prim;   // [4][Np] operator input
+  T* __restrict__ side;         // [4][S][Np]
+  T* __restrict__ ux;           // [4][Np]
+  T* __restrict__ uy;
+  T* __restrict__ ghostP;       // [4][G] ghost element states
+  T* __restrict__ ut;           // [4][Np]
+  // time: time = tdev ? tdev[0] + tcoef * tdev[1] : time0
+  double time0;
+  const double* tdev;
+  double tcoef;
+  // RK epilogue: U = U0 + (rkc * dt) * ut; prim_out = consPrim(U)
+  int rk;
+  double rkc;
+  const double* dtdev;   // device dt (if null use dt0)
+  double dt0;
+  const double* __restrict__ U0;
+  double* __restrict__ U;
+  T* __restrict__ prim_out;
+  int write_ut;
+  // residual partial sums [4][nblocks] of area * ut^2
+  double* res_part;
+  Eq<T> eq;
+  ExactCfg ex;
+  BCTable bct;
+};
+
+// XCD-aware block order: consecutive logical blocks (neighbouring cells
+// along the space-filling curve) go to the same XCD, which dispatches
+// hardware blocks round-robin over 8 XCDs, so neighbour data is shared in
+// one L2.
+__device__ inline int logical_block() {
+  int b = blockIdx.x, nb = gridDim.x;
+  int nb8 = (nb / 8) * 8;
+  if (b >= nb8) return b;
+  return (b % 8) * (nb8 / 8) + b / 8;
+}
+
+template <typename T>
+__device__ inline T op_time(const OpArgs<T>& a) {
+  return a.tdev ? T(a.tdev[0] + a.tcoef * a.tdev[1]) : T(a.time0);
+}
+
+// ------------------------------------------------------------ recon kernel
+template <typename T, int LIM, int NS>
+__global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
+  const int i = logical_block() * kBlock + threadIdx.x;
+  if (i >= a.N) return;
+  const int Np = a.Np, S = a.S;
+  T p[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) p[v] = a.prim[v * Np + i];
+  T q[kMaxSlots][4];
+  int lk[kMaxSlots];
+  T gx[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+  T gy[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+  T umin[4], umax[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) umin[v] = umax[v] = p[v];
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    lk[s] = s < S ? a.link[s * Np + i] : kNoSide;
+    if (lk[s] == kNoSide) continue;
+    if (lk[s] >= 0) {
+      int j = lk[s] >> 2;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) q[s][v] = a.prim[v * Np + j];
+    } else {
+      int g = -lk[s] - 1;
+      const BCDev& bc = a.bct.bc[a.ghostBC[g]];
+      T x[2] = {T(a.ghostBary[g]), T(a.ghostBary[a.G + g])};
+      bc_ghost<T>(a.eq, bc, a.ex, a.fn[s * Np + i], a.fn[(S + s) * Np + i],
+                  op_time(a), p, q[s], x);
+      if (NS) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a.ghostP[v * a.G + g] = q[s][v];
+      }
+    }
+    T wx = a.w[s * Np + i], wy = a.w[(S + s) * Np + i];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      T d = q[s][v] - p[v];
+      gx[v] += wx * d;
+      gy[v] += wy * d;
+      umin[v] = tmin(umin[v], q[s][v]);
+      umax[v] = tmax(umax[v], q[s][v]);
+    }
+  }
+  T dmin[4], dmax[4], dmin2[4], dmax2[4], phi[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    dmin[v] = umin[v] - p[v];
+    dmax[v] = umax[v] - p[v];
+    dmin2[v] = dmin[v] * dmin[v];
+    dmax2[v] = dmax[v] * dmax[v];
+    phi[v] = T(1.0);
+  }
+  const T eps = LIM == 2 ? a.venk[i] : T(0.0);
+  T GPx[kMaxSlots], GPy[kMaxSlots];
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    if (lk[s] == kNoSide) continue;
+    GPx[s] = a.gp[s * Np + i];
+    GPy[s] = a.gp[(S + s) * Np + i];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      T d = gx[v] * GPx[s] + gy[v] * GPy[s];
+      phi[v] = tmin(phi[v], limiter_phi<LIM, T>(d, dmax[v], dmin[v], dmax2[v],
+                                                dmin2[v], eps));
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    gx[v] *= phi[v];
+    gy[v] *= phi[v];
+  }
+  if (a.ux) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      a.ux[v * Np + i] = gx[v];
+      a.uy[v * Np + i] = gy[v];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    if (lk[s] == kNoSide) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      a.side[(v * S + s) * Np + i] = p[v] + GPx[s] * gx[v] + GPy[s] * gy[v];
+  }
+}
+
+// ------------------------------------------------------------- flux kernel
+template <typename T>
+__device__ inline void block_sum4(double v[4], double* res_part, int lb,
+                                  int nblocks) {
+  __shared__ double red[4][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double x = v[k];
+    // fixed butterfly order: deterministic for a given block
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[k][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
+    res_part[threadIdx.x * nblocks + lb] = s;
+  }
+}
+
+template <int CODE, typename T, int ORDER, int NS>
+__global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
+  const int lb = logical_block();
+  const int i = lb * kBlock + threadIdx.x;
+  const bool active = i < a.N;
+  const int Np = a.Np, S = a.S;
+  double r2[4] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    T acc[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+    T pc[4];
+    if (ORDER == 1 || NS) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) pc[v] = a.prim[v * Np + i];
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; ++s) {
+      const int lk = s < S ? a.link[s * Np + i] : kNoSide;
+      if (lk == kNoSide) continue;
+      T own[4], oth[4];
+      if (ORDER == 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) own[v] = pc[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) own[v] = a.side[(v * S + s) * Np + i];
+      }
+      const T nx = a.fn[s * Np + i], ny = a.fn[(S + s) * Np + i];
+      T fl = a.flen[s * Np + i];
+      const bool owner = fl > T(0.0);
+      const T len = owner ? fl : -fl;
+      int j = -1, g = -1;
+      if (lk >= 0) {
+        j = lk >> 2;
+        const int ps = lk & 3;
+        if (ORDER == 1) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) oth[v] = a.prim[v * Np + j];
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) oth[v] = a.side[(v * S + ps) * Np + j];
+        }
+      } else {
+        g = -lk - 1;
+        const BCDev& bc = a.bct.bc[a.ghostBC[g]];
+        T x[2] = {T(0.0), T(0.0)};
+        if (bc.type == 6) {
+          x[0] = a.gp[s * Np + i] + T(a.bary[i]);
+          x[1] = a.gp[(S + s) * Np + i] + T(a.bary[Np + i]);
+        }
+        bc_ghost<T>(a.eq, bc, a.ex, nx, ny, op_time(a), own, oth, x);
+      }
+      const T* L = owner ? own : oth;
+      const T* R = owner ? oth : own;
+      T Lr[4] = {L[0], nx * L[1] + ny * L[2], -ny * L[1] + nx * L[2], L[3]};
+      T Rr[4] = {R[0], nx * R[1] + ny * R[2], -ny * R[1] + nx * R[2], R[3]};
+      T F[4];
+      conv_flux_t<CODE, T>(a.eq, Lr, Rr, F);
+      T f[4];
+      f[0] = F[0];
+      f[1] = nx * F[1] - ny * F[2];
+      f[2] = ny * F[1] + nx * F[2];
+      f[3] = F[3];
+      if (NS) {
+        // owner element "a" and connection element "b" of the face
+        T pa_[4], pb_[4], uxa[4], uya[4], uxb[4], uyb[4];
+        const int ea = owner ? i : j;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          pa_[v] = owner ? pc[v] : a.prim[v * Np + ea];
+          uxa[v] = ORDER == 2 ? a.ux[v * Np + ea] : T(0.0);
+          uya[v] = ORDER == 2 ? a.uy[v * Np + ea] : T(0.0);
+        }
+        if (g >= 0) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            pb_[v] = a.ghostP[v * a.G + g];
+            uxb[v] = T(0.0);
+            uyb[v] = T(0.0);
+          }
+        } else {
+          const int eb = owner ? j : i;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            pb_[v] = owner ? a.prim[v * Np + eb] : pc[v];
+            uxb[v] = ORDER == 2 ? a.ux[v * Np + eb] : T(0.0);
+            uyb[v] = ORDER == 2 ? a.uy[v * Np + eb] : T(0.0);
+          }
+        }
+        T bvx = a.bbv[s * Np + i], bvy = a.bbv[(S + s) * Np + i];
+        if (!owner) {
+          bvx = -bvx;
+          bvy = -bvy;
+        }
+        const T bd = a.bbd[s * Np + i];
+        const T bb0 = bvx / bd, bb1 = bvy / bd;
+        T st[4], gxf[4], gyf[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          st[v] = T(0.5) * (R[v] + L[v]);
+          T gmx = T(0.5) * (uxa[v] + uxb[v]);
+          T gmy = T(0.5) * (uya[v] + uyb[v]);
+          T cr = gmx * bb0 + gmy * bb1 - (pb_[v] - pa_[v]) / bd;
+          gxf[v] = gmx - cr * bb0;
+          gyf[v] = gmy - cr * bb1;
+        }
+        T fx[4], fy[4];
+        diffusion_flux<T>(a.eq, st, gxf, gyf, fx, fy);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) f[v] -= (fx[v] * nx + fy[v] * ny);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        f[v] *= len;
+        if (!owner) f[v] = -f[v];
+        acc[v] += f[v];
+      }
+    }
+    const T aq = a.areaq[i];
+    T ut[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
+    if (a.write_ut) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
+    }
+    if (a.rk) {
+      const double dt = a.dtdev ? *a.dtdev : a.dt0;
+      const double c = a.rkc * dt;
+      double Uc[4], pn[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Uc[v] = a.U0[v * Np + i] + c * double(ut[v]);
+        a.U[v * Np + i] = Uc[v];
+      }
+      Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
+                       double(a.eq.mu), double(a.eq.Pr)};
+      cons_prim<double>(qd, Uc, pn);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
+    }
+    if (a.res_part) {
+      const double ar = a.area[i];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
+    }
+  }
+  if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
+}
+
+// launch entry of one flux code (defined in op_flux.hip, one object per code)
+template <typename T>
+void launch_flux_code(int code, const OpArgs<T>& a, int nblocks,
+                      hipStream_t st);
+
+}  // namespace ccfd

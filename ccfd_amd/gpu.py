@@ -1,0 +1,210 @@
+"""Python front of the C ABI in include/ccfd_gpu.h (libccfd_gpu.so).
+
+`Solver` owns one device context: upload a `Mesh` and parameters, set the
+state, evaluate fvTimeDerivative, step in time. Errors raise `GpuError`
+with the library's message; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+FP64, FP32_FLUX = 0, 1
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """struct ccfd_gpu_params"""
+    _fields_ = [
+        ("gam", C.c_double), ("mu", C.c_double), ("Pr", C.c_double),
+        ("navierstokes", C.c_int), ("iFlux", C.c_int),
+        ("spatialOrder", C.c_int), ("limiter", C.c_int),
+        ("venk_k", C.c_double), ("refState", (C.c_double * 4) * 2),
+        ("rp1Dinterface", C.c_double), ("cfl", C.c_double),
+        ("dfl", C.c_double), ("timeStep1D", C.c_int),
+        ("nRKstages", C.c_int), ("timeOrder", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class ImplicitCfg(C.Structure):
+    """struct ccfd_gpu_implicit_cfg"""
+    _fields_ = [("nKdim", C.c_int), ("nNewtonIter", C.c_int),
+                ("precond", C.c_int), ("epsNewton", C.c_double),
+                ("epsGMRES", C.c_double), ("gammaEW", C.c_double)]
+
+
+def params(gamma=1.4, mu=0.0, Pr=0.72, navierstokes=0, flux=5, order=2,
+           limiter=1, venk_k=1.0, left=(1, 0, 0, 1), right=(0.125, 0, 0, 0.1),
+           x0=0.5, cfl=0.9, dfl=0.9, ts1d=0, nrk=1, torder=1,
+           precision=FP64):
+    p = Params()
+    p.gam, p.mu, p.Pr, p.navierstokes = gamma, mu, Pr, navierstokes
+    p.iFlux, p.spatialOrder, p.limiter, p.venk_k = flux, order, limiter, venk_k
+    for i in range(4):
+        p.refState[0][i] = left[i]
+        p.refState[1][i] = right[i]
+    p.rp1Dinterface, p.cfl, p.dfl, p.timeStep1D = x0, cfl, dfl, ts1d
+    p.nRKstages, p.timeOrder, p.precision = nrk, torder, precision
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    L = N._load("libccfd_gpu.so")
+    vp, dp, ip = C.c_void_p, N.c_double_p, N.c_int_p
+    sig = {
+        "ccfd_gpu_last_error": (C.c_char_p, []),
+        "ccfd_gpu_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "ccfd_gpu_destroy": (C.c_int, [vp]),
+        "ccfd_gpu_set_mesh": (C.c_int, [vp, C.c_void_p, C.POINTER(Params)]),
+        "ccfd_gpu_set_params": (C.c_int, [vp, C.POINTER(Params)]),
+        "ccfd_gpu_set_prim": (C.c_int, [vp, dp]),
+        "ccfd_gpu_set_cons": (C.c_int, [vp, dp]),
+        "ccfd_gpu_get_prim": (C.c_int, [vp, dp]),
+        "ccfd_gpu_get_cons": (C.c_int, [vp, dp]),
+        "ccfd_gpu_get_ut": (C.c_int, [vp, dp]),
+        "ccfd_gpu_get_recon": (C.c_int, [vp, dp, dp, dp]),
+        "ccfd_gpu_fvtd": (C.c_int, [vp, C.c_double]),
+        "ccfd_gpu_calc_dt": (C.c_int, [vp, C.c_double, C.c_double, C.c_double,
+                                       dp, ip]),
+        "ccfd_gpu_explicit_step": (C.c_int, [vp, C.c_double, C.c_double, dp]),
+        "ccfd_gpu_advance": (C.c_int, [vp, C.c_int, dp, C.c_double,
+                                       C.c_double, dp]),
+        "ccfd_gpu_residual": (C.c_int, [vp, dp]),
+        "ccfd_gpu_implicit_step": (C.c_int, [vp, C.POINTER(ImplicitCfg),
+                                             C.c_double, C.c_double, dp, ip,
+                                             ip]),
+        "ccfd_gpu_riemann_batch": (C.c_int, [C.c_int, C.c_int, C.c_double,
+                                             C.c_long, dp, dp, dp]),
+        "ccfd_gpu_synchronize": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().ccfd_gpu_last_error()
+        raise GpuError("ccfd_gpu error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def riemann_batch(code, L, R, gamma=1.4, precision=FP64):
+    """convectiveFlux on n rotated state pairs, evaluated on the GPU."""
+    L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 4)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 4)
+    F = np.empty_like(L)
+    check(lib().ccfd_gpu_riemann_batch(code, precision, gamma, len(L),
+                                       N.dptr(L), N.dptr(R), N.dptr(F)))
+    return F
+
+
+class Solver:
+    """One device context with a resident mesh and state."""
+
+    def __init__(self, mesh, prm, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.ccfd_gpu_create(C.byref(h), device))
+        self.h = h
+        self.mesh = mesh
+        self.prm = prm
+        check(self.L.ccfd_gpu_set_mesh(self.h, C.cast(mesh.ptr, C.c_void_p),
+                                       C.byref(prm)))
+        self.N = mesh.n_elems
+        self.S = mesh.n_slots
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ccfd_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, prm):
+        self.prm = prm
+        check(self.L.ccfd_gpu_set_params(self.h, C.byref(prm)))
+
+    def set_prim(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (self.N, 4)
+        check(self.L.ccfd_gpu_set_prim(self.h, N.dptr(prim)))
+
+    def set_cons(self, cons):
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        check(self.L.ccfd_gpu_set_cons(self.h, N.dptr(cons)))
+
+    def _get(self, fn):
+        out = np.empty((self.N, 4))
+        check(fn(self.h, N.dptr(out)))
+        return out
+
+    def prim(self):
+        return self._get(self.L.ccfd_gpu_get_prim)
+
+    def cons(self):
+        return self._get(self.L.ccfd_gpu_get_cons)
+
+    def ut(self):
+        return self._get(self.L.ccfd_gpu_get_ut)
+
+    def recon(self):
+        ux = np.empty((self.N, 4))
+        uy = np.empty((self.N, 4))
+        sp = np.empty((self.S, 4))
+        check(self.L.ccfd_gpu_get_recon(self.h, N.dptr(ux), N.dptr(uy),
+                                        N.dptr(sp)))
+        return ux, uy, sp
+
+    def fvtd(self, time=0.0):
+        check(self.L.ccfd_gpu_fvtd(self.h, time))
+
+    def calc_dt(self, t, ptime, tend):
+        dt = C.c_double()
+        vd = C.c_int()
+        check(self.L.ccfd_gpu_calc_dt(self.h, t, ptime, tend, C.byref(dt),
+                                      C.byref(vd)))
+        return dt.value, bool(vd.value)
+
+    def explicit_step(self, t, dt, residual=True):
+        r = np.zeros(4)
+        check(self.L.ccfd_gpu_explicit_step(self.h, t, dt,
+                                            N.dptr(r) if residual else None))
+        return r
+
+    def advance(self, nsteps, t, ptime, tend, residual=True):
+        tt = C.c_double(t)
+        r = np.zeros(4)
+        check(self.L.ccfd_gpu_advance(self.h, nsteps, C.byref(tt), ptime, tend,
+                                      N.dptr(r) if residual else None))
+        return tt.value, r
+
+    def residual(self):
+        r = np.zeros(4)
+        check(self.L.ccfd_gpu_residual(self.h, N.dptr(r)))
+        return r
+
+    def implicit_step(self, cfg, t, dt):
+        r = np.zeros(4)
+        nn, ng = C.c_int(0), C.c_int(0)
+        check(self.L.ccfd_gpu_implicit_step(self.h, C.byref(cfg), t, dt,
+                                            N.dptr(r), C.byref(nn),
+                                            C.byref(ng)))
+        return r, nn.value, ng.value
+
+    def sync(self):
+        check(self.L.ccfd_gpu_synchronize(self.h))

@@ -14,6 +14,7 @@ boundary edges) for the workloads of this path:
   channel_bump     - channel with a circular-arc bump on the lower wall
 """
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -43,11 +44,12 @@ class BC:
         if pvar is not None:
             self.pvar = list(pvar)
         elif self.type in (INFLOW, CHARACTERISTIC):
-            pi = np.arccos(-1.0)
-            c = np.sqrt(gamma * pressure / rho)
+            # same libm as the reference (src/boundary.c:83-96)
+            pi = math.acos(-1.0)
+            c = math.sqrt(gamma * pressure / rho)
             v = mach * c
-            self.pvar = [rho, v * np.cos(alpha * pi / 180.0),
-                         v * np.sin(alpha * pi / 180.0), pressure]
+            self.pvar = [rho, v * math.cos(alpha * pi / 180.0),
+                         v * math.sin(alpha * pi / 180.0), pressure]
         elif self.type == PRESSURE_OUT:
             self.pvar = [0.0, 0.0, 0.0, pressure]
 

@@ -1,0 +1,163 @@
+/*
+ * ccfd_gpu.h - C ABI of the MI355X (gfx950) implementation of ccfd's
+ * finite-volume operator and the time / linear-solver loops around it.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Host arrays are in the reference's element order (as built by
+ * ccfd_mesh_build), 4 doubles per element: primitive (rho, vx, vy, p) or
+ * conservative (rho, mx, my, E). The device keeps its own renumbered
+ * (space-filling-curve) structure-of-arrays copy; state stays resident
+ * between calls and is only copied back on request.
+ *
+ * Every entry point returns CCFD_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is ccfd_gpu_last_error().
+ * A context is used by one host thread at a time.
+ */
+#ifndef CCFD_GPU_H
+#define CCFD_GPU_H
+
+#include "ccfd_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ccfd_status {
+	CCFD_OK = 0,
+	CCFD_EINVAL = -1,       /* bad argument / state */
+	CCFD_ENOMEM = -2,       /* device allocation failed */
+	CCFD_ENOTSUP = -3,      /* unsupported option */
+	CCFD_EHIP = -4,         /* HIP runtime error */
+	CCFD_ENONFINITE = -5,   /* non-finite time step (src/timeDiscretization.c:259-262) */
+	CCFD_ENOCONV = -6,      /* Newton not converged (src/timeDiscretization.c:544-549) */
+	CCFD_ESINGULAR = -7     /* singular LU-SGS diagonal block (src/linearSolver.c:245-249) */
+};
+
+/* arithmetic of the operator */
+enum ccfd_precision {
+	CCFD_FP64 = 0,          /* bit-identical drop-in */
+	CCFD_FP32_FLUX = 1      /* FP32 reconstruction / flux / residual and
+	                           preconditioner, FP64 state update, FP64
+	                           Frechet difference and Arnoldi */
+};
+
+/*
+ * Physics and discretisation settings; meaning and defaults follow the
+ * reference's ini keys (initEquation src/equation.c:40-124, initFV
+ * src/finiteVolume.c:30-66, initTimeDisc src/timeDiscretization.c:69-240).
+ */
+typedef struct ccfd_gpu_params {
+	double gam;             /* gamma */
+	double mu;              /* dynamic viscosity (Navier-Stokes only) */
+	double Pr;              /* Prandtl number */
+	int navierstokes;       /* 1: -Dnavierstokes semantics (viscous flux, WALL BC) */
+	int iFlux;              /* flux function 1..11, src/main.h:76-88 */
+	int spatialOrder;       /* 1 or 2 */
+	int limiter;            /* 1 Barth-Jespersen, 2 Venkatakrishnan */
+	double venk_k;          /* Venkatakrishnan K */
+	double refState[2][4];  /* exact function 5 (1-D Riemann problem) */
+	double rp1Dinterface;
+	double cfl, dfl;        /* convective / diffusive CFL */
+	int timeStep1D;         /* 1-D time step (src/timeDiscretization.c:253-271) */
+	int nRKstages;          /* 1, 3, 4, 5 */
+	int timeOrder;          /* 1, 2, 3 */
+	int precision;          /* ccfd_precision */
+} ccfd_gpu_params;
+
+typedef struct ccfd_gpu_ctx ccfd_gpu_ctx;
+
+/* last error message of the calling thread */
+const char *ccfd_gpu_last_error(void);
+
+/* create a context on a HIP device (one stream per context) */
+int ccfd_gpu_create(ccfd_gpu_ctx **ctx, int device);
+int ccfd_gpu_destroy(ccfd_gpu_ctx *ctx);
+
+/*
+ * Upload a preprocessed mesh with its parameters (the device side of
+ * initMesh + initFV + initTimeDisc, src/main.c:101-108). Renumbers the
+ * elements along a Hilbert curve; host I/O stays in reference order.
+ */
+int ccfd_gpu_set_mesh(ccfd_gpu_ctx *ctx, const ccfd_mesh *mesh,
+		const ccfd_gpu_params *params);
+
+/* Change parameters that do not alter the mesh (flux, order, limiter, ...) */
+int ccfd_gpu_set_params(ccfd_gpu_ctx *ctx, const ccfd_gpu_params *params);
+
+/* ---- element state, reference order, 4 doubles per element ---- */
+/* primitive state; also sets the conservative state (primCons,
+ * setInitialCondition src/initialCondition.c:218-222) */
+int ccfd_gpu_set_prim(ccfd_gpu_ctx *ctx, const double *prim);
+/* conservative state; primitive from consPrim (src/equationOfState.c:37-52) */
+int ccfd_gpu_set_cons(ccfd_gpu_ctx *ctx, const double *cons);
+int ccfd_gpu_get_prim(ccfd_gpu_ctx *ctx, double *prim);
+int ccfd_gpu_get_cons(ccfd_gpu_ctx *ctx, double *cons);
+/* dU/dt of the last operator evaluation (elem->u_t) */
+int ccfd_gpu_get_ut(ccfd_gpu_ctx *ctx, double *ut);
+/*
+ * Side effects of the last operator evaluation that host consumers read
+ * (src/analyze.c:314-468): limited gradients u_x, u_y (4 per element) and
+ * reconstructed side states (4 per slot, ccfd_mesh slot order). Any pointer
+ * may be NULL. At 1st order gradients are zero and side states equal the
+ * element state.
+ */
+int ccfd_gpu_get_recon(ccfd_gpu_ctx *ctx, double *u_x, double *u_y,
+		double *sideP);
+
+/* ---- operator ---- */
+/* fvTimeDerivative (src/finiteVolume.c:78-121) at the given time on the
+ * resident primitive state; dU/dt stays on the device */
+int ccfd_gpu_fvtd(ccfd_gpu_ctx *ctx, double time);
+
+/* ---- time stepping (src/timeDiscretization.c) ---- */
+/* calcTimeStep (:249-329): global CFL time step with the output / stop
+ * clamp; returns CCFD_ENONFINITE if any element gives a non-finite step */
+int ccfd_gpu_calc_dt(ccfd_gpu_ctx *ctx, double t, double pTime,
+		double stopTime, double *dt, int *viscousDominates);
+/* one explicit Euler / low-storage RK step (:337-401); res (may be NULL)
+ * receives globalResidual (src/analyze.c:521-539) */
+int ccfd_gpu_explicit_step(ccfd_gpu_ctx *ctx, double t, double dt,
+		double res[4]);
+/*
+ * nSteps explicit steps with the time step computed and kept on the device
+ * (calcTimeStep + step, no host round trip per step). *t is advanced; res
+ * (may be NULL) gets the last step's residual.
+ */
+int ccfd_gpu_advance(ccfd_gpu_ctx *ctx, int nSteps, double *t, double pTime,
+		double stopTime, double res[4]);
+/* globalResidual of the current dU/dt */
+int ccfd_gpu_residual(ccfd_gpu_ctx *ctx, double res[4]);
+
+/* ---- implicit backward Euler, JFNK-GMRES with optional LU-SGS ---- */
+typedef struct ccfd_gpu_implicit_cfg {
+	int nKdim;              /* Krylov dimension (single cycle, as GMRES_M) */
+	int nNewtonIter;        /* Newton iteration limit */
+	int precond;            /* 1: LU-SGS preconditioner */
+	double epsNewton;       /* relative Newton tolerance */
+	double epsGMRES;        /* relative GMRES tolerance */
+	double gammaEW;         /* Eisenstat-Walker gamma */
+} ccfd_gpu_implicit_cfg;
+
+/* implicitTimeStep (src/timeDiscretization.c:412-552) with GMRES_M,
+ * matrixVector, buildMatrix and LUSGS (src/linearSolver.c:136-510) */
+int ccfd_gpu_implicit_step(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
+		double t, double dt, double res[4], int *nNewton, int *nGMRES);
+
+/* ---- flux functions in isolation ---- */
+/*
+ * convectiveFlux (src/fluxCalculation.c:803-844) on n rotated state pairs
+ * (host arrays, 4 doubles per state: rho, vn, vt, p) on the device; code is
+ * the reference's flux id 1..11.
+ */
+int ccfd_gpu_riemann_batch(int code, int precision, double gam, long n,
+		const double *L, const double *R, double *F);
+
+/* ---- timing helper ---- */
+/* wait for all work queued on the context's stream */
+int ccfd_gpu_synchronize(ccfd_gpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
