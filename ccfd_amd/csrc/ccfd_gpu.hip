@@ -348,6 +348,7 @@ extern "C" int ccfd_gpu_destroy(ccfd_gpu_ctx* c) {
   hipStreamSynchronize(c->st);
   ccfd::implicit_free(c);
   free_all(c);
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->hscal) hipHostFree(c->hscal);
   hipStreamDestroy(c->st);
   delete c;
@@ -500,8 +501,14 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   }
   // ghosts (host order)
   std::vector<int> ghostBC(std::max(G, 1), 0);
+  std::vector<int> ghostCell(std::max(G, 1), 0), ghostK(std::max(G, 1), 0);
   std::vector<double> ghostBary(2 * (size_t)std::max(G, 1), 0.0);
   for (int g = 0; g < G; ++g) {
+    long s = m->ghostSlot[g];
+    long e = std::upper_bound(m->slotStart, m->slotStart + N + 1, s) -
+             m->slotStart - 1;
+    ghostCell[g] = c->inv[e];
+    ghostK[g] = (int)(s - m->slotStart[e]);
     ghostBC[g] = m->ghostBC[g];
     ghostBary[g] = m->ghostBary[2 * g];
     ghostBary[G + g] = m->ghostBary[2 * g + 1];
@@ -526,6 +533,8 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   };
   if ((rc = upd(link.data(), link.size() * 4, (void**)&c->link)) ||
       (rc = upd(ghostBC.data(), ghostBC.size() * 4, (void**)&c->ghostBC)) ||
+      (rc = upd(ghostCell.data(), ghostCell.size() * 4, (void**)&c->ghostCell)) ||
+      (rc = upd(ghostK.data(), ghostK.size() * 4, (void**)&c->ghostK)) ||
       (rc = upd(ghostBary.data(), ghostBary.size() * 8, (void**)&c->ghostBary)) ||
       (rc = upd(bary.data(), bary.size() * 8, (void**)&c->bary)) ||
       (rc = upd(area.data(), area.size() * 8, (void**)&c->area)) ||
@@ -547,11 +556,12 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   c->uy = dmalloc(c, e4 * 8);
   c->side = dmalloc(c, 4 * sl * 8);
   c->ghostP = dmalloc(c, 4 * (size_t)std::max(G, 1) * 8);
+  c->ghostS = dmalloc(c, 4 * (size_t)std::max(G, 1) * 8);
   c->stage = (double*)dmalloc(c, std::max(e4, 4 * (size_t)S * Np) * 8);
   c->part = (double*)dmalloc(c, 4 * (size_t)c->nb * 8 + 64);
   c->scal = (double*)dmalloc(c, 64 * 8);
   if (!c->prim[0] || !c->prim[1] || !c->U || !c->U0 || !c->ut || !c->ux ||
-      !c->uy || !c->side || !c->ghostP || !c->stage || !c->part || !c->scal)
+      !c->uy || !c->side || !c->ghostP || !c->ghostS || !c->stage || !c->part || !c->scal)
     return fail(CCFD_ENOMEM, "device state allocation failed");
   c->cur = 0;
   c->hasMesh = true;
@@ -717,6 +727,8 @@ OpArgs<T> ccfd::op_args(ccfd_gpu_ctx* c) {
   a.bbv = g.bbv; a.bbd = g.bbd; a.areaq = g.areaq; a.venk = g.venk;
   a.bary = c->bary; a.area = c->area;
   a.ghostBC = c->ghostBC; a.ghostBary = c->ghostBary;
+  a.ghostCell = c->ghostCell; a.ghostK = c->ghostK;
+  a.ghostS = (T*)c->ghostS;
   a.prim = (const T*)c->prim[c->cur];
   a.side = (T*)c->side;
   a.ux = (T*)c->ux; a.uy = (T*)c->uy;
@@ -733,20 +745,56 @@ OpArgs<T> ccfd::op_args(ccfd_gpu_ctx* c) {
 template OpArgs<double> ccfd::op_args<double>(ccfd_gpu_ctx*);
 template OpArgs<float> ccfd::op_args<float>(ccfd_gpu_ctx*);
 
+static hipEvent_t take_event(ccfd_gpu_ctx* c) {
+  if (c->evUsed == c->evpool.size()) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    c->evpool.push_back(e);
+  }
+  return c->evpool[c->evUsed++];
+}
+
+void ccfd::tspan_begin(ccfd_gpu_ctx* c, hipEvent_t* e) {
+  if (!c->timing) return;
+  *e = take_event(c);
+  hipEventRecord(*e, c->st);
+}
+
+void ccfd::tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind) {
+  if (!c->timing) return;
+  hipEvent_t b = take_event(c);
+  hipEventRecord(b, c->st);
+  c->spans.push_back({a, b, kind});
+}
+
 template <typename T>
 void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (a.order == 2) {
-    int lim = c->prm.limiter;
-    // gradients are stored for the viscous flux and for host readback
-    if (lim == 1) {
-      if (a.ns) hipLaunchKernelGGL((recon_kernel<T, 1, 1>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
-      else hipLaunchKernelGGL((recon_kernel<T, 1, 0>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
-    } else {
-      if (a.ns) hipLaunchKernelGGL((recon_kernel<T, 2, 1>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
-      else hipLaunchKernelGGL((recon_kernel<T, 2, 0>), dim3(c->nb), dim3(kBlock), 0, c->st, a);
+    const int lim = c->prm.limiter;
+    dim3 g(c->nb), b(kBlock);
+    if (c->G > 0) {
+      hipEvent_t eb = nullptr;
+      tspan_begin(c, &eb);
+      hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
+                         0, c->st, a);
+      tspan_end(c, eb, KT_OTHER);
     }
+    tspan_begin(c, &e0);
+    if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1>), g, b, 0, c->st, a);
+    else hipLaunchKernelGGL((recon_kernel<T, 2>), g, b, 0, c->st, a);
+    tspan_end(c, e0, KT_RECON);
   }
+  if (c->G > 0) {
+    hipEvent_t es = nullptr;
+    tspan_begin(c, &es);
+    hipLaunchKernelGGL(bc_side_kernel<T>, dim3((c->G + 127) / 128), dim3(128), 0,
+                       c->st, a);
+    tspan_end(c, es, KT_OTHER);
+  }
+  tspan_begin(c, &e1);
   launch_flux_code<T>(c->prm.iFlux, a, c->nb, c->st);
+  tspan_end(c, e1, KT_FLUX);
 }
 template void ccfd::launch_operator<double>(ccfd_gpu_ctx*, OpArgs<double>&);
 template void ccfd::launch_operator<float>(ccfd_gpu_ctx*, OpArgs<float>&);
@@ -775,6 +823,8 @@ extern "C" int ccfd_gpu_fvtd(ccfd_gpu_ctx* c, double time) {
 
 void ccfd::launch_dt(ccfd_gpu_ctx* c, int useDevT, double tHost, double pTime,
                      double stopTime) {
+  hipEvent_t e0 = nullptr;
+  tspan_begin(c, &e0);
   double gam = c->prm.gam;
   double mu = c->prm.navierstokes ? c->prm.mu : 0.0;
   double gamPrMax = fmax(4.0 / 3.0, gam / c->prm.Pr);
@@ -790,6 +840,7 @@ void ccfd::launch_dt(ccfd_gpu_ctx* c, int useDevT, double tHost, double pTime,
                        c->prm.timeStep1D, c->part);
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, c->st, c->nb, c->part,
                      c->scal, useDevT, tHost, pTime, stopTime);
+  tspan_end(c, e0, KT_DT);
 }
 
 extern "C" int ccfd_gpu_calc_dt(ccfd_gpu_ctx* c, double t, double pTime,
@@ -831,7 +882,9 @@ static void step_t(ccfd_gpu_ctx* c, double t, double dt, bool devDt, bool res) {
     a.U0 = c->U0;
     a.U = c->U;
     a.prim_out = (T*)c->prim[1 - c->cur];
-    a.write_ut = 1;
+    // dU/dt of the last stage is kept for host readback; earlier stages
+    // feed the update only
+    a.write_ut = (k == nst) ? 1 : 0;
     a.res_part = (res && k == nst) ? c->part : nullptr;
     if (!c->keepGradients && !a.ns) { a.ux = nullptr; a.uy = nullptr; }
     launch_operator<T>(c, a);
@@ -951,6 +1004,44 @@ extern "C" int ccfd_gpu_riemann_batch(int code, int precision, double gam,
   hipError_t e = hipMemcpy(F, dF, b, hipMemcpyDeviceToHost);
   hipFree(dL); hipFree(dR); hipFree(dF);
   return e == hipSuccess ? CCFD_OK : fail(CCFD_EHIP, "riemann batch failed");
+}
+
+extern "C" int ccfd_gpu_profile_steps(ccfd_gpu_ctx* c, int nSteps, double pTime,
+                                     double stopTime, double* ms,
+                                     int* launches) {
+  if (!c || !c->hasMesh || !ms || nSteps <= 0)
+    return fail(CCFD_EINVAL, "bad profile arguments");
+  hipSetDevice(c->device);
+  c->timing = true;
+  c->evUsed = 0;
+  c->spans.clear();
+  double t = 0.0;
+  hipEvent_t all0 = take_event(c), all1 = nullptr;
+  hipEventRecord(all0, c->st);
+  int rc = ccfd_gpu_advance(c, nSteps, &t, pTime, stopTime, nullptr);
+  all1 = take_event(c);
+  hipEventRecord(all1, c->st);
+  c->timing = false;
+  if (rc) return rc;
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "profile sync failed");
+  double sum[KT_N] = {0}, n[KT_N] = {0};
+  for (auto& sp : c->spans) {
+    float x = 0.f;
+    hipEventElapsedTime(&x, sp.a, sp.b);
+    sum[sp.kind] += x;
+    n[sp.kind] += 1;
+  }
+  float tot = 0.f;
+  hipEventElapsedTime(&tot, all0, all1);
+  for (int k = 0; k < KT_N; ++k) {
+    ms[k] = n[k] > 0 ? sum[k] / n[k] : 0.0;
+    if (launches) launches[k] = (int)n[k];
+  }
+  ms[KT_N] = tot / nSteps;
+  c->spans.clear();
+  c->evUsed = 0;
+  return CCFD_OK;
 }
 
 extern "C" int ccfd_gpu_synchronize(ccfd_gpu_ctx* c) {

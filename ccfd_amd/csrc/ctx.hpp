@@ -43,6 +43,8 @@ struct ccfd_gpu_ctx {
   // mesh
   int* link = nullptr;
   int* ghostBC = nullptr;
+  int* ghostCell = nullptr;
+  int* ghostK = nullptr;
   double* ghostBary = nullptr;
   double* bary = nullptr;
   double* area = nullptr;
@@ -61,12 +63,19 @@ struct ccfd_gpu_ctx {
   void* ux = nullptr;
   void* uy = nullptr;
   void* side = nullptr;
-  void* ghostP = nullptr;
+  void* ghostP = nullptr;   // ghost cell states [4][G]
+  void* ghostS = nullptr;   // ghost side states [4][G]
   double* stage = nullptr;   // staging for host I/O
   double* part = nullptr;    // per-block reduction partials
   double* scal = nullptr;    // device scalars: t, dt, viscDom, nonfinite, res[4]
   double* hscal = nullptr;   // pinned host mirror of scal
   bool keepGradients = false;
+  // optional per-kernel timing with HIP events on the context stream
+  struct Span { hipEvent_t a, b; int kind; };
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evUsed = 0;
+  std::vector<Span> spans;
   std::vector<void*> allocs;
   ccfd::ImplicitState* imp = nullptr;
 };
@@ -87,5 +96,9 @@ void launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a);
 void launch_dt(ccfd_gpu_ctx* c, int useDevT, double tHost, double pTime,
                double stopTime);
 void implicit_free(ccfd_gpu_ctx* c);
+// kernel kinds for timing spans
+enum { KT_DT = 0, KT_RECON = 1, KT_FLUX = 2, KT_OTHER = 3, KT_N = 4 };
+void tspan_begin(ccfd_gpu_ctx* c, hipEvent_t* e);
+void tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind);
 
 }  // namespace ccfd

@@ -60,12 +60,15 @@ struct OpArgs {
   // ghosts [G]
   const int* __restrict__ ghostBC;
   const double* __restrict__ ghostBary;  // [2][G]
+  const int* __restrict__ ghostCell;     // interior element (renumbered)
+  const int* __restrict__ ghostK;        // its slot facing the ghost
   // state
   const T* __restrict__ prim;   // [4][Np] operator input
   T* __restrict__ side;         // [4][S][Np]
   T* __restrict__ ux;           // [4][Np]
   T* __restrict__ uy;
-  T* __restrict__ ghostP;       // [4][G] ghost element states
+  T* __restrict__ ghostP;       // [4][G] ghost element states (barycentres)
+  T* __restrict__ ghostS;       // [4][G] ghost side states
   T* __restrict__ ut;           // [4][Np]
   // time: time = tdev ? tdev[0] + tcoef * tdev[1] : time0
   double time0;
@@ -104,7 +107,7 @@ __device__ inline T op_time(const OpArgs<T>& a) {
 }
 
 // ------------------------------------------------------------ recon kernel
-template <typename T, int LIM, int NS>
+template <typename T, int LIM>
 __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
   const int i = logical_block() * kBlock + threadIdx.x;
   if (i >= a.N) return;
@@ -112,63 +115,53 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
   T p[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) p[v] = a.prim[v * Np + i];
-  T q[kMaxSlots][4];
-  int lk[kMaxSlots];
   T gx[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
   T gy[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
   T umin[4], umax[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) umin[v] = umax[v] = p[v];
+  // least-squares gradient and neighbour extrema, sides in reference order
+  for (int s = 0; s < S; ++s) {
+    const int lk = a.link[s * Np + i];
+    if (lk == kNoSide) continue;
+    T q[4];
+    if (lk >= 0) {
+      const int j = lk >> 2;
 #pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) {
-    lk[s] = s < S ? a.link[s * Np + i] : kNoSide;
-    if (lk[s] == kNoSide) continue;
-    if (lk[s] >= 0) {
-      int j = lk[s] >> 2;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) q[s][v] = a.prim[v * Np + j];
+      for (int v = 0; v < 4; ++v) q[v] = a.prim[v * Np + j];
     } else {
-      int g = -lk[s] - 1;
-      const BCDev& bc = a.bct.bc[a.ghostBC[g]];
-      T x[2] = {T(a.ghostBary[g]), T(a.ghostBary[a.G + g])};
-      bc_ghost<T>(a.eq, bc, a.ex, a.fn[s * Np + i], a.fn[(S + s) * Np + i],
-                  op_time(a), p, q[s], x);
-      if (NS) {
+      // ghost cell state from setBCatBarys (bc_bary_kernel)
+      const int g = -lk - 1;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) a.ghostP[v * a.G + g] = q[s][v];
-      }
+      for (int v = 0; v < 4; ++v) q[v] = a.ghostP[v * a.G + g];
     }
-    T wx = a.w[s * Np + i], wy = a.w[(S + s) * Np + i];
+    const T wx = a.w[s * Np + i], wy = a.w[(S + s) * Np + i];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      T d = q[s][v] - p[v];
+      const T d = q[v] - p[v];
       gx[v] += wx * d;
       gy[v] += wy * d;
-      umin[v] = tmin(umin[v], q[s][v]);
-      umax[v] = tmax(umax[v], q[s][v]);
+      umin[v] = tmin(umin[v], q[v]);
+      umax[v] = tmax(umax[v], q[v]);
     }
   }
-  T dmin[4], dmax[4], dmin2[4], dmax2[4], phi[4];
+  T dmin[4], dmax[4], phi[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     dmin[v] = umin[v] - p[v];
     dmax[v] = umax[v] - p[v];
-    dmin2[v] = dmin[v] * dmin[v];
-    dmax2[v] = dmax[v] * dmax[v];
     phi[v] = T(1.0);
   }
   const T eps = LIM == 2 ? a.venk[i] : T(0.0);
-  T GPx[kMaxSlots], GPy[kMaxSlots];
-#pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) {
-    if (lk[s] == kNoSide) continue;
-    GPx[s] = a.gp[s * Np + i];
-    GPy[s] = a.gp[(S + s) * Np + i];
+  for (int s = 0; s < S; ++s) {
+    if (a.link[s * Np + i] == kNoSide) continue;
+    const T GPx = a.gp[s * Np + i], GPy = a.gp[(S + s) * Np + i];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      T d = gx[v] * GPx[s] + gy[v] * GPy[s];
-      phi[v] = tmin(phi[v], limiter_phi<LIM, T>(d, dmax[v], dmin[v], dmax2[v],
-                                                dmin2[v], eps));
+      const T d = gx[v] * GPx + gy[v] * GPy;
+      phi[v] = tmin(phi[v], limiter_phi<LIM, T>(d, dmax[v], dmin[v],
+                                                dmax[v] * dmax[v],
+                                                dmin[v] * dmin[v], eps));
     }
   }
 #pragma unroll
@@ -183,13 +176,57 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
       a.uy[v * Np + i] = gy[v];
     }
   }
-#pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) {
-    if (lk[s] == kNoSide) continue;
+  for (int s = 0; s < S; ++s) {
+    if (a.link[s * Np + i] == kNoSide) continue;
+    const T GPx = a.gp[s * Np + i], GPy = a.gp[(S + s) * Np + i];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      a.side[(v * S + s) * Np + i] = p[v] + GPx[s] * gx[v] + GPy[s] * gy[v];
+      a.side[(v * S + s) * Np + i] = p[v] + GPx * gx[v] + GPy * gy[v];
   }
+}
+
+// ------------------------------------------------------- boundary kernels
+// setBCatBarys (src/boundary.c:320-331): ghost cell states at the mirrored
+// barycentres from the interior element state; one thread per ghost side.
+template <typename T>
+__global__ void bc_bary_kernel(OpArgs<T> a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G) return;
+  const int i = a.ghostCell[g], k = a.ghostK[g];
+  const int Np = a.Np, S = a.S;
+  T in[4], out[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) in[v] = a.prim[v * Np + i];
+  T x[2] = {T(a.ghostBary[g]), T(a.ghostBary[a.G + g])};
+  bc_ghost<T>(a.eq, a.bct.bc[a.ghostBC[g]], a.ex, a.fn[k * Np + i],
+              a.fn[(S + k) * Np + i], op_time(a), in, out, x);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.ghostP[v * a.G + g] = out[v];
+}
+
+// setBCatSides (src/boundary.c:300-314): ghost side states from the
+// reconstructed interior side state (the element state at 1st order), at
+// x = side midpoint.
+template <typename T>
+__global__ void bc_side_kernel(OpArgs<T> a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G) return;
+  const int i = a.ghostCell[g], k = a.ghostK[g];
+  const int Np = a.Np, S = a.S;
+  T in[4], out[4];
+  if (a.order == 1) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) in[v] = a.prim[v * Np + i];
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) in[v] = a.side[(v * S + k) * Np + i];
+  }
+  T x[2] = {a.gp[k * Np + i] + T(a.bary[i]),
+            a.gp[(S + k) * Np + i] + T(a.bary[Np + i])};
+  bc_ghost<T>(a.eq, a.bct.bc[a.ghostBC[g]], a.ex, a.fn[k * Np + i],
+              a.fn[(S + k) * Np + i], op_time(a), in, out, x);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
 }
 
 // ------------------------------------------------------------- flux kernel
@@ -228,9 +265,10 @@ __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) pc[v] = a.prim[v * Np + i];
     }
-#pragma unroll
-    for (int s = 0; s < kMaxSlots; ++s) {
-      const int lk = s < S ? a.link[s * Np + i] : kNoSide;
+    // one side at a time keeps register pressure (and occupancy) in check
+#pragma unroll 1
+    for (int s = 0; s < S; ++s) {
+      const int lk = a.link[s * Np + i];
       if (lk == kNoSide) continue;
       T own[4], oth[4];
       if (ORDER == 1) {
@@ -256,14 +294,10 @@ __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
           for (int v = 0; v < 4; ++v) oth[v] = a.side[(v * S + ps) * Np + j];
         }
       } else {
+        // ghost side state from setBCatSides (bc_side_kernel)
         g = -lk - 1;
-        const BCDev& bc = a.bct.bc[a.ghostBC[g]];
-        T x[2] = {T(0.0), T(0.0)};
-        if (bc.type == 6) {
-          x[0] = a.gp[s * Np + i] + T(a.bary[i]);
-          x[1] = a.gp[(S + s) * Np + i] + T(a.bary[Np + i]);
-        }
-        bc_ghost<T>(a.eq, bc, a.ex, nx, ny, op_time(a), own, oth, x);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) oth[v] = a.ghostS[v * a.G + g];
       }
       const T* L = owner ? own : oth;
       const T* R = owner ? oth : own;

@@ -17,8 +17,10 @@
 
 #ifdef __HIPCC__
 #define CCFD_HD __host__ __device__ inline
+#define CCFD_HD_NOINLINE __host__ __device__ __attribute__((noinline))
 #else
 #define CCFD_HD inline
+#define CCFD_HD_NOINLINE __attribute__((noinline))
 #endif
 
 namespace ccfd {
@@ -350,36 +352,54 @@ CCFD_HD void flux_hlle(const Eq<T>& q, const T* L, const T* R, T* F) {
   hll_blend(h, arp, alm, F);
 }
 
+// HLLC (src/fluxCalculation.c:315-396). The upwind side K is selected
+// first and one state / flux / star state is evaluated for it: the same
+// arithmetic as each branch of the reference, with half the live values.
 template <typename T>
 CCFD_HD void flux_hllc(const Eq<T>& q, const T* L, const T* R, T* F) {
-  HllPre<T> h;
-  hll_prefix(q, L, R, h);
-  T uM, cM;
-  roe_means(q, h, &uM, &cM);
-  T arp = tmax(h.vxR + h.cR, uM + cM);
-  T alm = tmin(h.vxL - h.cL, uM - cM);
-  if (alm > T(0.0)) {
-    for (int i = 0; i < 4; ++i) F[i] = h.fL[i];
-  } else if (arp < T(0.0)) {
-    for (int i = 0; i < 4; ++i) F[i] = h.fR[i];
-  } else {
-    T as = (h.pR - h.pL + h.uL[1] * (alm - h.vxL) - h.uR[1] * (arp - h.vxR)) /
-           (h.rhoL * (alm - h.vxL) - h.rhoR * (arp - h.vxR));
-    if ((alm <= T(0.0)) && (as >= T(0.0))) {
-      T fac = h.rhoL * (alm - h.vxL) / (alm - as);
-      T us[4] = {fac, as * fac, h.vyL * fac,
-                 fac * (h.eL / h.rhoL +
-                        (as - h.vxL) * (as + h.pL / (h.rhoL * (alm - h.vxL))))};
-#pragma unroll
-      for (int i = 0; i < 4; ++i) F[i] = h.fL[i] + alm * (us[i] - h.uL[i]);
-    } else {
-      T fac = h.rhoR * (arp - h.vxR) / (arp - as);
-      T us[4] = {fac, as * fac, h.vyR * fac,
-                 fac * (h.eR / h.rhoR +
-                        (as - h.vxR) * (as + h.pR / (h.rhoR * (arp - h.vxR))))};
-#pragma unroll
-      for (int i = 0; i < 4; ++i) F[i] = h.fR[i] + arp * (us[i] - h.uR[i]);
-    }
+  const T rhoL = L[0], vxL = L[1], vyL = L[2], pL = L[3];
+  const T rhoR = R[0], vxR = R[1], vyR = R[2], pR = R[3];
+  const T rLq = T(1.0) / rhoL;
+  const T rRq = T(1.0) / rhoR;
+  const T sL = tsqrt(rhoL);
+  const T sR = tsqrt(rhoR);
+  const T sQ = T(1.0) / (sL + sR);
+  const T eR = q.gam1q * pR + T(0.5) * rhoR * (vxR * vxR + vyR * vyR);
+  const T eL = q.gam1q * pL + T(0.5) * rhoL * (vxL * vxL + vyL * vyL);
+  const T cL = tsqrt(q.gam * pL * rLq);
+  const T cR = tsqrt(q.gam * pR * rRq);
+  const T HL = (eL + pL) * rLq;
+  const T HR = (eR + pR) * rRq;
+  const T uM = (sR * vxR + sL * vxL) * sQ;
+  const T vM = (sR * vyR + sL * vyL) * sQ;
+  const T HM = (sR * HR + sL * HL) * sQ;
+  const T cM = tsqrt(q.gam1 * (HM - T(0.5) * (uM * uM + vM * vM)));
+  const T arp = tmax(vxR + cR, uM + cM);
+  const T alm = tmin(vxL - cL, uM - cM);
+  const bool supL = alm > T(0.0);
+  const bool supR = !supL && (arp < T(0.0));
+  T as = T(0.0);
+  bool left = supL;
+  if (!supL && !supR) {
+    as = (pR - pL + (rhoL * vxL) * (alm - vxL) - (rhoR * vxR) * (arp - vxR)) /
+         (rhoL * (alm - vxL) - rhoR * (arp - vxR));
+    left = (alm <= T(0.0)) && (as >= T(0.0));
+  }
+  const T rho = left ? rhoL : rhoR, vx = left ? vxL : vxR;
+  const T vy = left ? vyL : vyR, p = left ? pL : pR, e = left ? eL : eR;
+  const T m = rho * vx;
+  F[0] = m;
+  F[1] = m * vx + p;
+  F[2] = m * vy;
+  F[3] = vx * (e + p);
+  if (!supL && !supR) {
+    const T Sk = left ? alm : arp;
+    const T fac = rho * (Sk - vx) / (Sk - as);
+    const T us3 = fac * (e / rho + (as - vx) * (as + p / (rho * (Sk - vx))));
+    F[0] = F[0] + Sk * (fac - rho);
+    F[1] = F[1] + Sk * (as * fac - m);
+    F[2] = F[2] + Sk * (vy * fac - rho * vy);
+    F[3] = F[3] + Sk * (us3 - e);
   }
 }
 
@@ -731,6 +751,38 @@ CCFD_HD void exact_func(const Eq<T>& q, const ExactCfg& ex, int f,
   }
 }
 
+// characteristic (type 5) and exact-function (type 6) ghost states: kept out
+// of line so their transcendental / iterative code does not set the register
+// budget of the operator kernels (src/boundary.c:217-274)
+template <typename T>
+CCFD_HD_NOINLINE void bc_ghost_rare(const Eq<T>& q, const BCDev& b,
+                                    const ExactCfg& ex, T nx, T ny, T time,
+                                    const T* in, T* out, const T* x) {
+  if (b.type == 6) {
+    exact_func(q, ex, b.exactFunc, x, time, out);
+    return;
+  }
+  T bp[4] = {T(b.pVar[0]), T(b.pVar[1]), T(b.pVar[2]), T(b.pVar[3])};
+  T c = tsqrt(q.gam * bp[3] / bp[0]);
+  T v = nx * bp[1] + ny * bp[2];
+  T il[4] = {in[0], nx * in[1] + ny * in[2], -ny * in[1] + nx * in[2], in[3]};
+  T gl[4] = {bp[0], nx * bp[1] + ny * bp[2], -ny * bp[1] + nx * bp[2], bp[3]};
+  T ic[4], gc[4], ich[3], gch[3];
+  prim_cons(q, il, ic);
+  prim_cons(q, gl, gc);
+  cons_char(q, ic, ich, in);
+  cons_char(q, gc, gch, in);
+  if (v + c > T(0.0)) gch[2] = ich[2];
+  if (v > T(0.0)) gch[1] = ich[1];
+  if (v - c > T(0.0)) gch[0] = ich[0];
+  char_cons(q, gch, gc, in);
+  if (v > T(0.0)) gc[2] = ic[2];
+  cons_prim(q, gc, out);
+  T vx = out[1], vy = out[2];
+  out[1] = nx * vx - ny * vy;
+  out[2] = ny * vx + nx * vy;
+}
+
 // boundary() (src/boundary.c:147-294). n: outward normal of the interior side.
 template <typename T>
 CCFD_HD void bc_ghost(const Eq<T>& q, const BCDev& b, const ExactCfg& ex,
@@ -757,30 +809,9 @@ CCFD_HD void bc_ghost(const Eq<T>& q, const BCDev& b, const ExactCfg& ex,
     case 4:
       out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
       break;
-    case 5: {
-      T bp[4] = {T(b.pVar[0]), T(b.pVar[1]), T(b.pVar[2]), T(b.pVar[3])};
-      T c = tsqrt(q.gam * bp[3] / bp[0]);
-      T v = nx * bp[1] + ny * bp[2];
-      T il[4] = {in[0], nx * in[1] + ny * in[2], -ny * in[1] + nx * in[2], in[3]};
-      T gl[4] = {bp[0], nx * bp[1] + ny * bp[2], -ny * bp[1] + nx * bp[2], bp[3]};
-      T ic[4], gc[4], ich[3], gch[3];
-      prim_cons(q, il, ic);
-      prim_cons(q, gl, gc);
-      cons_char(q, ic, ich, in);
-      cons_char(q, gc, gch, in);
-      if (v + c > T(0.0)) gch[2] = ich[2];
-      if (v > T(0.0)) gch[1] = ich[1];
-      if (v - c > T(0.0)) gch[0] = ich[0];
-      char_cons(q, gch, gc, in);
-      if (v > T(0.0)) gc[2] = ic[2];
-      cons_prim(q, gc, out);
-      T vx = out[1], vy = out[2];
-      out[1] = nx * vx - ny * vy;
-      out[2] = ny * vx + nx * vy;
-      break;
-    }
+    case 5:
     case 6:
-      exact_func(q, ex, b.exactFunc, x, time, out);
+      bc_ghost_rare<T>(q, b, ex, nx, ny, time, in, out, x);
       break;
     case 8: {
       T c = tsqrt(q.gam * in[3] / in[0]);

@@ -87,6 +87,8 @@ def lib():
         "ccfd_gpu_riemann_batch": (C.c_int, [C.c_int, C.c_int, C.c_double,
                                              C.c_long, dp, dp, dp]),
         "ccfd_gpu_synchronize": (C.c_int, [vp]),
+        "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
+                                             C.c_double, dp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -208,3 +210,13 @@ class Solver:
 
     def sync(self):
         check(self.L.ccfd_gpu_synchronize(self.h))
+
+    def profile_steps(self, nsteps, ptime=1e300, tend=1e300):
+        """average kernel durations (ms) over nsteps device-driven steps:
+        dict(dt, recon, flux, other, step) and launch counts"""
+        ms = np.zeros(5)
+        cnt = (C.c_int * 4)()
+        check(self.L.ccfd_gpu_profile_steps(self.h, nsteps, ptime, tend,
+                                            N.dptr(ms), cnt))
+        keys = ("dt", "recon", "flux", "other", "step")
+        return dict(zip(keys, ms.tolist())), list(cnt)

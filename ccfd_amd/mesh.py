@@ -132,30 +132,49 @@ class Mesh:
         self.n_ghosts = m.nGhosts
         self.total_area_q = m.totalArea_q
         self.bbox = (m.xMin, m.xMax, m.yMin, m.yMax)
-        ne, ns, ng = m.nElems, m.nSlots, m.nGhosts
-        self.elem_type = N.view(m.elemType, ne, np.int32)
-        self.bary = N.view(m.bary, 2 * ne, np.float64).reshape(ne, 2)
-        self.area = N.view(m.area, ne, np.float64)
-        self.areaq = N.view(m.areaq, ne, np.float64)
-        self.sx = N.view(m.sx, ne, np.float64)
-        self.sy = N.view(m.sy, ne, np.float64)
-        self.slot_start = N.view(m.slotStart, ne + 1, np.int64)
-        self.nbr = N.view(m.nbr, ns, np.int64)
-        self.partner = N.view(m.partner, ns, np.int64)
-        self.owner = N.view(m.owner, ns, np.uint8)
-        self.n = N.view(m.n, 2 * ns, np.float64).reshape(ns, 2)
-        self.len = N.view(m.len, ns, np.float64)
-        self.fn = N.view(m.fn, 2 * ns, np.float64).reshape(ns, 2)
-        self.flen = N.view(m.flen, ns, np.float64)
-        self.GP = N.view(m.GP, 2 * ns, np.float64).reshape(ns, 2)
-        self.GPc = N.view(m.GPc, 2 * ns, np.float64).reshape(ns, 2)
-        self.w = N.view(m.w, 2 * ns, np.float64).reshape(ns, 2)
-        self.bbv = N.view(m.bbv, 2 * ns, np.float64).reshape(ns, 2)
-        self.bbd = N.view(m.bbd, ns, np.float64)
-        self.ghost_slot = N.view(m.ghostSlot, ng, np.int64)
-        self.ghost_bc = N.view(m.ghostBC, ng, np.int32)
-        self.ghost_bary = N.view(m.ghostBary, 2 * ng, np.float64).reshape(ng, 2)
-        self.face_slot = N.view(m.faceSlot, m.nFaces, np.int64)
+        self._cache = {}
+
+    # name -> (struct field, count kind, per-item width, dtype); arrays are
+    # copied out of the C structure on first access
+    _ARRAYS = {
+        "elem_type": ("elemType", "e", 1, np.int32),
+        "bary": ("bary", "e", 2, np.float64),
+        "area": ("area", "e", 1, np.float64),
+        "areaq": ("areaq", "e", 1, np.float64),
+        "sx": ("sx", "e", 1, np.float64),
+        "sy": ("sy", "e", 1, np.float64),
+        "slot_start": ("slotStart", "e+1", 1, np.int64),
+        "nbr": ("nbr", "s", 1, np.int64),
+        "partner": ("partner", "s", 1, np.int64),
+        "owner": ("owner", "s", 1, np.uint8),
+        "n": ("n", "s", 2, np.float64),
+        "len": ("len", "s", 1, np.float64),
+        "fn": ("fn", "s", 2, np.float64),
+        "flen": ("flen", "s", 1, np.float64),
+        "GP": ("GP", "s", 2, np.float64),
+        "GPc": ("GPc", "s", 2, np.float64),
+        "w": ("w", "s", 2, np.float64),
+        "bbv": ("bbv", "s", 2, np.float64),
+        "bbd": ("bbd", "s", 1, np.float64),
+        "ghost_slot": ("ghostSlot", "g", 1, np.int64),
+        "ghost_bc": ("ghostBC", "g", 1, np.int32),
+        "ghost_bary": ("ghostBary", "g", 2, np.float64),
+        "face_slot": ("faceSlot", "f", 1, np.int64),
+    }
+
+    def __getattr__(self, name):
+        spec = Mesh._ARRAYS.get(name)
+        if spec is None or "_p" not in self.__dict__:
+            raise AttributeError(name)
+        cache = self.__dict__["_cache"]
+        if name not in cache:
+            field, kind, width, dt = spec
+            n = {"e": self.n_elems, "e+1": self.n_elems + 1,
+                 "s": self.n_slots, "g": self.n_ghosts,
+                 "f": self.n_faces}[kind]
+            a = N.view(getattr(self._p.contents, field), n * width, dt)
+            cache[name] = a.reshape(n, width) if width > 1 else a
+        return cache[name]
 
     @property
     def ptr(self):

@@ -152,7 +152,16 @@ int ccfd_gpu_implicit_step(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
 int ccfd_gpu_riemann_batch(int code, int precision, double gam, long n,
 		const double *L, const double *R, double *F);
 
-/* ---- timing helper ---- */
+/* ---- timing helpers ---- */
+/*
+ * Run nSteps device-driven steps (as ccfd_gpu_advance from t = 0) with HIP
+ * events around every kernel on the context's stream. ms[0..3]: average
+ * duration of the dt reduction, reconstruction kernel, flux / update kernel
+ * and other kernels; ms[4]: wall time per step. launches[0..3] (may be NULL)
+ * gets the launch counts.
+ */
+int ccfd_gpu_profile_steps(ccfd_gpu_ctx *ctx, int nSteps, double pTime,
+		double stopTime, double *ms, int *launches);
 /* wait for all work queued on the context's stream */
 int ccfd_gpu_synchronize(ccfd_gpu_ctx *ctx);
 
