@@ -137,6 +137,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a 1 GiB streaming copy (reachable HBM "
+                         "bandwidth; counter calibration under rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -185,6 +188,11 @@ def main():
             "kernel_ms": round(ms["flux"], 4),
             "bytes_per_cell": bflux}
 
+    copy_gbs = None
+    if args.copy_probe:
+        copy_gbs = G.copy_probe(1 << 30, 20)
+        log("copy probe: %.0f GB/s" % copy_gbs)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -217,6 +225,7 @@ def main():
         },
         "hbm_gbs_stage_effective": round(value * bstage / 1e9, 1),
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "copy_probe_gbs": copy_gbs,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

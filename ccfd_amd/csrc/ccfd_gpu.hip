@@ -781,8 +781,13 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
       tspan_end(c, eb, KT_OTHER);
     }
     tspan_begin(c, &e0);
-    if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1>), g, b, 0, c->st, a);
-    else hipLaunchKernelGGL((recon_kernel<T, 2>), g, b, 0, c->st, a);
+    if (c->S == 3) {
+      if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1, 3>), g, b, 0, c->st, a);
+      else hipLaunchKernelGGL((recon_kernel<T, 2, 3>), g, b, 0, c->st, a);
+    } else {
+      if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1, 4>), g, b, 0, c->st, a);
+      else hipLaunchKernelGGL((recon_kernel<T, 2, 4>), g, b, 0, c->st, a);
+    }
     tspan_end(c, e0, KT_RECON);
   }
   if (c->G > 0) {
@@ -1041,6 +1046,49 @@ extern "C" int ccfd_gpu_profile_steps(ccfd_gpu_ctx* c, int nSteps, double pTime,
   ms[KT_N] = tot / nSteps;
   c->spans.clear();
   c->evUsed = 0;
+  return CCFD_OK;
+}
+
+// plain streaming copy, 16 B per thread per iteration: the calibration
+// kernel for HBM byte counters and the reachable-bandwidth reference
+__global__ __launch_bounds__(256) void copy_probe_kernel(
+    const double2* __restrict__ src, double2* __restrict__ dst, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long st = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) dst[i] = src[i];
+}
+
+extern "C" int ccfd_gpu_copy_probe(int device, size_t bytes, int reps,
+                                   double* gbs) {
+  if (bytes < 4096 || reps <= 0 || !gbs)
+    return fail(CCFD_EINVAL, "bad copy probe arguments");
+  if (hipSetDevice(device) != hipSuccess)
+    return fail(CCFD_EHIP, "hipSetDevice failed");
+  long n = (long)(bytes / 16);
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    hipFree(a);
+    return fail(CCFD_ENOMEM, "copy probe allocation failed");
+  }
+  hipMemset(a, 0, n * 16);
+  int blocks = 256 * 8 * 4;   // 8 waves per CU on 256 CUs, grid-stride loop
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  hipEventRecord(e1, 0);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (err != hipSuccess) return fail(CCFD_EHIP, "copy probe failed");
+  *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
   return CCFD_OK;
 }
 

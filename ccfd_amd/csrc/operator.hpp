@@ -107,31 +107,38 @@ __device__ inline T op_time(const OpArgs<T>& a) {
 }
 
 // ------------------------------------------------------------ recon kernel
-template <typename T, int LIM>
-__global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
+// SC: sides per element known at compile time (3: triangles, 4: quads /
+// mixed with padding) so all neighbour loads of an element issue together
+template <typename T, int LIM, int SC>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void recon_kernel(OpArgs<T> a) {
   const int i = logical_block() * kBlock + threadIdx.x;
   if (i >= a.N) return;
-  const int Np = a.Np, S = a.S;
+  const int Np = a.Np;
+  constexpr int S = SC;
   T p[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) p[v] = a.prim[v * Np + i];
+  int lk[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) lk[s] = a.link[s * Np + i];
   T gx[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
   T gy[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
   T umin[4], umax[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) umin[v] = umax[v] = p[v];
   // least-squares gradient and neighbour extrema, sides in reference order
+#pragma unroll
   for (int s = 0; s < S; ++s) {
-    const int lk = a.link[s * Np + i];
-    if (lk == kNoSide) continue;
+    if (lk[s] == kNoSide) continue;
     T q[4];
-    if (lk >= 0) {
-      const int j = lk >> 2;
+    if (lk[s] >= 0) {
+      const int j = lk[s] >> 2;
 #pragma unroll
       for (int v = 0; v < 4; ++v) q[v] = a.prim[v * Np + j];
     } else {
       // ghost cell state from setBCatBarys (bc_bary_kernel)
-      const int g = -lk - 1;
+      const int g = -lk[s] - 1;
 #pragma unroll
       for (int v = 0; v < 4; ++v) q[v] = a.ghostP[v * a.G + g];
     }
@@ -145,29 +152,27 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
       umax[v] = tmax(umax[v], q[v]);
     }
   }
-  T dmin[4], dmax[4], phi[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    dmin[v] = umin[v] - p[v];
-    dmax[v] = umax[v] - p[v];
-    phi[v] = T(1.0);
-  }
   const T eps = LIM == 2 ? a.venk[i] : T(0.0);
+  T GPx[S], GPy[S];
+#pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (a.link[s * Np + i] == kNoSide) continue;
-    const T GPx = a.gp[s * Np + i], GPy = a.gp[(S + s) * Np + i];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const T d = gx[v] * GPx + gy[v] * GPy;
-      phi[v] = tmin(phi[v], limiter_phi<LIM, T>(d, dmax[v], dmin[v],
-                                                dmax[v] * dmax[v],
-                                                dmin[v] * dmin[v], eps));
-    }
+    GPx[s] = a.gp[s * Np + i];
+    GPy[s] = a.gp[(S + s) * Np + i];
   }
-#pragma unroll
+  // limiter: one variable at a time bounds the live division temporaries
+#pragma unroll 1
   for (int v = 0; v < 4; ++v) {
-    gx[v] *= phi[v];
-    gy[v] *= phi[v];
+    const T dmin = umin[v] - p[v], dmax = umax[v] - p[v];
+    T phi = T(1.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (lk[s] == kNoSide) continue;
+      const T d = gx[v] * GPx[s] + gy[v] * GPy[s];
+      phi = tmin(phi, limiter_phi<LIM, T>(d, dmax, dmin, dmax * dmax,
+                                          dmin * dmin, eps));
+    }
+    gx[v] *= phi;
+    gy[v] *= phi;
   }
   if (a.ux) {
 #pragma unroll
@@ -176,12 +181,12 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
       a.uy[v * Np + i] = gy[v];
     }
   }
+#pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (a.link[s * Np + i] == kNoSide) continue;
-    const T GPx = a.gp[s * Np + i], GPy = a.gp[(S + s) * Np + i];
+    if (lk[s] == kNoSide) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      a.side[(v * S + s) * Np + i] = p[v] + GPx * gx[v] + GPy * gy[v];
+      a.side[(v * S + s) * Np + i] = p[v] + GPx[s] * gx[v] + GPy[s] * gy[v];
   }
 }
 

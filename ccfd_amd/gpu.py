@@ -89,6 +89,7 @@ def lib():
         "ccfd_gpu_synchronize": (C.c_int, [vp]),
         "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
                                              C.c_double, dp, ip]),
+        "ccfd_gpu_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -112,6 +113,13 @@ def riemann_batch(code, L, R, gamma=1.4, precision=FP64):
     check(lib().ccfd_gpu_riemann_batch(code, precision, gamma, len(L),
                                        N.dptr(L), N.dptr(R), N.dptr(F)))
     return F
+
+
+def copy_probe(nbytes=1 << 30, reps=20, device=0):
+    """streaming-copy bandwidth (GB/s, read + write) on the device"""
+    g = C.c_double()
+    check(lib().ccfd_gpu_copy_probe(device, nbytes, reps, C.byref(g)))
+    return g.value
 
 
 class Solver:

@@ -162,6 +162,13 @@ int ccfd_gpu_riemann_batch(int code, int precision, double gam, long n,
  */
 int ccfd_gpu_profile_steps(ccfd_gpu_ctx *ctx, int nSteps, double pTime,
 		double stopTime, double *ms, int *launches);
+/*
+ * Streaming copy of `bytes` between two device buffers, repeated `reps`
+ * times; *gbs = (read + write bytes) / time. Reference for the HBM
+ * bandwidth reachable on the device and calibration kernel for traffic
+ * counters.
+ */
+int ccfd_gpu_copy_probe(int device, size_t bytes, int reps, double *gbs);
 /* wait for all work queued on the context's stream */
 int ccfd_gpu_synchronize(ccfd_gpu_ctx *ctx);
 

@@ -1,0 +1,131 @@
+"""Mesh preprocessing (ccfd_amd/csrc/mesh.c): what the mathematics and the
+reference's construction fix, checked independently of the oracle."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from ccfd_amd import mesh as M
+from tests import cases
+
+
+def _per_slot_elem(m):
+    return m.slot_elem()
+
+
+def test_cartesian_closed_forms():
+    """uniform quads: area dx*dy, axis-aligned unit normals, GP = half
+    cell extent, LSQ weights of the central-difference stencil"""
+    m = cases.sod_mesh(100, 2)
+    dx, dy = 0.01, 0.005
+    assert np.allclose(m.area, dx * dy, rtol=1e-12, atol=0)
+    assert np.allclose(np.abs(m.n).max(axis=1), 1.0, rtol=0, atol=1e-15)
+    assert np.allclose(np.abs(m.n).min(axis=1), 0.0, rtol=0, atol=1e-15)
+    assert np.allclose(m.sx, dy, rtol=1e-12) and np.allclose(m.sy, dx, rtol=1e-12)
+    assert abs(m.total_area_q * 0.01 - 1.0) < 1e-12
+    # interior cell: w = d / (2 |d|^2) along each axis
+    e = 150
+    for s in range(m.slot_start[e], m.slot_start[e + 1]):
+        d = m.bbv[s]
+        ax = np.argmax(np.abs(d))
+        assert abs(m.w[s, ax] - d[ax] / (2 * d[ax] ** 2)) < 1e-9 / abs(d[ax])
+        assert abs(m.w[s, 1 - ax]) < 1e-6
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_delaunay_geometry_invariants(periodic):
+    m = cases.delaunay_mesh(25, seed=5, periodic=periodic)
+    se = _per_slot_elem(m)
+    # closed cells: sum of n * len vanishes
+    acc = np.zeros((m.n_elems, 2))
+    np.add.at(acc, se, m.n * m.len[:, None])
+    per = np.zeros(m.n_elems)
+    np.add.at(per, se, m.len)
+    assert (np.abs(acc).max(axis=1) / per).max() < 1e-13
+    # outward normals
+    assert ((m.n * m.GP).sum(axis=1) > 0).all()
+    # total area is the unit square
+    assert abs(m.area.sum() - 1.0) < 1e-12
+    assert abs(1.0 / m.total_area_q - 1.0) < 1e-12
+    # least squares: sum_s w_s (x) bbv_s = identity
+    W = np.zeros((m.n_elems, 2, 2))
+    np.add.at(W, se, m.w[:, :, None] * m.bbv[:, None, :])
+    assert np.abs(W - np.eye(2)).max() < 1e-9
+    # every non-ghost side has a partner facing back
+    inner = m.nbr >= 0
+    t = m.partner[inner]
+    assert (m.partner[t] == np.nonzero(inner)[0]).all()
+    assert np.array_equal(m.fn[inner], m.fn[t])
+    if periodic:
+        assert m.n_ghosts == 0
+    else:
+        assert m.n_ghosts == 4 * 24
+
+
+def test_face_ownership_follows_creation_order():
+    """after the stable (node0, node1, BC) sort the earlier-created side owns
+    the face (src/mesh.c:1713-1755): the lower element index"""
+    m = cases.delaunay_mesh(20, seed=1, periodic=False)
+    se = _per_slot_elem(m)
+    inner = np.nonzero(m.nbr >= 0)[0]
+    own = inner[m.owner[inner] == 1]
+    assert (se[own] < se[m.partner[own]]).all()
+    # ghosts: the physical side owns
+    assert (m.owner[m.nbr < 0] == 1).all()
+    # one owner per face
+    assert m.n_faces == (m.owner == 1).sum()
+
+
+def test_periodic_partners_translate():
+    m = cases.delaunay_mesh(20, seed=1, periodic=True)
+    se = _per_slot_elem(m)
+    mid = m.bary[se] + m.GP
+    midp = mid[m.partner]
+    d = np.abs(mid - midp)
+    wrap = (d > 0.5).any(axis=1)
+    assert wrap.sum() == 4 * 19   # 19 edges on each of the 4 sides
+    # translated by exactly one period in x or y
+    dd = np.sort(d[wrap], axis=1)
+    assert np.allclose(dd[:, 1], 1.0, atol=1e-12)
+    assert np.allclose(dd[:, 0], 0.0, atol=1e-12)
+
+
+def test_delaunay_empty_circumcircle():
+    raw, _ = M.jittered_delaunay(15, 0.3, 9, periodic=False)
+    xy, tri = raw.xy, raw.tri[:, :3]
+    for t in tri[::7]:
+        a, b, c = xy[t]
+        d = 2 * (a[0] * (b[1] - c[1]) + b[0] * (c[1] - a[1]) + c[0] * (a[1] - b[1]))
+        ux = ((a @ a) * (b[1] - c[1]) + (b @ b) * (c[1] - a[1]) + (c @ c) * (a[1] - b[1])) / d
+        uy = ((a @ a) * (c[0] - b[0]) + (b @ b) * (a[0] - c[0]) + (c @ c) * (b[0] - a[0])) / d
+        r2 = (a[0] - ux) ** 2 + (a[1] - uy) ** 2
+        dist = ((xy - [ux, uy]) ** 2).sum(axis=1)
+        dist[t] = np.inf
+        assert (dist >= r2 * (1 - 1e-9)).all()
+
+
+def test_gmsh_roundtrip():
+    raw, bcs = M.jittered_delaunay(12, 0.3, 3, periodic=True)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "m.msh")
+        raw.write_gmsh(p)
+        back = M.Raw.read_gmsh(p)
+    assert np.array_equal(back.xy, raw.xy)
+    assert np.array_equal(back.tri, raw.tri)
+    assert np.array_equal(back.bcedge, raw.bcedge)
+    m1, m2 = M.Mesh(raw, bcs), M.Mesh(back, bcs)
+    assert np.array_equal(m1.w, m2.w) and np.array_equal(m1.nbr, m2.nbr)
+
+
+def test_open_mesh_is_rejected():
+    raw = M.cartesian(4, 4)
+    raw.bcedge = raw.bcedge[:-1].copy()
+    with pytest.raises(ValueError):
+        M.Mesh(raw, [M.BC(101), M.BC(401)])
+
+
+def test_undefined_bc_is_rejected():
+    raw = M.cartesian(4, 4, side_bc=(101, 401, 101, 999))
+    with pytest.raises(ValueError):
+        M.Mesh(raw, [M.BC(101), M.BC(401)])
