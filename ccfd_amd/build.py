@@ -54,9 +54,14 @@ def build_host():
     cmd = ["gcc", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-O2",
            "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-I" + INC,
            "-o", out] + srcs + ["-lm"]
-    if os.path.exists(gpu):
-        cmd += ["-L" + LIB, "-lccfd_gpu", "-Wl,-rpath,$ORIGIN"]
+    if not os.path.exists(gpu):
+        raise RuntimeError("build the GPU library first (libccfd_gpu.so)")
+    cmd += ["-L" + LIB, "-lccfd_gpu", "-Wl,-rpath,$ORIGIN"]
     _run(cmd)
+    # `ccfd case.ini`: the reference's command line on the GPU path
+    _run(["gcc", "-std=c99", "-O2", "-I" + INC, "-o", os.path.join(LIB, "ccfd"),
+          os.path.join(CSRC, "ccfd_main.c"), "-L" + LIB, "-lccfd_host",
+          "-lccfd_gpu", "-Wl,-rpath,$ORIGIN", "-lm"])
     return out
 
 

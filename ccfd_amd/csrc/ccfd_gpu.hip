@@ -983,6 +983,95 @@ extern "C" int ccfd_gpu_residual(ccfd_gpu_ctx* c, double res[4]) {
   return CCFD_OK;
 }
 
+// dot product of two [4][Np] vectors: fixed-order block partials + final sum
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int N, int Np,
+                                                      const double* __restrict__ a,
+                                                      const double* __restrict__ b,
+                                                      double* part) {
+  __shared__ double red[kBlock / 64];
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  double x = 0.0;
+  if (i < N) {
+    x += a[i] * b[i];
+    x += a[Np + i] * b[Np + i];
+    x += a[2 * Np + i] * b[2 * Np + i];
+    x += a[3 * Np + i] * b[3 * Np + i];
+  }
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+    part[blockIdx.x] = s;
+  }
+}
+
+__global__ void k_sum_final(int nb, const double* __restrict__ part,
+                            double* out) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) acc += part[k];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+void ccfd::launch_dot(ccfd_gpu_ctx* c, const double* a, const double* b,
+                      double* out, double* part) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(c->nb), dim3(kBlock), 0, c->st, c->N,
+                     c->Np, a, b, part);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, c->st, c->nb, part, out);
+}
+
+int ccfd::upload_vec(ccfd_gpu_ctx* c, const double* h, double* d) {
+  std::vector<double> tmp(4 * (size_t)c->Np, 0.0);
+  for (int i = 0; i < c->N; ++i) {
+    int e = c->perm[i];
+    for (int v = 0; v < 4; ++v) tmp[(size_t)v * c->Np + i] = h[4 * (size_t)e + v];
+  }
+  if (hipMemcpyAsync(d, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "vector upload failed");
+  return CCFD_OK;
+}
+
+int ccfd::download_vec(ccfd_gpu_ctx* c, const double* d, double* h) {
+  return download_aos(c, d, h);
+}
+
+extern "C" int ccfd_gpu_dot(ccfd_gpu_ctx* c, const double* a, const double* b,
+                            double* out) {
+  if (!c || !c->hasMesh || !a || !b || !out)
+    return fail(CCFD_EINVAL, "bad dot arguments");
+  hipSetDevice(c->device);
+  size_t bytes = 4 * (size_t)c->Np * 8;
+  double *da = nullptr, *db = nullptr;
+  if (hipMalloc(&da, bytes) != hipSuccess || hipMalloc(&db, bytes) != hipSuccess) {
+    hipFree(da);
+    return fail(CCFD_ENOMEM, "dot allocation failed");
+  }
+  int rc = upload_vec(c, a, da);
+  if (!rc) rc = upload_vec(c, b, db);
+  if (!rc) {
+    launch_dot(c, da, db, c->scal + 8, c->part);
+    if (hipMemcpyAsync(c->hscal, c->scal, 16 * 8, hipMemcpyDeviceToHost,
+                       c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      rc = fail(CCFD_EHIP, "dot failed");
+    else
+      *out = c->hscal[8];
+  }
+  hipFree(da);
+  hipFree(db);
+  return rc;
+}
+
 extern "C" int ccfd_gpu_riemann_batch(int code, int precision, double gam,
                                       long n, const double* L, const double* R,
                                       double* F) {

@@ -96,6 +96,12 @@ void launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a);
 void launch_dt(ccfd_gpu_ctx* c, int useDevT, double tHost, double pTime,
                double stopTime);
 void implicit_free(ccfd_gpu_ctx* c);
+// out = sum a.b over [4][Np] vectors (device scalar), part: >= nb scratch
+void launch_dot(ccfd_gpu_ctx* c, const double* a, const double* b, double* out,
+                double* part);
+// host AoS (reference order) <-> device [4][Np] (renumbered)
+int upload_vec(ccfd_gpu_ctx* c, const double* h, double* d);
+int download_vec(ccfd_gpu_ctx* c, const double* d, double* h);
 // kernel kinds for timing spans
 enum { KT_DT = 0, KT_RECON = 1, KT_FLUX = 2, KT_OTHER = 3, KT_N = 4 };
 void tspan_begin(ccfd_gpu_ctx* c, hipEvent_t* e);

@@ -143,6 +143,22 @@ typedef struct ccfd_gpu_implicit_cfg {
 int ccfd_gpu_implicit_step(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
 		double t, double dt, double res[4], int *nNewton, int *nGMRES);
 
+/*
+ * GMRES_M (src/linearSolver.c:385-510) on a host right-hand side B (4 doubles
+ * per element, reference order) about the resident state: XK is the current
+ * conservative state and R_XK its dU/dt at `time`; LU-SGS built at tGlobal
+ * when cfg->precond. dX receives the update, *nIter the Krylov iterations.
+ */
+int ccfd_gpu_gmres(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
+		double tGlobal, double time, double dt, double alpha,
+		const double *B, double normB, double *dX, int *nIter);
+
+/* vectorDotProduct (src/linearSolver.c:115-128) of two host vectors (4
+ * doubles per element, reference order), summed on the device in a fixed
+ * order (deterministic, not the reference's OpenMP order) */
+int ccfd_gpu_dot(ccfd_gpu_ctx *ctx, const double *a, const double *b,
+		double *out);
+
 /* ---- flux functions in isolation ---- */
 /*
  * convectiveFlux (src/fluxCalculation.c:803-844) on n rotated state pairs

@@ -1,0 +1,111 @@
+"""Implicit path on the GPU (JFNK-GMRES + LU-SGS) against the oracle.
+
+The oracle restates implicitTimeStep / GMRES_M / buildMatrix / LUSGS with
+the reference's serial sums; the device sums dot products in a fixed tree
+and assembles the same Jacobian blocks by coloured differences, so the two
+agree to the Newton / GMRES tolerances, not bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+from ccfd_amd import gpu as G
+from ccfd_amd import mesh as M
+from oracle import bindings as B
+from oracle import timedisc as TD
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SOD_IMPLICIT = os.path.join(HERE, "golden", "calc", "sod_implicit.ini")
+
+
+def _run_pair(mesh, prm, cfg, prim, icfg_g, icfg_o, steps, ptime=1e9,
+              tend=1e9):
+    gs = G.Solver(mesh, prm)
+    orc = B.Oracle(mesh, cfg)
+    gs.set_prim(prim)
+    orc.set_prim(prim)
+    t = 0.0
+    ng = no = 0
+    for _ in range(steps):
+        dt, _ = gs.calc_dt(t, ptime, tend)
+        dto, _, _ = orc.calc_dt(t, ptime, tend)
+        # states agree to the Newton tolerance, so do the CFL time steps
+        assert abs(dt - dto) <= 1e-6 * abs(dto)
+        _, nn, gg = gs.implicit_step(icfg_g, t, dt)
+        rc, _, nno, ggo = orc.implicit_step(icfg_o, t, dto)
+        assert rc == 0
+        ng += nn
+        no += nno
+        t += dt
+    out = gs.cons(), orc.cons(), ng, no
+    gs.close()
+    return out
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_sod_implicit_matches_oracle(precond):
+    """calc/riemann/sod_implicit.ini (100 cells, HLLC, CFL 4.99, nKdim 5,
+    epsNewton 1e-6): ten backward-Euler steps"""
+    mesh, cfg, prim, c = TD.setup_case(SOD_IMPLICIT)
+    prm = G.params(gamma=c["gam"], flux=c["flux"], order=c["order"],
+                   limiter=c["limiter"], venk_k=c["venk_k"], cfl=c["cfl"],
+                   ts1d=int(c["ts1d"]), nrk=c["nRK"], torder=c["timeOrder"])
+    ig = G.ImplicitCfg(c["nKdim"], c["nNewtonIter"], precond, c["epsNewton"],
+                       c["epsGMRES"], c["gammaEW"])
+    io = B.ImplicitCfg(c["nKdim"], c["nNewtonIter"], precond, c["epsNewton"],
+                       c["epsGMRES"], c["gammaEW"])
+    ug, uo, ng, no = _run_pair(mesh, prm, cfg, prim, ig, io, 10,
+                               ptime=c["IOtimeInterval"], tend=c["tEnd"])
+    err = cases.rel_err(ug, uo)
+    print("precond=%d newton gpu %d oracle %d rel err %.2e" % (precond, ng, no, err))
+    assert abs(ng - no) <= 2
+    assert err < 1e-6
+
+
+def test_implicit_second_order_periodic():
+    mesh = cases.delaunay_mesh(20, seed=6, periodic=True)
+    prim = M.smooth_state(mesh.bary, seed=3)
+    prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0, cfl=5.0)
+    cfg = B.make_cfg(flux=2, order=2, limiter=2, venk_k=5.0, cfl=5.0)
+    ig = G.ImplicitCfg(10, 20, 1, 1e-6, 1e-3, 0.9)
+    io = B.ImplicitCfg(10, 20, 1, 1e-6, 1e-3, 0.9)
+    ug, uo, ng, no = _run_pair(mesh, prm, cfg, prim, ig, io, 3)
+    err = cases.rel_err(ug, uo)
+    print("2nd order: newton gpu %d oracle %d rel err %.2e" % (ng, no, err))
+    assert err < 1e-7
+
+
+def test_implicit_fp32_flux_close_to_fp64():
+    mesh = cases.delaunay_mesh(20, seed=6, periodic=True)
+    prim = M.smooth_state(mesh.bary, seed=3)
+    p64 = G.params(flux=5, order=2, limiter=2, venk_k=5.0, cfl=5.0)
+    p32 = G.params(flux=5, order=2, limiter=2, venk_k=5.0, cfl=5.0,
+                   precision=G.FP32_FLUX)
+    ig = G.ImplicitCfg(10, 20, 1, 1e-5, 1e-3, 0.9)
+    out = []
+    for p in (p64, p32):
+        gs = G.Solver(mesh, p)
+        gs.set_prim(prim)
+        dt, _ = gs.calc_dt(0.0, 1e9, 1e9)
+        for k in range(3):
+            gs.implicit_step(ig, k * dt, dt)
+        out.append(gs.cons())
+        gs.close()
+    err = cases.rel_err(out[1], out[0])
+    print("fp32 vs fp64 implicit rel err %.2e" % err)
+    assert err < 1e-4
+
+
+def test_newton_failure_is_reported():
+    """nNewtonIter = 1 cannot converge a large step: the status code says so
+    (the reference prints and exits, src/timeDiscretization.c:544-549)"""
+    mesh = cases.sod_mesh(100, 1)
+    gs = G.Solver(mesh, G.params(flux=5, order=1, cfl=50.0, ts1d=1))
+    gs.set_prim(cases.sod_state(mesh))
+    dt, _ = gs.calc_dt(0.0, 1.0, 1.0)
+    with pytest.raises(G.GpuError, match="Newton NOT converged"):
+        gs.implicit_step(G.ImplicitCfg(5, 1, 0, 1e-12, 1e-3, 0.9), 0.0, dt)
+    gs.close()
