@@ -10,8 +10,9 @@ nRKstages = 4 / timeOrder = 2, FP64. A step is one calcTimeStep plus four
 fvTimeDerivative + RK stage updates; the metric counts cell-stage updates.
 
 Prints one JSON line (rank 0) with the metric, a roofline entry for the
-dominant kernel (flux + gather + stage update) from HIP-event kernel timings
-over the timed region, and a CPU baseline: the reference's own OpenMP
+dominant kernel (the fused operator + stage update, or the flux + gather +
+stage update kernel of the split pair) from HIP-event kernel timings over the
+timed region, and a CPU baseline: the reference's own OpenMP
 fvTimeDerivative on the same mesh and state, timed on this host.
 """
 import argparse
@@ -31,9 +32,19 @@ PEAK_HBM_GBS = 8000.0
 # write primitive state 32); the dU/dt round trip is fused away.
 B_FLUX_FP64 = 296.0
 B_FLUX_FP32 = 186.0   # same with FP32 operator data (FP64 U0 / U)
-# whole stage (fvTimeDerivative + update), FP64 / FP32-flux, triangles + VK
+# whole stage (fvTimeDerivative + update), FP64 / FP32-flux, triangles + VK:
+# the canonical algorithmic bytes of the two-phase dataflow (limited
+# gradients materialised once), used for the fused kernel, which runs the
+# whole stage in one launch
 B_STAGE_FP64 = 572.0
 B_STAGE_FP32 = 330.0
+# what the fused kernel itself must move per element and stage (each array
+# once, side states in LDS): state 32 + links 12 + LSQ weights 48 + GP 48 +
+# eps^2 8 + side normals 48 + signed lengths 24 + 1/area 8 + U0 32 + U 32 +
+# primitive out 32 = 324 B (+32 B dU/dt on the last stage) in FP64; halo
+# re-reads come on top
+B_FUSED_MIN_FP64 = 324.0
+B_FUSED_MIN_FP32 = 200.0
 
 
 def log(*a):
@@ -135,6 +146,8 @@ def main():
     ap.add_argument("--flux", type=int, default=5)
     ap.add_argument("--venk-k", type=float, default=5.0)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--variant", choices=["auto", "split", "fused"],
+                    default="auto", help="operator implementation")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -143,8 +156,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the operator would shard); run with "
+        print("bench.py measures one GPU: there is no multi-GPU path (a "
+              "large mesh would shard as contiguous space-filling-curve "
+              "ranges with a 2-ring halo exchange per stage); run with "
               "--gpus 1", file=sys.stderr)
         sys.exit(2)
 
@@ -156,6 +170,9 @@ def main():
                    nrk=4, torder=2, cfl=0.9, precision=prec)
     t0 = time.time()
     solver = G.Solver(mesh, prm)
+    solver.set_variant({"auto": G.KV_AUTO, "split": G.KV_SPLIT,
+                        "fused": G.KV_FUSED}[args.variant])
+    fused = solver.variant() == G.KV_FUSED
     solver.set_prim(prim)
     log("upload %.1f s" % (time.time() - t0))
     big = 1e300
@@ -176,17 +193,28 @@ def main():
     value = N * stages / el
     ms_step = 1e3 * el / args.steps
 
-    bflux = B_FLUX_FP64 if prec == G.FP64 else B_FLUX_FP32
     bstage = B_STAGE_FP64 if prec == G.FP64 else B_STAGE_FP32
-    achieved = bflux * N / (ms["flux"] * 1e-3) / 1e9
-    traffic = pmc_traffic("flux_%s" % args.precision)
+    if fused:
+        bker = bstage
+        kname = ("fused_kernel (reconstruction + limiter + Riemann flux + "
+                 "gather + RK stage update, side states in LDS)")
+        kkey = "fused_%s" % args.precision
+    else:
+        bker = B_FLUX_FP64 if prec == G.FP64 else B_FLUX_FP32
+        kname = "flux_kernel (Riemann flux + gather + RK stage update)"
+        kkey = "flux_%s" % args.precision
+    achieved = bker * N / (ms["flux"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": traffic,
-            "kernel": "flux_kernel (Riemann flux + gather + RK stage update)",
+            "traffic": pmc_traffic(kkey),
+            "kernel": kname,
             "kernel_ms": round(ms["flux"], 4),
-            "bytes_per_cell": bflux}
+            "bytes_per_cell": bker}
+    if fused:
+        bmin = B_FUSED_MIN_FP64 if prec == G.FP64 else B_FUSED_MIN_FP32
+        roof["compulsory_bytes_per_cell"] = bmin
+        roof["compulsory_gbs"] = round(bmin * N / (ms["flux"] * 1e-3) / 1e9, 1)
 
     copy_gbs = None
     if args.copy_probe:
@@ -222,6 +250,7 @@ def main():
                         % (N, args.venk_k, args.precision),
             "cells": N, "stages_per_step": 4, "flux": args.flux,
             "precision": args.precision,
+            "operator": "fused" if fused else "split",
         },
         "hbm_gbs_stage_effective": round(value * bstage / 1e9, 1),
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},

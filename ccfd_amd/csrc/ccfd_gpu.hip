@@ -51,6 +51,43 @@ static const flux_fn_f kFluxF[12] = {
     ccfd_launch_flux_7_f, ccfd_launch_flux_8_f, ccfd_launch_flux_9_f,
     ccfd_launch_flux_10_f, ccfd_launch_flux_11_f};
 
+#define DECL_FUSED(k)                                                        \
+  void ccfd_launch_fused_##k##_d(const OpArgs<double>&, const FusedArgs&, int, \
+                                 int, size_t, hipStream_t);                   \
+  void ccfd_launch_fused_##k##_f(const OpArgs<float>&, const FusedArgs&, int,  \
+                                 int, size_t, hipStream_t);
+DECL_FUSED(1) DECL_FUSED(2) DECL_FUSED(3) DECL_FUSED(4) DECL_FUSED(5)
+DECL_FUSED(6) DECL_FUSED(7) DECL_FUSED(8) DECL_FUSED(9) DECL_FUSED(10)
+DECL_FUSED(11)
+
+typedef void (*fused_fn_d)(const OpArgs<double>&, const FusedArgs&, int, int,
+                           size_t, hipStream_t);
+typedef void (*fused_fn_f)(const OpArgs<float>&, const FusedArgs&, int, int,
+                           size_t, hipStream_t);
+static const fused_fn_d kFusedD[12] = {
+    nullptr, ccfd_launch_fused_1_d, ccfd_launch_fused_2_d, ccfd_launch_fused_3_d,
+    ccfd_launch_fused_4_d, ccfd_launch_fused_5_d, ccfd_launch_fused_6_d,
+    ccfd_launch_fused_7_d, ccfd_launch_fused_8_d, ccfd_launch_fused_9_d,
+    ccfd_launch_fused_10_d, ccfd_launch_fused_11_d};
+static const fused_fn_f kFusedF[12] = {
+    nullptr, ccfd_launch_fused_1_f, ccfd_launch_fused_2_f, ccfd_launch_fused_3_f,
+    ccfd_launch_fused_4_f, ccfd_launch_fused_5_f, ccfd_launch_fused_6_f,
+    ccfd_launch_fused_7_f, ccfd_launch_fused_8_f, ccfd_launch_fused_9_f,
+    ccfd_launch_fused_10_f, ccfd_launch_fused_11_f};
+
+template <>
+void ccfd::launch_fused_code<double>(int code, const OpArgs<double>& a,
+                                     const FusedArgs& f, int lim, int nb,
+                                     size_t lds, hipStream_t st) {
+  kFusedD[code](a, f, lim, nb, lds, st);
+}
+template <>
+void ccfd::launch_fused_code<float>(int code, const OpArgs<float>& a,
+                                    const FusedArgs& f, int lim, int nb,
+                                    size_t lds, hipStream_t st) {
+  kFusedF[code](a, f, lim, nb, lds, st);
+}
+
 template <>
 void ccfd::launch_flux_code<double>(int code, const OpArgs<double>& a, int nb,
                                     hipStream_t st) {
@@ -329,6 +366,10 @@ extern "C" int ccfd_gpu_create(ccfd_gpu_ctx** out, int device) {
     return fail(CCFD_EHIP, "hipSetDevice(%d) failed", device);
   ccfd_gpu_ctx* c = new ccfd_gpu_ctx();
   c->device = device;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            device) == hipSuccess && lds > 0)
+    c->ldsLimit = (size_t)lds;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(CCFD_EHIP, "stream creation failed");
@@ -499,6 +540,31 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
       bbd[o] = m->bbd[s];
     }
   }
+  // halo of each block for the fused operator: elements of other blocks
+  // that share a face with it, ascending
+  std::vector<int> haloStart(c->nb + 1, 0), haloCells;
+  {
+    std::vector<int> tmp;
+    int maxHalo = 0;
+    for (int b = 0; b < c->nb; ++b) {
+      tmp.clear();
+      const int lo = b * kBlock, hi = std::min(N, lo + kBlock);
+      for (int i = lo; i < hi; ++i)
+        for (int k = 0; k < S; ++k) {
+          const int l = link[(size_t)k * Np + i];
+          if (l < 0) continue;
+          const int j = l >> 2;
+          if (j < lo || j >= lo + kBlock) tmp.push_back(j);
+        }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      haloCells.insert(haloCells.end(), tmp.begin(), tmp.end());
+      haloStart[b + 1] = (int)haloCells.size();
+      maxHalo = std::max(maxHalo, (int)tmp.size());
+    }
+    c->tile = kBlock + maxHalo;
+    if (haloCells.empty()) haloCells.push_back(0);
+  }
   // ghosts (host order)
   std::vector<int> ghostBC(std::max(G, 1), 0);
   std::vector<int> ghostCell(std::max(G, 1), 0), ghostK(std::max(G, 1), 0);
@@ -539,7 +605,9 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
       (rc = upd(bary.data(), bary.size() * 8, (void**)&c->bary)) ||
       (rc = upd(area.data(), area.size() * 8, (void**)&c->area)) ||
       (rc = upd(sx.data(), sx.size() * 8, (void**)&c->sx)) ||
-      (rc = upd(sy.data(), sy.size() * 8, (void**)&c->sy)))
+      (rc = upd(sy.data(), sy.size() * 8, (void**)&c->sy)) ||
+      (rc = upd(haloStart.data(), haloStart.size() * 4, (void**)&c->haloStart)) ||
+      (rc = upd(haloCells.data(), haloCells.size() * 4, (void**)&c->haloCells)))
     return rc;
   if ((rc = upload_geom<double>(c, w, gp, fn, flen, bbv, bbd, areaq, venk)) ||
       (rc = upload_geom<float>(c, w, gp, fn, flen, bbv, bbd, areaq, venk)))
@@ -676,11 +744,36 @@ extern "C" int ccfd_gpu_get_ut(ccfd_gpu_ctx* c, double* ut) {
   return download_aos(c, as_double(c, c->ut, 4 * (size_t)c->Np), ut);
 }
 
+// limited gradients and side states of the current state at the time of
+// the last ccfd_gpu_fvtd call, into c->ux, c->uy, c->side (the fused
+// operator does not materialise them)
+template <typename T>
+static void recompute_recon(ccfd_gpu_ctx* c) {
+  OpArgs<T> a = op_args<T>(c);
+  a.time0 = c->lastTime;
+  if (c->G > 0)
+    hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
+                       0, c->st, a);
+  dim3 g(c->nb), b(kBlock);
+  const int lim = c->prm.limiter;
+  if (c->S == 3) {
+    if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1, 3>), g, b, 0, c->st, a);
+    else hipLaunchKernelGGL((recon_kernel<T, 2, 3>), g, b, 0, c->st, a);
+  } else {
+    if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1, 4>), g, b, 0, c->st, a);
+    else hipLaunchKernelGGL((recon_kernel<T, 2, 4>), g, b, 0, c->st, a);
+  }
+}
+
 extern "C" int ccfd_gpu_get_recon(ccfd_gpu_ctx* c, double* u_x, double* u_y,
                                   double* sideP) {
   if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
   hipSetDevice(c->device);
   int rc;
+  if (c->prm.spatialOrder == 2) {
+    if (c->prm.precision == CCFD_FP64) recompute_recon<double>(c);
+    else recompute_recon<float>(c);
+  }
   if (u_x && (rc = download_aos(c, as_double(c, c->ux, 4 * (size_t)c->Np), u_x))) return rc;
   if (u_y && (rc = download_aos(c, as_double(c, c->uy, 4 * (size_t)c->Np), u_y))) return rc;
   if (sideP) {
@@ -767,9 +860,39 @@ void ccfd::tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind) {
   c->spans.push_back({a, b, kind});
 }
 
+bool ccfd::use_fused(const ccfd_gpu_ctx* c, size_t tsize) {
+  if (c->variant == CCFD_KV_SPLIT) return false;
+  if (c->prm.spatialOrder != 2 || c->prm.navierstokes) return false;
+  return fused_lds_bytes(tsize, c->S, c->tile) + 256 <= c->ldsLimit;
+}
+
 template <typename T>
 void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (use_fused(c, sizeof(T))) {
+    if (c->G > 0) {
+      hipEvent_t eb = nullptr;
+      tspan_begin(c, &eb);
+      hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
+                         0, c->st, a);
+      const dim3 gg((c->G + 127) / 128), gb(128);
+      const int lim = c->prm.limiter;
+      if (c->S == 3) {
+        if (lim == 1) hipLaunchKernelGGL((bc_side_recon_kernel<T, 1, 3>), gg, gb, 0, c->st, a);
+        else hipLaunchKernelGGL((bc_side_recon_kernel<T, 2, 3>), gg, gb, 0, c->st, a);
+      } else {
+        if (lim == 1) hipLaunchKernelGGL((bc_side_recon_kernel<T, 1, 4>), gg, gb, 0, c->st, a);
+        else hipLaunchKernelGGL((bc_side_recon_kernel<T, 2, 4>), gg, gb, 0, c->st, a);
+      }
+      tspan_end(c, eb, KT_OTHER);
+    }
+    FusedArgs f{c->haloStart, c->haloCells, c->tile};
+    tspan_begin(c, &e1);
+    launch_fused_code<T>(c->prm.iFlux, a, f, c->prm.limiter, c->nb,
+                         fused_lds_bytes(sizeof(T), c->S, c->tile), c->st);
+    tspan_end(c, e1, KT_FLUX);
+    return;
+  }
   if (a.order == 2) {
     const int lim = c->prm.limiter;
     dim3 g(c->nb), b(kBlock);
@@ -815,6 +938,7 @@ static int fvtd_t(ccfd_gpu_ctx* c, double time) {
 extern "C" int ccfd_gpu_fvtd(ccfd_gpu_ctx* c, double time) {
   if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
   hipSetDevice(c->device);
+  c->lastTime = time;
   int rc = c->prm.precision == CCFD_FP64 ? fvtd_t<double>(c, time)
                                          : fvtd_t<float>(c, time);
   if (rc) return rc;
@@ -1147,6 +1271,24 @@ __global__ __launch_bounds__(256) void copy_probe_kernel(
   for (; i < n; i += st) dst[i] = src[i];
 }
 
+// the same copy with one 4 x 16 B tile per thread and one block per 16 KiB
+// (no grid-stride loop): all loads of a thread are issued before its stores
+__global__ __launch_bounds__(256) void copy_probe_tile_kernel(
+    const double2* __restrict__ src, double2* __restrict__ dst, long n) {
+  const long i0 = (long)blockIdx.x * 1024 + threadIdx.x;
+  double2 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long i = i0 + 256 * k;
+    if (i < n) v[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long i = i0 + 256 * k;
+    if (i < n) dst[i] = v[k];
+  }
+}
+
 extern "C" int ccfd_gpu_copy_probe(int device, size_t bytes, int reps,
                                    double* gbs) {
   if (bytes < 4096 || reps <= 0 || !gbs)
@@ -1160,24 +1302,59 @@ extern "C" int ccfd_gpu_copy_probe(int device, size_t bytes, int reps,
     return fail(CCFD_ENOMEM, "copy probe allocation failed");
   }
   hipMemset(a, 0, n * 16);
-  int blocks = 256 * 8 * 4;   // 8 waves per CU on 256 CUs, grid-stride loop
-  hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  // two launch shapes, the faster one counts: a grid-stride loop over
+  // 8 waves per CU on 256 CUs, and one 16 KiB tile per block
+  const int strideBlocks = 256 * 8 * 4;
+  const long tileBlocks = (n + 1023) / 1024;
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  hipEventRecord(e0, 0);
-  for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n);
-  hipEventRecord(e1, 0);
-  hipError_t err = hipEventSynchronize(e1);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
+  double best = 0.0;
+  hipError_t err = hipSuccess;
+  for (int shape = 0; shape < 2 && err == hipSuccess; ++shape) {
+    for (int r = 0; r <= reps; ++r) {   // first launch: warm-up
+      if (r == 1) hipEventRecord(e0, 0);
+      if (shape == 0)
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(strideBlocks), dim3(256), 0,
+                           0, a, b, n);
+      else
+        hipLaunchKernelGGL(copy_probe_tile_kernel, dim3(tileBlocks), dim3(256),
+                           0, 0, a, b, n);
+    }
+    hipEventRecord(e1, 0);
+    err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (err == hipSuccess && ms > 0.f)
+      best = std::max(best, 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9);
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipFree(a);
   hipFree(b);
   if (err != hipSuccess) return fail(CCFD_EHIP, "copy probe failed");
-  *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  *gbs = best;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_set_variant(ccfd_gpu_ctx* c, int variant) {
+  if (!c) return fail(CCFD_EINVAL, "null context");
+  if (variant < CCFD_KV_AUTO || variant > CCFD_KV_FUSED)
+    return fail(CCFD_EINVAL, "unknown kernel variant %d", variant);
+  c->variant = variant;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_get_variant(ccfd_gpu_ctx* c, int* variant) {
+  if (!c || !variant) return fail(CCFD_EINVAL, "null argument");
+  if (!c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
+  const size_t ts = c->prm.precision == CCFD_FP64 ? 8 : 4;
+  if (c->variant == CCFD_KV_FUSED && !use_fused(c, ts))
+    return fail(CCFD_ENOTSUP,
+                "fused operator needs 2nd order Euler and %zu bytes of LDS "
+                "(device limit %zu)",
+                fused_lds_bytes(ts, c->S, c->tile), c->ldsLimit);
+  *variant = use_fused(c, ts) ? CCFD_KV_FUSED : CCFD_KV_SPLIT;
   return CCFD_OK;
 }
 

@@ -7,7 +7,7 @@
 #include <vector>
 
 #include "ccfd_gpu.h"
-#include "operator.hpp"
+#include "fused.hpp"
 
 namespace ccfd {
 
@@ -70,6 +70,13 @@ struct ccfd_gpu_ctx {
   double* scal = nullptr;    // device scalars: t, dt, viscDom, nonfinite, res[4]
   double* hscal = nullptr;   // pinned host mirror of scal
   bool keepGradients = false;
+  double lastTime = 0.0;   // time of the last ccfd_gpu_fvtd call
+  // fused operator: per-block halo element lists and LDS tile columns
+  int* haloStart = nullptr;
+  int* haloCells = nullptr;
+  int tile = 0;
+  int variant = 0;          // ccfd_kernel_variant
+  size_t ldsLimit = 65536;  // per-block LDS of the device
   // optional per-kernel timing with HIP events on the context stream
   struct Span { hipEvent_t a, b; int kind; };
   bool timing = false;
@@ -106,5 +113,7 @@ int download_vec(ccfd_gpu_ctx* c, const double* d, double* h);
 enum { KT_DT = 0, KT_RECON = 1, KT_FLUX = 2, KT_OTHER = 3, KT_N = 4 };
 void tspan_begin(ccfd_gpu_ctx* c, hipEvent_t* e);
 void tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind);
+// does the next operator call of type T run the fused kernel
+bool use_fused(const ccfd_gpu_ctx* c, size_t tsize);
 
 }  // namespace ccfd

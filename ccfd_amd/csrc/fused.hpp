@@ -1,0 +1,256 @@
+// fused.hpp - one-kernel 2nd-order operator evaluation with the side states
+// held in LDS (Euler; ccfd fvTimeDerivative, src/finiteVolume.c:78-121).
+//
+// Each block owns kBlock consecutive elements of the space-filling-curve
+// order. Phase 1 reconstructs the limited side states (least-squares
+// gradient, limiter, p + GP.grad: src/reconstruction.c:198-311) of the owned
+// elements and of the block's halo - the elements outside the block that
+// share a face with it, listed per block at mesh set-up - into LDS. Phase 2
+// evaluates each owned element's side fluxes from LDS in the owning side's
+// orientation (src/fluxCalculation.c:884-997), gathers them in the
+// reference's side order and runs the optional RK stage / residual epilogue,
+// exactly as flux_kernel does. The side states never cross HBM: compared
+// with recon_kernel + flux_kernel this drops their write and both reads
+// (own and neighbour); halo elements are reconstructed by every block that
+// borders them, extra arithmetic the FP64 units absorb.
+//
+// The arithmetic is the same expression for expression as the two-kernel
+// path, so results are bit-identical to it. Ghost element states come from
+// bc_bary_kernel and ghost side states from bc_side_recon_kernel, both
+// launched before the fused kernel.
+#pragma once
+
+#include "operator.hpp"
+
+namespace ccfd {
+
+struct FusedArgs {
+  const int* __restrict__ haloStart;  // [nblocks + 1] into haloCells
+  const int* __restrict__ haloCells;  // per block, ascending element ids
+  int tile;                           // LDS columns: kBlock + max halo
+};
+
+// limited gradient (gx, gy) of element i, its state p, side offsets GP and
+// links (src/reconstruction.c:252-290)
+template <typename T, int LIM, int S>
+__device__ inline void limited_gradient(const OpArgs<T>& a, int i, T p[4],
+                                        int lk[S], T gx[4], T gy[4], T GPx[S],
+                                        T GPy[S]) {
+  const int Np = a.Np;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) p[v] = a.prim[v * Np + i];
+#pragma unroll
+  for (int s = 0; s < S; ++s) lk[s] = a.link[s * Np + i];
+  T umin[4], umax[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    gx[v] = gy[v] = T(0.0);
+    umin[v] = umax[v] = p[v];
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (lk[s] == kNoSide) continue;
+    T q[4];
+    if (lk[s] >= 0) {
+      const int j = lk[s] >> 2;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) q[v] = a.prim[v * Np + j];
+    } else {
+      const int g = -lk[s] - 1;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) q[v] = a.ghostP[v * a.G + g];
+    }
+    const T wx = a.w[s * Np + i], wy = a.w[(S + s) * Np + i];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const T d = q[v] - p[v];
+      gx[v] += wx * d;
+      gy[v] += wy * d;
+      umin[v] = tmin(umin[v], q[v]);
+      umax[v] = tmax(umax[v], q[v]);
+    }
+  }
+  const T eps = LIM == 2 ? a.venk[i] : T(0.0);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    GPx[s] = a.gp[s * Np + i];
+    GPy[s] = a.gp[(S + s) * Np + i];
+  }
+#pragma unroll 1
+  for (int v = 0; v < 4; ++v) {
+    const T dmin = umin[v] - p[v], dmax = umax[v] - p[v];
+    T phi = T(1.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (lk[s] == kNoSide) continue;
+      const T d = gx[v] * GPx[s] + gy[v] * GPy[s];
+      phi = tmin(phi, limiter_phi<LIM, T>(d, dmax, dmin, dmax * dmax,
+                                          dmin * dmin, eps));
+    }
+    gx[v] *= phi;
+    gy[v] *= phi;
+  }
+}
+
+// limited side states of element i into LDS column col; returns its links
+template <typename T, int LIM, int S>
+__device__ inline void fused_recon(const OpArgs<T>& a, int i, int col,
+                                   int tile, T* __restrict__ lds, int lk[S]) {
+  T p[4], gx[4], gy[4], GPx[S], GPy[S];
+  limited_gradient<T, LIM, S>(a, i, p, lk, gx, gy, GPx, GPy);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      lds[(v * S + s) * tile + col] = p[v] + GPx[s] * gx[v] + GPy[s] * gy[v];
+  }
+}
+
+// ghost side states for the fused operator (setBCatSides,
+// src/boundary.c:300-314): one thread per ghost side reconstructs its
+// interior element and applies the boundary condition at the side midpoint
+template <typename T, int LIM, int S>
+__global__ void bc_side_recon_kernel(OpArgs<T> a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G) return;
+  const int i = a.ghostCell[g], k = a.ghostK[g];
+  const int Np = a.Np;
+  T p[4], gx[4], gy[4], GPx[S], GPy[S];
+  int lk[S];
+  limited_gradient<T, LIM, S>(a, i, p, lk, gx, gy, GPx, GPy);
+  T in[4], out[4];
+  T gpx = GPx[0], gpy = GPy[0];
+#pragma unroll
+  for (int s = 1; s < S; ++s)
+    if (s == k) { gpx = GPx[s]; gpy = GPy[s]; }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) in[v] = p[v] + gpx * gx[v] + gpy * gy[v];
+  T x[2] = {gpx + T(a.bary[i]), gpy + T(a.bary[Np + i])};
+  bc_ghost<T>(a.eq, a.bct.bc[a.ghostBC[g]], a.ex, a.fn[k * Np + i],
+              a.fn[(S + k) * Np + i], op_time(a), in, out, x);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
+}
+
+template <int CODE, typename T, int LIM, int S>
+__global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
+                                                       FusedArgs f) {
+  extern __shared__ __align__(16) unsigned char fused_smem[];
+  T* lds = reinterpret_cast<T*>(fused_smem);
+  const int tile = f.tile;
+  int* hcell = reinterpret_cast<int*>(lds + 4 * S * tile);
+  const int lb = logical_block();
+  const int base = lb * kBlock;
+  const int i = base + threadIdx.x;
+  const bool active = i < a.N;
+  const int Np = a.Np;
+  const int h0 = f.haloStart[lb], nh = f.haloStart[lb + 1] - h0;
+
+  // phase 1: owned, then halo side states into LDS
+  int lk[S];
+  if (active) fused_recon<T, LIM, S>(a, i, threadIdx.x, tile, lds, lk);
+  for (int h = threadIdx.x; h < nh; h += kBlock) {
+    const int e = f.haloCells[h0 + h];
+    hcell[h] = e;
+    int lkh[S];
+    fused_recon<T, LIM, S>(a, e, kBlock + h, tile, lds, lkh);
+  }
+  __syncthreads();
+
+  // phase 2: side fluxes from LDS, gathered in side order
+  double r2[4] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    T acc[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+#pragma unroll 1
+    for (int s = 0; s < S; ++s) {
+      const int l = lk[s];
+      if (l == kNoSide) continue;
+      T own[4], oth[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) own[v] = lds[(v * S + s) * tile + threadIdx.x];
+      const T nx = a.fn[s * Np + i], ny = a.fn[(S + s) * Np + i];
+      const T fl = a.flen[s * Np + i];
+      const bool owner = fl > T(0.0);
+      const T len = owner ? fl : -fl;
+      if (l >= 0) {
+        const int j = l >> 2, ps = l & 3;
+        int col = j - base;
+        if (col < 0 || col >= kBlock) {
+          // halo column: lower bound of j in the block's ascending list
+          int lo = 0, hi = nh - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (hcell[mid] < j) lo = mid + 1; else hi = mid;
+          }
+          col = kBlock + lo;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) oth[v] = lds[(v * S + ps) * tile + col];
+      } else {
+        // ghost side state from bc_side_recon_kernel
+        const int g = -l - 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) oth[v] = a.ghostS[v * a.G + g];
+      }
+      const T* L = owner ? own : oth;
+      const T* R = owner ? oth : own;
+      T Lr[4] = {L[0], nx * L[1] + ny * L[2], -ny * L[1] + nx * L[2], L[3]};
+      T Rr[4] = {R[0], nx * R[1] + ny * R[2], -ny * R[1] + nx * R[2], R[3]};
+      T F[4];
+      conv_flux_t<CODE, T>(a.eq, Lr, Rr, F);
+      T fv[4];
+      fv[0] = F[0];
+      fv[1] = nx * F[1] - ny * F[2];
+      fv[2] = ny * F[1] + nx * F[2];
+      fv[3] = F[3];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        fv[v] *= len;
+        if (!owner) fv[v] = -fv[v];
+        acc[v] += fv[v];
+      }
+    }
+    const T aq = a.areaq[i];
+    T ut[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
+    if (a.write_ut) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
+    }
+    if (a.rk) {
+      const double dt = a.dtdev ? *a.dtdev : a.dt0;
+      const double c = a.rkc * dt;
+      double Uc[4], pn[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Uc[v] = a.U0[v * Np + i] + c * double(ut[v]);
+        a.U[v * Np + i] = Uc[v];
+      }
+      Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
+                       double(a.eq.mu), double(a.eq.Pr)};
+      cons_prim<double>(qd, Uc, pn);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
+    }
+    if (a.res_part) {
+      const double ar = a.area[i];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
+    }
+  }
+  if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
+}
+
+// dynamic LDS bytes of one fused block: side states of tile columns plus
+// the halo id list
+inline size_t fused_lds_bytes(size_t tsize, int S, int tile) {
+  return 4 * (size_t)S * tile * tsize + (size_t)(tile - kBlock) * 4;
+}
+
+// launch entry of one flux code (defined in op_flux.hip)
+template <typename T>
+void launch_fused_code(int code, const OpArgs<T>& a, const FusedArgs& f,
+                       int limiter, int nblocks, size_t lds, hipStream_t st);
+
+}  // namespace ccfd

@@ -1,8 +1,9 @@
 // op_flux.hip - instantiates the operator kernels of one Riemann flux.
 // Built once per flux code with -DFLUX_CODE=<1..11> so the 11 flux
 // specialisations compile in parallel; each object exports
-// ccfd_launch_flux_<code>_{d,f} (FP64 / FP32) used by ccfd_gpu.hip.
-#include "operator.hpp"
+// ccfd_launch_flux_<code>_{d,f} (two-kernel path, FP64 / FP32) and
+// ccfd_launch_fused_<code>_{d,f} (fused LDS path) used by ccfd_gpu.hip.
+#include "fused.hpp"
 
 #ifndef FLUX_CODE
 #error "compile with -DFLUX_CODE=<1..11>"
@@ -28,6 +29,25 @@ static void dispatch(const OpArgs<T>& a, int nb, hipStream_t st) {
   }
 }
 
+template <typename T, int LIM, int S>
+static void go_fused(const OpArgs<T>& a, const FusedArgs& f, int nb,
+                     size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((fused_kernel<FLUX_CODE, T, LIM, S>), dim3(nb),
+                     dim3(kBlock), lds, st, a, f);
+}
+
+template <typename T>
+static void dispatch_fused(const OpArgs<T>& a, const FusedArgs& f, int lim,
+                           int nb, size_t lds, hipStream_t st) {
+  if (a.S == 3) {
+    if (lim == 1) go_fused<T, 1, 3>(a, f, nb, lds, st);
+    else go_fused<T, 2, 3>(a, f, nb, lds, st);
+  } else {
+    if (lim == 1) go_fused<T, 1, 4>(a, f, nb, lds, st);
+    else go_fused<T, 2, 4>(a, f, nb, lds, st);
+  }
+}
+
 }  // namespace ccfd
 
 void CCFD_CAT(CCFD_CAT(ccfd_launch_flux_, FLUX_CODE), _d)(
@@ -38,4 +58,16 @@ void CCFD_CAT(CCFD_CAT(ccfd_launch_flux_, FLUX_CODE), _d)(
 void CCFD_CAT(CCFD_CAT(ccfd_launch_flux_, FLUX_CODE), _f)(
     const ccfd::OpArgs<float>& a, int nb, hipStream_t st) {
   ccfd::dispatch<float>(a, nb, st);
+}
+
+void CCFD_CAT(CCFD_CAT(ccfd_launch_fused_, FLUX_CODE), _d)(
+    const ccfd::OpArgs<double>& a, const ccfd::FusedArgs& f, int lim, int nb,
+    size_t lds, hipStream_t st) {
+  ccfd::dispatch_fused<double>(a, f, lim, nb, lds, st);
+}
+
+void CCFD_CAT(CCFD_CAT(ccfd_launch_fused_, FLUX_CODE), _f)(
+    const ccfd::OpArgs<float>& a, const ccfd::FusedArgs& f, int lim, int nb,
+    size_t lds, hipStream_t st) {
+  ccfd::dispatch_fused<float>(a, f, lim, nb, lds, st);
 }

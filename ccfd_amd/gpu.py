@@ -11,6 +11,8 @@ import numpy as np
 from . import _native as N
 
 FP64, FP32_FLUX = 0, 1
+# operator implementations (enum ccfd_kernel_variant)
+KV_AUTO, KV_SPLIT, KV_FUSED = 0, 1, 2
 
 
 class GpuError(RuntimeError):
@@ -87,6 +89,8 @@ def lib():
         "ccfd_gpu_riemann_batch": (C.c_int, [C.c_int, C.c_int, C.c_double,
                                              C.c_long, dp, dp, dp]),
         "ccfd_gpu_synchronize": (C.c_int, [vp]),
+        "ccfd_gpu_set_variant": (C.c_int, [vp, C.c_int]),
+        "ccfd_gpu_get_variant": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
                                              C.c_double, dp, ip]),
         "ccfd_gpu_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
@@ -136,6 +140,16 @@ class Solver:
                                        C.byref(prm)))
         self.N = mesh.n_elems
         self.S = mesh.n_slots
+
+    def set_variant(self, variant):
+        """Operator implementation: KV_AUTO, KV_SPLIT or KV_FUSED."""
+        check(self.L.ccfd_gpu_set_variant(self.h, variant))
+
+    def variant(self):
+        """The implementation the next operator call runs."""
+        v = C.c_int()
+        check(self.L.ccfd_gpu_get_variant(self.h, C.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):

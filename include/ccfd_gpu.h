@@ -95,11 +95,13 @@ int ccfd_gpu_get_cons(ccfd_gpu_ctx *ctx, double *cons);
 /* dU/dt of the last operator evaluation (elem->u_t) */
 int ccfd_gpu_get_ut(ccfd_gpu_ctx *ctx, double *ut);
 /*
- * Side effects of the last operator evaluation that host consumers read
+ * Side effects of the operator that host consumers read
  * (src/analyze.c:314-468): limited gradients u_x, u_y (4 per element) and
- * reconstructed side states (4 per slot, ccfd_mesh slot order). Any pointer
- * may be NULL. At 1st order gradients are zero and side states equal the
- * element state.
+ * reconstructed side states (4 per slot, ccfd_mesh slot order) of the
+ * current primitive state at the time of the last ccfd_gpu_fvtd call
+ * (recomputed on demand: the fused operator keeps them in LDS only). Any
+ * pointer may be NULL. At 1st order gradients are zero and side states
+ * equal the element state.
  */
 int ccfd_gpu_get_recon(ccfd_gpu_ctx *ctx, double *u_x, double *u_y,
 		double *sideP);
@@ -168,6 +170,24 @@ int ccfd_gpu_dot(ccfd_gpu_ctx *ctx, const double *a, const double *b,
 int ccfd_gpu_riemann_batch(int code, int precision, double gam, long n,
 		const double *L, const double *R, double *F);
 
+/* ---- kernel variants ---- */
+/*
+ * The 2nd-order Euler operator has two implementations with bit-identical
+ * results: CCFD_KV_SPLIT (reconstruction kernel writes side states to HBM,
+ * flux kernel reads them) and CCFD_KV_FUSED (one kernel per block keeps the
+ * side states of the block and its halo in LDS). CCFD_KV_AUTO (default)
+ * takes the fused kernel where it applies (2nd order, no viscous flux,
+ * LDS tile within the device limit) and the split pair elsewhere.
+ * ccfd_gpu_get_variant reports what the next operator call will run.
+ */
+enum ccfd_kernel_variant {
+	CCFD_KV_AUTO = 0,
+	CCFD_KV_SPLIT = 1,
+	CCFD_KV_FUSED = 2
+};
+int ccfd_gpu_set_variant(ccfd_gpu_ctx *ctx, int variant);
+int ccfd_gpu_get_variant(ccfd_gpu_ctx *ctx, int *variant);
+
 /* ---- timing helpers ---- */
 /*
  * Run nSteps device-driven steps (as ccfd_gpu_advance from t = 0) with HIP
@@ -180,9 +200,10 @@ int ccfd_gpu_profile_steps(ccfd_gpu_ctx *ctx, int nSteps, double pTime,
 		double stopTime, double *ms, int *launches);
 /*
  * Streaming copy of `bytes` between two device buffers, repeated `reps`
- * times; *gbs = (read + write bytes) / time. Reference for the HBM
- * bandwidth reachable on the device and calibration kernel for traffic
- * counters.
+ * times in each of two launch shapes (grid-stride loop; one 16 KiB tile per
+ * block); *gbs = (read + write bytes) / time of the faster shape. Reference
+ * for the HBM bandwidth reachable on the device and calibration kernel for
+ * traffic counters.
  */
 int ccfd_gpu_copy_probe(int device, size_t bytes, int reps, double *gbs);
 /* wait for all work queued on the context's stream */
