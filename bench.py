@@ -135,6 +135,136 @@ def pmc_traffic(kernel_key):
         return None
 
 
+def _density_err(a, b):
+    """relative L2 and Linf of the density of state a against state b"""
+    import numpy as np
+    d = a[:, 0] - b[:, 0]
+    return (float(np.linalg.norm(d) / np.linalg.norm(b[:, 0])),
+            float(np.abs(d).max() / np.abs(b[:, 0]).max()))
+
+
+def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
+    """Served steady solves: Poisson arrivals, continuous batching in an
+    FP32-flux arena with local time steps, retire on a 1e-6 density
+    residual drop (or the step cap); then the same requests one at a time in
+    FP64. Returns the summary with p50 / p99 latency, solves/s of both and
+    the density error of the FP32 results against the FP64 ones."""
+    import numpy as np
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+    from ccfd_amd import serve as SV
+    from ccfd_amd import workloads as W
+    specs = W.served_specs(n_req, seed=seed, cells=cells)
+    arr = SV.poisson_arrivals(n_req, rate, seed)
+
+    def requests(arrivals):
+        return [SV.Request(sp["id"], a, (lambda sp=sp: W.served_case(sp)), sp)
+                for sp, a in zip(specs, arrivals)]
+
+    def scheme(prec):
+        return G.params(flux=9, order=2, limiter=2, venk_k=5.0, nrk=4,
+                        torder=2, precision=prec)
+    out = {"requests": n_req, "cells_range": list(cells),
+           "arrival_rate_per_s": rate, "max_steps": max_steps,
+           "target_drop": 1e-6,
+           "scheme": "AUSMD, VK K=5, RK4 local time steps, FP32 flux / FP64 "
+                     "state; baseline FP64 one solve at a time"}
+    b = BT.Batch(scheme(G.FP32_FLUX), max_sides=4, local_dt=True,
+                 hbm_bytes=hbm_gb << 30)
+    srv = SV.Server(b, target_drop=1e-6, max_steps=max_steps,
+                    steps_per_poll=100, keep_state=baseline)
+    res = srv.run(requests(arr))
+    b.close()
+    out["served"] = SV.summary(res)
+    if baseline:
+        b64 = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
+                       hbm_bytes=hbm_gb << 30)
+        srv64 = SV.Server(b64, target_drop=1e-6, max_steps=max_steps,
+                          steps_per_poll=100, keep_state=True, max_active=1)
+        res64 = srv64.run(requests(np.zeros(n_req)))
+        b64.close()
+        out["one_at_a_time_fp64"] = SV.summary(res64)
+        errs = [_density_err(r["prim"], q["prim"]) for r, q in zip(res, res64)]
+        out["density_err_vs_fp64"] = {
+            "l2_max": max(e[0] for e in errs),
+            "l2_median": float(np.median([e[0] for e in errs])),
+            "linf_max": max(e[1] for e in errs)}
+        out["speedup_solves_per_s"] = (out["served"]["solves_per_s"] /
+                                       out["one_at_a_time_fp64"]["solves_per_s"])
+    return out
+
+
+def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
+    """Batched explicit Euler: n_inst seeded jittered Delaunay meshes (cells
+    log-uniform in the range), 4-quadrant Riemann states, HLLC + BJ, RK3
+    (nRKstages=3, timeOrder=2), CFL 0.9, transmissive boundaries, FP32 flux
+    with FP64 state, all in one arena with graph-captured steps and the
+    autotuner's operator; then every instance one at a time on the plain
+    FP64 single-mesh path."""
+    import concurrent.futures as cf
+    import numpy as np
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+    from ccfd_amd import workloads as W
+    specs = W.batched_specs(n_inst, seed=4, cells=cells)
+    t0 = time.time()
+    with cf.ProcessPoolExecutor(min(16, os.cpu_count() or 4)) as ex:
+        raws = list(ex.map(W.batched_raw, specs, chunksize=2))
+    with cf.ThreadPoolExecutor(8) as ex:
+        cases = list(ex.map(lambda sr: W.batched_case(*sr), zip(specs, raws)))
+    del raws
+    ncell = sum(m.n_elems for m, _ in cases)
+    log("batched: %d meshes, %d cells, built in %.1f s"
+        % (len(cases), ncell, time.time() - t0))
+    sch32 = G.params(flux=5, order=2, limiter=1, nrk=3, torder=2, cfl=0.9,
+                     precision=G.FP32_FLUX)
+    sch64 = G.params(flux=5, order=2, limiter=1, nrk=3, torder=2, cfl=0.9)
+    b = BT.Batch(sch32, max_sides=3, hbm_bytes=hbm_gb << 30,
+                 max_instances=max(64, n_inst))
+    t0 = time.time()
+    ids = [b.add(m, p) for m, p in cases]
+    log("admitted in %.1f s" % (time.time() - t0))
+    variant, ms_var = b.autotune(6)
+    b.sync()
+    w0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    el = time.perf_counter() - w0
+    out = {"instances": len(ids), "cells": ncell, "steps": steps,
+           "cells_range": list(cells),
+           "scheme": "HLLC, Barth-Jespersen, RK3 (3 stages, order 2), CFL "
+                     "0.9, FP32 flux / FP64 state",
+           "autotune_ms_per_step": {"split": ms_var[1], "fused": ms_var[2],
+                                    "fused_face": ms_var[3]},
+           "variant": {1: "split", 2: "fused", 3: "fused_face"}[variant],
+           "seconds": el,
+           "cell_stage_updates_per_s": ncell * 3 * steps / el,
+           "solves_per_s": len(ids) / el}
+    if baseline:
+        tot, errs = 0.0, []
+        for k, (m, p) in zip(ids, cases):
+            s = G.Solver(m, sch64)
+            s.set_prim(p)
+            s.sync()
+            w0 = time.perf_counter()
+            s.advance(steps, 0.0, 1e300, 1e300, residual=True)
+            s.sync()
+            tot += time.perf_counter() - w0
+            errs.append(_density_err(b.prim(k), s.prim()))
+            s.close()
+        out["one_at_a_time_fp64"] = {"seconds": tot,
+                                     "solves_per_s": len(ids) / tot,
+                                     "cell_stage_updates_per_s":
+                                         ncell * 3 * steps / tot}
+        out["speedup_solves_per_s"] = out["solves_per_s"] / (len(ids) / tot)
+        out["density_err_vs_fp64"] = {
+            "l2_max": max(e[0] for e in errs),
+            "l2_median": float(np.median([e[0] for e in errs])),
+            "linf_max": max(e[1] for e in errs)}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,13 +276,28 @@ def main():
     ap.add_argument("--flux", type=int, default=5)
     ap.add_argument("--venk-k", type=float, default=5.0)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--variant", choices=["auto", "split", "fused"],
+    ap.add_argument("--variant", choices=["auto", "split", "fused",
+                                          "fused_face"],
                     default="auto", help="operator implementation")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 1 GiB streaming copy (reachable HBM "
                          "bandwidth; counter calibration under rocprofv3)")
+    ap.add_argument("--workload", choices=["operator", "batched", "served"],
+                    default="operator",
+                    help="operator: the 4M-triangle stage benchmark (plus a "
+                         "short served sample unless --no-served); batched: "
+                         "many meshes in one arena; served: the request loop")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--min-cells", type=float, default=None)
+    ap.add_argument("--max-cells", type=float, default=None)
+    ap.add_argument("--max-steps", type=int, default=20000)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--hbm-gb", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -164,6 +309,40 @@ def main():
 
     from ccfd_amd import gpu as G
 
+    if args.workload == "batched":
+        cells = (args.min_cells or 2e3, args.max_cells or 5e5)
+        r = batched_run(args.instances, cells, args.batch_steps, args.hbm_gb)
+        out = {"metric": "batched cell-stage updates/s (many meshes, one "
+                         "launch per stage)",
+               "value": r["cell_stage_updates_per_s"],
+               "unit": "cell-stage updates/s", "n_gpus": 1,
+               "steps": args.batch_steps, "warmup": 0,
+               "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32 flux / f64 state",
+               "data": "synthetic (seeded Delaunay meshes, 4-quadrant "
+                       "Riemann states)",
+               "config": {"workload": "%d instances, %d cells, %d steps"
+                          % (r["instances"], r["cells"], r["steps"])},
+               "batched": r}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "served":
+        cells = (args.min_cells or 1e4, args.max_cells or 2e6)
+        r = served_run(args.requests, cells, args.rate, args.max_steps,
+                       args.hbm_gb)
+        out = {"metric": "served steady solves/s (p50/p99 latency)",
+               "value": r["served"]["solves_per_s"], "unit": "solves/s",
+               "n_gpus": 1, "steps": None, "warmup": 0,
+               "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32 flux / f64 state",
+               "data": "synthetic (channel-with-bump and NACA O-grid "
+                       "requests, Poisson arrivals)",
+               "config": {"workload": "%d requests, cells %g..%g, rate %g/s"
+                          % (args.requests, cells[0], cells[1], args.rate)},
+               "served": r}
+        print(json.dumps(out), flush=True)
+        return
+
     raw, bcs, mesh, prim = build_case(args.n, args.seed)
     prec = G.FP64 if args.precision == "fp64" else G.FP32_FLUX
     prm = G.params(flux=args.flux, order=2, limiter=2, venk_k=args.venk_k,
@@ -171,8 +350,10 @@ def main():
     t0 = time.time()
     solver = G.Solver(mesh, prm)
     solver.set_variant({"auto": G.KV_AUTO, "split": G.KV_SPLIT,
-                        "fused": G.KV_FUSED}[args.variant])
-    fused = solver.variant() == G.KV_FUSED
+                        "fused": G.KV_FUSED,
+                        "fused_face": G.KV_FUSED_FACE}[args.variant])
+    kv = solver.variant()
+    fused = kv in (G.KV_FUSED, G.KV_FUSED_FACE)
     solver.set_prim(prim)
     log("upload %.1f s" % (time.time() - t0))
     big = 1e300
@@ -196,9 +377,12 @@ def main():
     bstage = B_STAGE_FP64 if prec == G.FP64 else B_STAGE_FP32
     if fused:
         bker = bstage
-        kname = ("fused_kernel (reconstruction + limiter + Riemann flux + "
-                 "gather + RK stage update, side states in LDS)")
-        kkey = "fused_%s" % args.precision
+        kname = ("%s (reconstruction + limiter + Riemann flux + gather + "
+                 "RK stage update, side states in LDS)"
+                 % ("fused_face_kernel" if kv == G.KV_FUSED_FACE
+                    else "fused_kernel"))
+        kkey = "%s_%s" % ("fused_face" if kv == G.KV_FUSED_FACE else "fused",
+                          args.precision)
     else:
         bker = B_FLUX_FP64 if prec == G.FP64 else B_FLUX_FP32
         kname = "flux_kernel (Riemann flux + gather + RK stage update)"
@@ -250,7 +434,8 @@ def main():
                         % (N, args.venk_k, args.precision),
             "cells": N, "stages_per_step": 4, "flux": args.flux,
             "precision": args.precision,
-            "operator": "fused" if fused else "split",
+            "operator": {G.KV_SPLIT: "split", G.KV_FUSED: "fused",
+                         G.KV_FUSED_FACE: "fused_face"}[kv],
         },
         "hbm_gbs_stage_effective": round(value * bstage / 1e9, 1),
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
@@ -258,8 +443,18 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     solver.close()
+    del solver
+    if not args.no_served:
+        # a short served sample beside the stage benchmark
+        try:
+            out["served_sample"] = served_run(
+                24, (args.min_cells or 1e4, args.max_cells or 6e4), 40.0,
+                min(args.max_steps, 8000), 16)
+        except Exception as e:   # must not hide the main result
+            log("served sample failed:", e)
+            out["served_sample"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

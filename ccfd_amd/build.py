@@ -21,7 +21,8 @@ INC = os.path.join(ROOT, "include")
 ORACLE = os.path.join(ROOT, "oracle")
 
 HOST_SRC = ["mesh.c", "ini.c", "driver.c"]
-GPU_SRC = ["ccfd_gpu.hip", "implicit.hip"]
+GPU_SRC = ["ccfd_gpu.hip", "implicit.hip", "batch.hip"]
+HOST_CXX_SRC = ["soa.cpp"]   # host C++ linked into libccfd_gpu.so
 FLUX_CODES = list(range(1, 12))
 OBJ = os.path.join(ROOT, "_build")
 
@@ -82,9 +83,17 @@ def build_gpu(jobs=None):
                       os.path.join(OBJ, "op_flux_%d.o" % k),
                       ["-DFLUX_CODE=%d" % k]))
 
+    for s in HOST_CXX_SRC:
+        units.append((os.path.join(CSRC, s),
+                      os.path.join(OBJ, s.replace(".cpp", ".o")), None))
+
     def comp(u):
         src, obj, extra = u
-        _run([hipcc] + HIP_FLAGS + inc + extra + ["-c", src, "-o", obj])
+        if extra is None:   # host-only C++
+            _run(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off"] +
+                 inc + ["-c", src, "-o", obj])
+        else:
+            _run([hipcc] + HIP_FLAGS + inc + extra + ["-c", src, "-o", obj])
         return obj
 
     jobs = jobs or min(8, os.cpu_count() or 4)

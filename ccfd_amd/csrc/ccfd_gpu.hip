@@ -15,6 +15,7 @@
 
 #include "ccfd_gpu.h"
 #include "ctx.hpp"
+#include "soa.hpp"
 
 using namespace ccfd;
 
@@ -53,17 +54,17 @@ static const flux_fn_f kFluxF[12] = {
 
 #define DECL_FUSED(k)                                                        \
   void ccfd_launch_fused_##k##_d(const OpArgs<double>&, const FusedArgs&, int, \
-                                 int, size_t, hipStream_t);                   \
+                                 int, int, size_t, hipStream_t);              \
   void ccfd_launch_fused_##k##_f(const OpArgs<float>&, const FusedArgs&, int,  \
-                                 int, size_t, hipStream_t);
+                                 int, int, size_t, hipStream_t);
 DECL_FUSED(1) DECL_FUSED(2) DECL_FUSED(3) DECL_FUSED(4) DECL_FUSED(5)
 DECL_FUSED(6) DECL_FUSED(7) DECL_FUSED(8) DECL_FUSED(9) DECL_FUSED(10)
 DECL_FUSED(11)
 
 typedef void (*fused_fn_d)(const OpArgs<double>&, const FusedArgs&, int, int,
-                           size_t, hipStream_t);
+                           int, size_t, hipStream_t);
 typedef void (*fused_fn_f)(const OpArgs<float>&, const FusedArgs&, int, int,
-                           size_t, hipStream_t);
+                           int, size_t, hipStream_t);
 static const fused_fn_d kFusedD[12] = {
     nullptr, ccfd_launch_fused_1_d, ccfd_launch_fused_2_d, ccfd_launch_fused_3_d,
     ccfd_launch_fused_4_d, ccfd_launch_fused_5_d, ccfd_launch_fused_6_d,
@@ -77,15 +78,15 @@ static const fused_fn_f kFusedF[12] = {
 
 template <>
 void ccfd::launch_fused_code<double>(int code, const OpArgs<double>& a,
-                                     const FusedArgs& f, int lim, int nb,
-                                     size_t lds, hipStream_t st) {
-  kFusedD[code](a, f, lim, nb, lds, st);
+                                     const FusedArgs& f, int lim, int face,
+                                     int nb, size_t lds, hipStream_t st) {
+  kFusedD[code](a, f, lim, face, nb, lds, st);
 }
 template <>
 void ccfd::launch_fused_code<float>(int code, const OpArgs<float>& a,
-                                    const FusedArgs& f, int lim, int nb,
-                                    size_t lds, hipStream_t st) {
-  kFusedF[code](a, f, lim, nb, lds, st);
+                                    const FusedArgs& f, int lim, int face,
+                                    int nb, size_t lds, hipStream_t st) {
+  kFusedF[code](a, f, lim, face, nb, lds, st);
 }
 
 template <>
@@ -261,46 +262,6 @@ __global__ void k_riemann(int code, long n, const double* __restrict__ L,
 
 // ------------------------------------------------------------ host helpers
 
-// Hilbert curve index of (x, y) on an n x n grid (n a power of two)
-static uint32_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
-  uint32_t d = 0;
-  for (uint32_t s = n / 2; s > 0; s /= 2) {
-    uint32_t rx = (x & s) > 0;
-    uint32_t ry = (y & s) > 0;
-    d += s * s * ((3 * rx) ^ ry);
-    if (ry == 0) {
-      if (rx == 1) {
-        x = n - 1 - x;
-        y = n - 1 - y;
-      }
-      uint32_t t = x; x = y; y = t;
-    }
-  }
-  return d;
-}
-
-void ccfd::sfc_order(const ccfd_mesh* m, std::vector<int>& perm) {
-  long N = m->nElems;
-  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
-  for (long e = 0; e < N; ++e) {
-    x0 = std::min(x0, m->bary[2 * e]); x1 = std::max(x1, m->bary[2 * e]);
-    y0 = std::min(y0, m->bary[2 * e + 1]); y1 = std::max(y1, m->bary[2 * e + 1]);
-  }
-  double span = std::max(x1 - x0, y1 - y0);
-  if (!(span > 0)) span = 1.0;
-  std::vector<std::pair<uint32_t, int>> key(N);
-  for (long e = 0; e < N; ++e) {
-    uint32_t ix = (uint32_t)std::min(65535.0, (m->bary[2 * e] - x0) / span * 65535.0);
-    uint32_t iy = (uint32_t)std::min(65535.0, (m->bary[2 * e + 1] - y0) / span * 65535.0);
-    key[e] = {hilbert_index(65536u, ix, iy), (int)e};
-  }
-  std::stable_sort(key.begin(), key.end(),
-                   [](const std::pair<uint32_t, int>& a,
-                      const std::pair<uint32_t, int>& b) { return a.first < b.first; });
-  perm.resize(N);
-  for (long k = 0; k < N; ++k) perm[k] = key[k].second;
-}
-
 // -------------------------------------------------------- context helpers
 
 void* ccfd::dmalloc(ccfd_gpu_ctx* c, size_t bytes) {
@@ -396,7 +357,7 @@ extern "C" int ccfd_gpu_destroy(ccfd_gpu_ctx* c) {
   return CCFD_OK;
 }
 
-int ccfd::apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p) {
+int ccfd::check_params(const ccfd_gpu_params* p, double rk[6]) {
   if (p->iFlux < 1 || p->iFlux > 11)
     return fail(CCFD_EINVAL, "unknown flux function %d", p->iFlux);
   if (p->spatialOrder != 1 && p->spatialOrder != 2)
@@ -405,7 +366,7 @@ int ccfd::apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p) {
     return fail(CCFD_EINVAL, "limiter must be 1 or 2");
   if (p->precision != CCFD_FP64 && p->precision != CCFD_FP32_FLUX)
     return fail(CCFD_EINVAL, "unknown precision %d", p->precision);
-  double rk[6] = {0};
+  for (int k = 0; k < 6; ++k) rk[k] = 0.0;
   if (p->nRKstages == 1 && p->timeOrder == 1) {
     rk[1] = 1.0;
   } else {
@@ -426,6 +387,13 @@ int ccfd::apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p) {
     else if (p->nRKstages == 5) for (int k = 0; k < 5; ++k) rk[k + 1] = t5[o - 1][k];
     else return fail(CCFD_EINVAL, "wrong number of RK stages %d", p->nRKstages);
   }
+  return CCFD_OK;
+}
+
+int ccfd::apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p) {
+  double rk[6];
+  int rc = check_params(p, rk);
+  if (rc) return rc;
   c->prm = *p;
   memcpy(c->rk, rk, sizeof(rk));
   c->ex.refState[0][0] = p->refState[0][0];
@@ -493,92 +461,23 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   c->totalArea_q = m->totalArea_q;
   c->nSlotsHost = m->nSlots;
 
-  sfc_order(m, c->perm);
-  c->inv.assign(N, 0);
-  for (int k = 0; k < N; ++k) c->inv[c->perm[k]] = k;
+  MeshSoA h;
+  build_soa(m, S, 0, 0, 0, h);
+  c->perm = h.perm;
+  c->inv = h.inv;
   c->slotStart.assign(m->slotStart, m->slotStart + N + 1);
-
-  // slot arrays [c][S][Np], element arrays [Np]
-  size_t sl = (size_t)S * Np;
-  std::vector<int> link(sl, kNoSide);
-  std::vector<double> w(2 * sl, 0.0), gp(2 * sl, 0.0), fn(2 * sl, 0.0),
-      flen(sl, 1.0), bbv(2 * sl, 0.0), bbd(sl, 1.0);
-  std::vector<double> areaq(Np, 0.0), area(Np, 0.0), sx(Np, 1.0), sy(Np, 1.0),
-      bary(2 * (size_t)Np, 0.0), venk(Np, 0.0);
-  c->hArea.assign(N, 0.0);
-  for (int i = 0; i < N; ++i) {
-    int e = c->perm[i];
-    areaq[i] = m->areaq[e];
-    area[i] = m->area[e];
-    sx[i] = m->sx[e];
-    sy[i] = m->sy[e];
-    bary[i] = m->bary[2 * e];
-    bary[Np + i] = m->bary[2 * e + 1];
-    c->hArea[i] = m->area[e];
-    long s0 = m->slotStart[e];
-    int ns = (int)(m->slotStart[e + 1] - s0);
-    for (int k = 0; k < ns; ++k) {
-      long s = s0 + k;
-      size_t o = (size_t)k * Np + i;
-      long nb = m->nbr[s];
-      if (nb >= 0) {
-        long ps = m->partner[s] - m->slotStart[nb];
-        link[o] = (c->inv[nb] << 2) | (int)ps;
-      } else {
-        link[o] = (int)nb;   // -(g + 1)
-      }
-      w[o] = m->w[2 * s];
-      w[sl + o] = m->w[2 * s + 1];
-      gp[o] = m->GP[2 * s];
-      gp[sl + o] = m->GP[2 * s + 1];
-      fn[o] = m->fn[2 * s];
-      fn[sl + o] = m->fn[2 * s + 1];
-      bool own = m->owner[s] || nb < 0;
-      flen[o] = own ? m->flen[s] : -m->flen[s];
-      bbv[o] = m->bbv[2 * s];
-      bbv[sl + o] = m->bbv[2 * s + 1];
-      bbd[o] = m->bbd[s];
-    }
-  }
-  // halo of each block for the fused operator: elements of other blocks
-  // that share a face with it, ascending
-  std::vector<int> haloStart(c->nb + 1, 0), haloCells;
-  {
-    std::vector<int> tmp;
-    int maxHalo = 0;
-    for (int b = 0; b < c->nb; ++b) {
-      tmp.clear();
-      const int lo = b * kBlock, hi = std::min(N, lo + kBlock);
-      for (int i = lo; i < hi; ++i)
-        for (int k = 0; k < S; ++k) {
-          const int l = link[(size_t)k * Np + i];
-          if (l < 0) continue;
-          const int j = l >> 2;
-          if (j < lo || j >= lo + kBlock) tmp.push_back(j);
-        }
-      std::sort(tmp.begin(), tmp.end());
-      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-      haloCells.insert(haloCells.end(), tmp.begin(), tmp.end());
-      haloStart[b + 1] = (int)haloCells.size();
-      maxHalo = std::max(maxHalo, (int)tmp.size());
-    }
-    c->tile = kBlock + maxHalo;
-    if (haloCells.empty()) haloCells.push_back(0);
-  }
-  // ghosts (host order)
-  std::vector<int> ghostBC(std::max(G, 1), 0);
-  std::vector<int> ghostCell(std::max(G, 1), 0), ghostK(std::max(G, 1), 0);
+  c->hArea.assign(h.area.begin(), h.area.begin() + N);
+  c->tile = kBlock + h.maxHalo;
+  if (h.haloCells.empty()) h.haloCells.push_back(0);
+  std::vector<int> ghostBC(std::max(G, 1), 0), ghostCell(std::max(G, 1), 0),
+      ghostK(std::max(G, 1), 0);
   std::vector<double> ghostBary(2 * (size_t)std::max(G, 1), 0.0);
-  for (int g = 0; g < G; ++g) {
-    long s = m->ghostSlot[g];
-    long e = std::upper_bound(m->slotStart, m->slotStart + N + 1, s) -
-             m->slotStart - 1;
-    ghostCell[g] = c->inv[e];
-    ghostK[g] = (int)(s - m->slotStart[e]);
-    ghostBC[g] = m->ghostBC[g];
-    ghostBary[g] = m->ghostBary[2 * g];
-    ghostBary[G + g] = m->ghostBary[2 * g + 1];
-  }
+  std::copy(h.ghostBC.begin(), h.ghostBC.end(), ghostBC.begin());
+  std::copy(h.ghostCell.begin(), h.ghostCell.end(), ghostCell.begin());
+  std::copy(h.ghostK.begin(), h.ghostK.end(), ghostK.begin());
+  std::copy(h.ghostBary.begin(), h.ghostBary.end(), ghostBary.begin());
+  std::vector<double> venk(Np, 0.0);
+  const size_t sl = (size_t)S * Np;
   for (int b = 0; b < m->nBC; ++b) {
     c->bct.bc[b].type = m->bc[b].type;
     c->bct.bc[b].exactFunc = m->bc[b].exactFunc;
@@ -597,20 +496,20 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
     return hipStreamSynchronize(c->st) == hipSuccess ? 0
                                                      : fail(CCFD_EHIP, "upload sync failed");
   };
-  if ((rc = upd(link.data(), link.size() * 4, (void**)&c->link)) ||
+  if ((rc = upd(h.link.data(), h.link.size() * 4, (void**)&c->link)) ||
       (rc = upd(ghostBC.data(), ghostBC.size() * 4, (void**)&c->ghostBC)) ||
       (rc = upd(ghostCell.data(), ghostCell.size() * 4, (void**)&c->ghostCell)) ||
       (rc = upd(ghostK.data(), ghostK.size() * 4, (void**)&c->ghostK)) ||
       (rc = upd(ghostBary.data(), ghostBary.size() * 8, (void**)&c->ghostBary)) ||
-      (rc = upd(bary.data(), bary.size() * 8, (void**)&c->bary)) ||
-      (rc = upd(area.data(), area.size() * 8, (void**)&c->area)) ||
-      (rc = upd(sx.data(), sx.size() * 8, (void**)&c->sx)) ||
-      (rc = upd(sy.data(), sy.size() * 8, (void**)&c->sy)) ||
-      (rc = upd(haloStart.data(), haloStart.size() * 4, (void**)&c->haloStart)) ||
-      (rc = upd(haloCells.data(), haloCells.size() * 4, (void**)&c->haloCells)))
+      (rc = upd(h.bary.data(), h.bary.size() * 8, (void**)&c->bary)) ||
+      (rc = upd(h.area.data(), h.area.size() * 8, (void**)&c->area)) ||
+      (rc = upd(h.sx.data(), h.sx.size() * 8, (void**)&c->sx)) ||
+      (rc = upd(h.sy.data(), h.sy.size() * 8, (void**)&c->sy)) ||
+      (rc = upd(h.haloStart.data(), h.haloStart.size() * 4, (void**)&c->haloStart)) ||
+      (rc = upd(h.haloCells.data(), h.haloCells.size() * 4, (void**)&c->haloCells)))
     return rc;
-  if ((rc = upload_geom<double>(c, w, gp, fn, flen, bbv, bbd, areaq, venk)) ||
-      (rc = upload_geom<float>(c, w, gp, fn, flen, bbv, bbd, areaq, venk)))
+  if ((rc = upload_geom<double>(c, h.w, h.gp, h.fn, h.flen, h.bbv, h.bbd, h.areaq, venk)) ||
+      (rc = upload_geom<float>(c, h.w, h.gp, h.fn, h.flen, h.bbv, h.bbd, h.areaq, venk)))
     return rc;
 
   // state buffers (sized for double, used as T)
@@ -860,16 +759,22 @@ void ccfd::tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind) {
   c->spans.push_back({a, b, kind});
 }
 
-bool ccfd::use_fused(const ccfd_gpu_ctx* c, size_t tsize) {
-  if (c->variant == CCFD_KV_SPLIT) return false;
+bool ccfd::fused_applies(const ccfd_gpu_ctx* c, size_t tsize) {
   if (c->prm.spatialOrder != 2 || c->prm.navierstokes) return false;
   return fused_lds_bytes(tsize, c->S, c->tile) + 256 <= c->ldsLimit;
+}
+
+int ccfd::active_variant(const ccfd_gpu_ctx* c, size_t tsize) {
+  if (c->variant == CCFD_KV_SPLIT || !fused_applies(c, tsize))
+    return CCFD_KV_SPLIT;
+  return c->variant == CCFD_KV_AUTO ? CCFD_KV_FUSED_FACE : c->variant;
 }
 
 template <typename T>
 void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (use_fused(c, sizeof(T))) {
+  const int kv = active_variant(c, sizeof(T));
+  if (kv != CCFD_KV_SPLIT) {
     if (c->G > 0) {
       hipEvent_t eb = nullptr;
       tspan_begin(c, &eb);
@@ -886,9 +791,10 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
       }
       tspan_end(c, eb, KT_OTHER);
     }
-    FusedArgs f{c->haloStart, c->haloCells, c->tile};
+    FusedArgs f{c->haloStart, c->haloStart + 1, c->haloCells, c->tile};
     tspan_begin(c, &e1);
-    launch_fused_code<T>(c->prm.iFlux, a, f, c->prm.limiter, c->nb,
+    launch_fused_code<T>(c->prm.iFlux, a, f, c->prm.limiter,
+                         kv == CCFD_KV_FUSED_FACE ? 1 : 0, c->nb,
                          fused_lds_bytes(sizeof(T), c->S, c->tile), c->st);
     tspan_end(c, e1, KT_FLUX);
     return;
@@ -1339,7 +1245,7 @@ extern "C" int ccfd_gpu_copy_probe(int device, size_t bytes, int reps,
 
 extern "C" int ccfd_gpu_set_variant(ccfd_gpu_ctx* c, int variant) {
   if (!c) return fail(CCFD_EINVAL, "null context");
-  if (variant < CCFD_KV_AUTO || variant > CCFD_KV_FUSED)
+  if (variant < CCFD_KV_AUTO || variant > CCFD_KV_FUSED_FACE)
     return fail(CCFD_EINVAL, "unknown kernel variant %d", variant);
   c->variant = variant;
   return CCFD_OK;
@@ -1349,12 +1255,13 @@ extern "C" int ccfd_gpu_get_variant(ccfd_gpu_ctx* c, int* variant) {
   if (!c || !variant) return fail(CCFD_EINVAL, "null argument");
   if (!c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
   const size_t ts = c->prm.precision == CCFD_FP64 ? 8 : 4;
-  if (c->variant == CCFD_KV_FUSED && !use_fused(c, ts))
+  if ((c->variant == CCFD_KV_FUSED || c->variant == CCFD_KV_FUSED_FACE) &&
+      !fused_applies(c, ts))
     return fail(CCFD_ENOTSUP,
                 "fused operator needs 2nd order Euler and %zu bytes of LDS "
                 "(device limit %zu)",
                 fused_lds_bytes(ts, c->S, c->tile), c->ldsLimit);
-  *variant = use_fused(c, ts) ? CCFD_KV_FUSED : CCFD_KV_SPLIT;
+  *variant = active_variant(c, ts);
   return CCFD_OK;
 }
 

@@ -91,7 +91,9 @@ namespace ccfd {
 
 int fail(int code, const char* fmt, ...);
 void* dmalloc(ccfd_gpu_ctx* c, size_t bytes);
-void sfc_order(const ccfd_mesh* m, std::vector<int>& perm);
+// validate parameters; rk receives the low-storage RK coefficients
+// (src/timeDiscretization.c:94-178), rk[k] for stage k = 1..nRKstages
+int check_params(const ccfd_gpu_params* p, double rk[6]);
 int apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p);
 int upload_venk(ccfd_gpu_ctx* c);
 int refresh_prim(ccfd_gpu_ctx* c);
@@ -113,7 +115,9 @@ int download_vec(ccfd_gpu_ctx* c, const double* d, double* h);
 enum { KT_DT = 0, KT_RECON = 1, KT_FLUX = 2, KT_OTHER = 3, KT_N = 4 };
 void tspan_begin(ccfd_gpu_ctx* c, hipEvent_t* e);
 void tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind);
-// does the next operator call of type T run the fused kernel
-bool use_fused(const ccfd_gpu_ctx* c, size_t tsize);
+// can the fused kernels run (2nd order Euler, LDS tile fits), and which
+// ccfd_kernel_variant the next operator call of type T runs
+bool fused_applies(const ccfd_gpu_ctx* c, size_t tsize);
+int active_variant(const ccfd_gpu_ctx* c, size_t tsize);
 
 }  // namespace ccfd

@@ -25,8 +25,11 @@
 namespace ccfd {
 
 struct FusedArgs {
-  const int* __restrict__ haloStart;  // [nblocks + 1] into haloCells
-  const int* __restrict__ haloCells;  // per block, ascending element ids
+  // halo of arena block b: haloCells[haloStart[b] .. haloEnd[b]), ascending
+  // element ids (one mesh: haloEnd = haloStart + 1)
+  const int* __restrict__ haloStart;
+  const int* __restrict__ haloEnd;
+  const int* __restrict__ haloCells;
   int tile;                           // LDS columns: kBlock + max halo
 };
 
@@ -112,7 +115,7 @@ __device__ inline void fused_recon(const OpArgs<T>& a, int i, int col,
 template <typename T, int LIM, int S>
 __global__ void bc_side_recon_kernel(OpArgs<T> a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.G) return;
+  if (g >= a.G || a.ghostBC[g] < 0) return;
   const int i = a.ghostCell[g], k = a.ghostK[g];
   const int Np = a.Np;
   T p[4], gx[4], gy[4], GPx[S], GPy[S];
@@ -126,10 +129,83 @@ __global__ void bc_side_recon_kernel(OpArgs<T> a) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) in[v] = p[v] + gpx * gx[v] + gpy * gy[v];
   T x[2] = {gpx + T(a.bary[i]), gpy + T(a.bary[Np + i])};
-  bc_ghost<T>(a.eq, a.bct.bc[a.ghostBC[g]], a.ex, a.fn[k * Np + i],
-              a.fn[(S + k) * Np + i], op_time(a), in, out, x);
+  bc_ghost<T>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
+              a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
+}
+
+// dU/dt of element i from its gathered side fluxes (src/finiteVolume.c:
+// 106-119), then the optional RK stage update (src/timeDiscretization.c:
+// 380-397) and the residual contribution area * ut^2 (src/analyze.c:521-539)
+template <typename T>
+__device__ inline void stage_epilogue(const OpArgs<T>& a, int i,
+                                      const T acc[4], double r2[4]) {
+  const int Np = a.Np;
+  const T aq = a.areaq[i];
+  T ut[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
+  if (a.write_ut) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
+  }
+  if (a.rk) {
+    const double c = a.rkc * stage_dt(a, i);
+    double Uc[4], pn[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Uc[v] = a.U0[v * Np + i] + c * double(ut[v]);
+      a.U[v * Np + i] = Uc[v];
+    }
+    Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
+                     double(a.eq.mu), double(a.eq.Pr)};
+    cons_prim<double>(qd, Uc, pn);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
+  }
+  if (a.res_part) {
+    const double ar = a.area[i];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
+  }
+}
+
+// LDS column of neighbour element j: its offset in the block, or kBlock +
+// its position in the block's ascending halo list
+__device__ inline int tile_column(int j, int base, const int* hcell, int nh) {
+  int col = j - base;
+  if (col < 0 || col >= kBlock) {
+    int lo = 0, hi = nh - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (hcell[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    col = kBlock + lo;
+  }
+  return col;
+}
+
+// exclusive prefix sum of x over the block; *total gets the block sum
+__device__ inline int block_scan_excl(int x, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    const int v = wsum[k];
+    if (k < w) pre += v;
+    tot += v;
+  }
+  *total = tot;
+  return pre + inc - x;
 }
 
 template <int CODE, typename T, int LIM, int S>
@@ -139,12 +215,13 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
   T* lds = reinterpret_cast<T*>(fused_smem);
   const int tile = f.tile;
   int* hcell = reinterpret_cast<int*>(lds + 4 * S * tile);
-  const int lb = logical_block();
-  const int base = lb * kBlock;
+  const BlockPos bp = block_pos(a);
+  const int lb = bp.lb, base = bp.base;
   const int i = base + threadIdx.x;
-  const bool active = i < a.N;
+  const bool active = (int)threadIdx.x < bp.n;
   const int Np = a.Np;
-  const int h0 = f.haloStart[lb], nh = f.haloStart[lb + 1] - h0;
+  const int h0 = f.haloStart[base / kBlock];
+  const int nh = f.haloEnd[base / kBlock] - h0;
 
   // phase 1: owned, then halo side states into LDS
   int lk[S];
@@ -173,17 +250,7 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
       const bool owner = fl > T(0.0);
       const T len = owner ? fl : -fl;
       if (l >= 0) {
-        const int j = l >> 2, ps = l & 3;
-        int col = j - base;
-        if (col < 0 || col >= kBlock) {
-          // halo column: lower bound of j in the block's ascending list
-          int lo = 0, hi = nh - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (hcell[mid] < j) lo = mid + 1; else hi = mid;
-          }
-          col = kBlock + lo;
-        }
+        const int col = tile_column(l >> 2, base, hcell, nh), ps = l & 3;
 #pragma unroll
         for (int v = 0; v < 4; ++v) oth[v] = lds[(v * S + ps) * tile + col];
       } else {
@@ -210,47 +277,140 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
         acc[v] += fv[v];
       }
     }
-    const T aq = a.areaq[i];
-    T ut[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
-    if (a.write_ut) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
-    }
-    if (a.rk) {
-      const double dt = a.dtdev ? *a.dtdev : a.dt0;
-      const double c = a.rkc * dt;
-      double Uc[4], pn[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        Uc[v] = a.U0[v * Np + i] + c * double(ut[v]);
-        a.U[v * Np + i] = Uc[v];
-      }
-      Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
-                       double(a.eq.mu), double(a.eq.Pr)};
-      cons_prim<double>(qd, Uc, pn);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
-    }
-    if (a.res_part) {
-      const double ar = a.area[i];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
-    }
+    stage_epilogue<T>(a, i, acc, r2);
   }
   if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
 }
 
-// dynamic LDS bytes of one fused block: side states of tile columns plus
-// the halo id list
+// Face-once form of fused_kernel: after phase 1, the block lists the sides
+// whose face flux it evaluates - each owning side of an element of the
+// block, and each non-owning side whose owner lies outside the block - so
+// every face inside the block is evaluated once instead of once per
+// element. Each flux (owner orientation, times the face length) overwrites
+// the listed side's own LDS slot, which no other list entry reads; phase 3
+// gathers them per element in the reference's side order with the sign of
+// the side (src/fluxCalculation.c:993-996), the same sums as fused_kernel.
+template <int CODE, typename T, int LIM, int S>
+__global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
+                                                            FusedArgs f) {
+  extern __shared__ __align__(16) unsigned char fused_smem[];
+  __shared__ int wsum[kBlock / 64];
+  T* lds = reinterpret_cast<T*>(fused_smem);
+  const int tile = f.tile;
+  int* hcell = reinterpret_cast<int*>(lds + 4 * S * tile);
+  unsigned short* ent =
+      reinterpret_cast<unsigned short*>(hcell + (tile - kBlock));
+  const BlockPos bp = block_pos(a);
+  const int lb = bp.lb, base = bp.base;
+  const int i = base + threadIdx.x;
+  const bool active = (int)threadIdx.x < bp.n;
+  const int Np = a.Np;
+  const int h0 = f.haloStart[base / kBlock];
+  const int nh = f.haloEnd[base / kBlock] - h0;
+
+  // phase 1: side states into LDS; which of the element's sides it owns
+  // and which face fluxes this block evaluates for it
+  int lk[S];
+  unsigned ownMask = 0, faceMask = 0;
+  if (active) {
+    fused_recon<T, LIM, S>(a, i, threadIdx.x, tile, lds, lk);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (lk[s] == kNoSide) continue;
+      const bool owner = a.flen[s * Np + i] > T(0.0);
+      const int jc = (lk[s] >> 2) - base;
+      if (owner) ownMask |= 1u << s;
+      if (owner || (lk[s] >= 0 && (jc < 0 || jc >= kBlock)))
+        faceMask |= 1u << s;
+    }
+  }
+  for (int h = threadIdx.x; h < nh; h += kBlock) {
+    const int e = f.haloCells[h0 + h];
+    hcell[h] = e;
+    int lkh[S];
+    fused_recon<T, LIM, S>(a, e, kBlock + h, tile, lds, lkh);
+  }
+  int nf;
+  int k = block_scan_excl(__popc(faceMask), wsum, &nf);
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    if (faceMask & (1u << s)) ent[k++] = (unsigned short)((threadIdx.x << 2) | s);
+  __syncthreads();
+
+  // phase 2: one flux per listed side, into that side's LDS slot
+  for (int e = threadIdx.x; e < nf; e += kBlock) {
+    const int c = ent[e] >> 2, s = ent[e] & 3;
+    const int ie = base + c;
+    const int l = a.link[s * Np + ie];
+    T own[4], oth[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) own[v] = lds[(v * S + s) * tile + c];
+    const T nx = a.fn[s * Np + ie], ny = a.fn[(S + s) * Np + ie];
+    const T fl = a.flen[s * Np + ie];
+    const bool owner = fl > T(0.0);
+    const T len = owner ? fl : -fl;
+    if (l >= 0) {
+      const int col = tile_column(l >> 2, base, hcell, nh), ps = l & 3;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) oth[v] = lds[(v * S + ps) * tile + col];
+    } else {
+      const int g = -l - 1;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) oth[v] = a.ghostS[v * a.G + g];
+    }
+    const T* L = owner ? own : oth;
+    const T* R = owner ? oth : own;
+    T Lr[4] = {L[0], nx * L[1] + ny * L[2], -ny * L[1] + nx * L[2], L[3]};
+    T Rr[4] = {R[0], nx * R[1] + ny * R[2], -ny * R[1] + nx * R[2], R[3]};
+    T F[4];
+    conv_flux_t<CODE, T>(a.eq, Lr, Rr, F);
+    T fv[4];
+    fv[0] = F[0];
+    fv[1] = nx * F[1] - ny * F[2];
+    fv[2] = ny * F[1] + nx * F[2];
+    fv[3] = F[3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lds[(v * S + s) * tile + c] = fv[v] * len;
+  }
+  __syncthreads();
+
+  // phase 3: gather in side order
+  double r2[4] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    T acc[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (lk[s] == kNoSide) continue;
+      int slot = s, col = threadIdx.x;
+      if (!(faceMask & (1u << s))) {
+        // evaluated by the owning side, an element of this block
+        slot = lk[s] & 3;
+        col = (lk[s] >> 2) - base;
+      }
+      const bool owner = ownMask & (1u << s);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T g = lds[(v * S + slot) * tile + col];
+        acc[v] += owner ? g : -g;
+      }
+    }
+    stage_epilogue<T>(a, i, acc, r2);
+  }
+  if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
+}
+
+// dynamic LDS bytes of one fused block: side states of tile columns, the
+// halo id list and the face list
 inline size_t fused_lds_bytes(size_t tsize, int S, int tile) {
-  return 4 * (size_t)S * tile * tsize + (size_t)(tile - kBlock) * 4;
+  return 4 * (size_t)S * tile * tsize + (size_t)(tile - kBlock) * 4 +
+         (size_t)S * kBlock * 2;
 }
 
 // launch entry of one flux code (defined in op_flux.hip)
+// face: 0 fused_kernel, 1 fused_face_kernel
 template <typename T>
 void launch_fused_code(int code, const OpArgs<T>& a, const FusedArgs& f,
-                       int limiter, int nblocks, size_t lds, hipStream_t st);
+                       int limiter, int face, int nblocks, size_t lds,
+                       hipStream_t st);
 
 }  // namespace ccfd

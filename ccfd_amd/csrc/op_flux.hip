@@ -2,7 +2,7 @@
 // Built once per flux code with -DFLUX_CODE=<1..11> so the 11 flux
 // specialisations compile in parallel; each object exports
 // ccfd_launch_flux_<code>_{d,f} (two-kernel path, FP64 / FP32) and
-// ccfd_launch_fused_<code>_{d,f} (fused LDS path) used by ccfd_gpu.hip.
+// ccfd_launch_fused_<code>_{d,f} (fused LDS paths) used by ccfd_gpu.hip.
 #include "fused.hpp"
 
 #ifndef FLUX_CODE
@@ -30,21 +30,25 @@ static void dispatch(const OpArgs<T>& a, int nb, hipStream_t st) {
 }
 
 template <typename T, int LIM, int S>
-static void go_fused(const OpArgs<T>& a, const FusedArgs& f, int nb,
+static void go_fused(const OpArgs<T>& a, const FusedArgs& f, int face, int nb,
                      size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((fused_kernel<FLUX_CODE, T, LIM, S>), dim3(nb),
-                     dim3(kBlock), lds, st, a, f);
+  if (face)
+    hipLaunchKernelGGL((fused_face_kernel<FLUX_CODE, T, LIM, S>), dim3(nb),
+                       dim3(kBlock), lds, st, a, f);
+  else
+    hipLaunchKernelGGL((fused_kernel<FLUX_CODE, T, LIM, S>), dim3(nb),
+                       dim3(kBlock), lds, st, a, f);
 }
 
 template <typename T>
 static void dispatch_fused(const OpArgs<T>& a, const FusedArgs& f, int lim,
-                           int nb, size_t lds, hipStream_t st) {
+                           int face, int nb, size_t lds, hipStream_t st) {
   if (a.S == 3) {
-    if (lim == 1) go_fused<T, 1, 3>(a, f, nb, lds, st);
-    else go_fused<T, 2, 3>(a, f, nb, lds, st);
+    if (lim == 1) go_fused<T, 1, 3>(a, f, face, nb, lds, st);
+    else go_fused<T, 2, 3>(a, f, face, nb, lds, st);
   } else {
-    if (lim == 1) go_fused<T, 1, 4>(a, f, nb, lds, st);
-    else go_fused<T, 2, 4>(a, f, nb, lds, st);
+    if (lim == 1) go_fused<T, 1, 4>(a, f, face, nb, lds, st);
+    else go_fused<T, 2, 4>(a, f, face, nb, lds, st);
   }
 }
 
@@ -61,13 +65,13 @@ void CCFD_CAT(CCFD_CAT(ccfd_launch_flux_, FLUX_CODE), _f)(
 }
 
 void CCFD_CAT(CCFD_CAT(ccfd_launch_fused_, FLUX_CODE), _d)(
-    const ccfd::OpArgs<double>& a, const ccfd::FusedArgs& f, int lim, int nb,
-    size_t lds, hipStream_t st) {
-  ccfd::dispatch_fused<double>(a, f, lim, nb, lds, st);
+    const ccfd::OpArgs<double>& a, const ccfd::FusedArgs& f, int lim, int face,
+    int nb, size_t lds, hipStream_t st) {
+  ccfd::dispatch_fused<double>(a, f, lim, face, nb, lds, st);
 }
 
 void CCFD_CAT(CCFD_CAT(ccfd_launch_fused_, FLUX_CODE), _f)(
-    const ccfd::OpArgs<float>& a, const ccfd::FusedArgs& f, int lim, int nb,
-    size_t lds, hipStream_t st) {
-  ccfd::dispatch_fused<float>(a, f, lim, nb, lds, st);
+    const ccfd::OpArgs<float>& a, const ccfd::FusedArgs& f, int lim, int face,
+    int nb, size_t lds, hipStream_t st) {
+  ccfd::dispatch_fused<float>(a, f, lim, face, nb, lds, st);
 }

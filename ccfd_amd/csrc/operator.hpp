@@ -25,16 +25,11 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include <limits.h>
 
+#include "layout.hpp"
 #include "physics.hpp"
 
 namespace ccfd {
-
-constexpr int kBlock = 256;
-constexpr int kMaxBC = 32;
-constexpr int kMaxSlots = 4;
-constexpr int kNoSide = INT_MIN;
 
 struct BCTable {
   BCDev bc[kMaxBC];
@@ -88,6 +83,16 @@ struct OpArgs {
   Eq<T> eq;
   ExactCfg ex;
   BCTable bct;
+  // batched arena (all null for one mesh): per launched block,
+  // (arena block << 9) | elements in it; instance of each arena block;
+  // per-instance time and dt; per-element local dt; the arena's boundary
+  // condition table (ghostBC indexes it instead of bct)
+  const int* __restrict__ blockMap;
+  const int* __restrict__ blockInst;
+  const double* __restrict__ tInst;
+  const double* __restrict__ dtInst;
+  const double* __restrict__ dtLoc;
+  const BCDev* __restrict__ bcp;
 };
 
 // XCD-aware block order: consecutive logical blocks (neighbouring cells
@@ -101,9 +106,46 @@ __device__ inline int logical_block() {
   return (b % 8) * (nb8 / 8) + b / 8;
 }
 
+// launched block -> first element and element count
+struct BlockPos {
+  int lb, base, n;
+};
 template <typename T>
-__device__ inline T op_time(const OpArgs<T>& a) {
+__device__ inline BlockPos block_pos(const OpArgs<T>& a) {
+  const int lb = logical_block();
+  if (a.blockMap) {
+    const int m = a.blockMap[lb];
+    return {lb, (m >> 9) * kBlock, m & 511};
+  }
+  return {lb, lb * kBlock, min(kBlock, a.N - lb * kBlock)};
+}
+
+// stage time of element i (boundary states that depend on time)
+template <typename T>
+__device__ inline T op_time(const OpArgs<T>& a, int i) {
+  if (a.tInst) {
+    const int k = a.blockInst[i / kBlock];
+    return T(a.tInst[k] + a.tcoef * a.dtInst[k]);
+  }
   return a.tdev ? T(a.tdev[0] + a.tcoef * a.tdev[1]) : T(a.time0);
+}
+
+// time step of element i in the RK update. The device pointers are read
+// with relaxed atomic loads so the compiler cannot fold the choice into one
+// load through a selected address (which would put dt0 in private memory).
+template <typename T>
+__device__ inline double stage_dt(const OpArgs<T>& a, int i) {
+  const double* p = a.dtLoc ? a.dtLoc + i
+                  : a.dtInst ? a.dtInst + a.blockInst[i / kBlock]
+                  : a.dtdev;
+  if (p) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return a.dt0;
+}
+
+template <typename T>
+__device__ inline BCDev op_bc(const OpArgs<T>& a, int id) {
+  if (a.bcp) return a.bcp[id];
+  return a.bct.bc[id];
 }
 
 // ------------------------------------------------------------ recon kernel
@@ -112,8 +154,9 @@ __device__ inline T op_time(const OpArgs<T>& a) {
 template <typename T, int LIM, int SC>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
 void recon_kernel(OpArgs<T> a) {
-  const int i = logical_block() * kBlock + threadIdx.x;
-  if (i >= a.N) return;
+  const BlockPos bp = block_pos(a);
+  if ((int)threadIdx.x >= bp.n) return;
+  const int i = bp.base + threadIdx.x;
   const int Np = a.Np;
   constexpr int S = SC;
   T p[4];
@@ -196,15 +239,15 @@ void recon_kernel(OpArgs<T> a) {
 template <typename T>
 __global__ void bc_bary_kernel(OpArgs<T> a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.G) return;
+  if (g >= a.G || a.ghostBC[g] < 0) return;
   const int i = a.ghostCell[g], k = a.ghostK[g];
   const int Np = a.Np, S = a.S;
   T in[4], out[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) in[v] = a.prim[v * Np + i];
   T x[2] = {T(a.ghostBary[g]), T(a.ghostBary[a.G + g])};
-  bc_ghost<T>(a.eq, a.bct.bc[a.ghostBC[g]], a.ex, a.fn[k * Np + i],
-              a.fn[(S + k) * Np + i], op_time(a), in, out, x);
+  bc_ghost<T>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
+              a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) a.ghostP[v * a.G + g] = out[v];
 }
@@ -215,7 +258,7 @@ __global__ void bc_bary_kernel(OpArgs<T> a) {
 template <typename T>
 __global__ void bc_side_kernel(OpArgs<T> a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.G) return;
+  if (g >= a.G || a.ghostBC[g] < 0) return;
   const int i = a.ghostCell[g], k = a.ghostK[g];
   const int Np = a.Np, S = a.S;
   T in[4], out[4];
@@ -228,8 +271,8 @@ __global__ void bc_side_kernel(OpArgs<T> a) {
   }
   T x[2] = {a.gp[k * Np + i] + T(a.bary[i]),
             a.gp[(S + k) * Np + i] + T(a.bary[Np + i])};
-  bc_ghost<T>(a.eq, a.bct.bc[a.ghostBC[g]], a.ex, a.fn[k * Np + i],
-              a.fn[(S + k) * Np + i], op_time(a), in, out, x);
+  bc_ghost<T>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
+              a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
 }
@@ -258,9 +301,10 @@ __device__ inline void block_sum4(double v[4], double* res_part, int lb,
 
 template <int CODE, typename T, int ORDER, int NS>
 __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
-  const int lb = logical_block();
-  const int i = lb * kBlock + threadIdx.x;
-  const bool active = i < a.N;
+  const BlockPos bp = block_pos(a);
+  const int lb = bp.lb;
+  const int i = bp.base + threadIdx.x;
+  const bool active = (int)threadIdx.x < bp.n;
   const int Np = a.Np, S = a.S;
   double r2[4] = {0.0, 0.0, 0.0, 0.0};
   if (active) {
@@ -379,8 +423,7 @@ __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
       for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
     }
     if (a.rk) {
-      const double dt = a.dtdev ? *a.dtdev : a.dt0;
-      const double c = a.rkc * dt;
+      const double c = a.rkc * stage_dt(a, i);
       double Uc[4], pn[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {

@@ -12,7 +12,7 @@ from . import _native as N
 
 FP64, FP32_FLUX = 0, 1
 # operator implementations (enum ccfd_kernel_variant)
-KV_AUTO, KV_SPLIT, KV_FUSED = 0, 1, 2
+KV_AUTO, KV_SPLIT, KV_FUSED, KV_FUSED_FACE = 0, 1, 2, 3
 
 
 class GpuError(RuntimeError):
@@ -142,7 +142,8 @@ class Solver:
         self.S = mesh.n_slots
 
     def set_variant(self, variant):
-        """Operator implementation: KV_AUTO, KV_SPLIT or KV_FUSED."""
+        """Operator implementation: KV_AUTO, KV_SPLIT, KV_FUSED or
+        KV_FUSED_FACE."""
         check(self.L.ccfd_gpu_set_variant(self.h, variant))
 
     def variant(self):

@@ -172,18 +172,21 @@ int ccfd_gpu_riemann_batch(int code, int precision, double gam, long n,
 
 /* ---- kernel variants ---- */
 /*
- * The 2nd-order Euler operator has two implementations with bit-identical
- * results: CCFD_KV_SPLIT (reconstruction kernel writes side states to HBM,
- * flux kernel reads them) and CCFD_KV_FUSED (one kernel per block keeps the
- * side states of the block and its halo in LDS). CCFD_KV_AUTO (default)
- * takes the fused kernel where it applies (2nd order, no viscous flux,
- * LDS tile within the device limit) and the split pair elsewhere.
+ * The 2nd-order Euler operator has three implementations with
+ * bit-identical results: CCFD_KV_SPLIT (reconstruction kernel writes side
+ * states to HBM, flux kernel reads them), CCFD_KV_FUSED (one kernel; each
+ * block keeps the side states of its elements and their halo in LDS and
+ * evaluates every side's flux) and CCFD_KV_FUSED_FACE (the same, with each
+ * face inside a block evaluated once). CCFD_KV_AUTO (default) takes
+ * CCFD_KV_FUSED_FACE where a fused kernel applies (2nd order, no viscous
+ * flux, LDS tile within the device limit) and the split pair elsewhere.
  * ccfd_gpu_get_variant reports what the next operator call will run.
  */
 enum ccfd_kernel_variant {
 	CCFD_KV_AUTO = 0,
 	CCFD_KV_SPLIT = 1,
-	CCFD_KV_FUSED = 2
+	CCFD_KV_FUSED = 2,
+	CCFD_KV_FUSED_FACE = 3
 };
 int ccfd_gpu_set_variant(ccfd_gpu_ctx *ctx, int variant);
 int ccfd_gpu_get_variant(ccfd_gpu_ctx *ctx, int *variant);

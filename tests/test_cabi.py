@@ -18,6 +18,7 @@ def declared(header):
 
 @pytest.mark.parametrize("header,lib", [
     ("ccfd_gpu.h", "libccfd_gpu.so"),
+    ("ccfd_batch.h", "libccfd_gpu.so"),
     ("ccfd_mesh.h", "libccfd_host.so"),
     ("ccfd_dropin.h", "libccfd_host.so"),
 ])
@@ -31,3 +32,10 @@ def test_library_exports_declared_symbols(header, lib):
     assert len(names) > 3
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_every_header_is_checked():
+    hdrs = sorted(f for f in os.listdir(os.path.join(ROOT, "include"))
+                  if f.endswith(".h"))
+    assert hdrs == ["ccfd_batch.h", "ccfd_dropin.h", "ccfd_gpu.h",
+                    "ccfd_mesh.h"]

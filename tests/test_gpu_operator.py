@@ -31,8 +31,9 @@ def _pair(mesh, flux, order, lim, k=5.0, ns=0, mu=0.0, precision=G.FP64,
 
 
 # the 2nd-order Euler operator is checked in both implementations
-VARIANTS = pytest.mark.parametrize("variant", [G.KV_SPLIT, G.KV_FUSED],
-                                   ids=["split", "fused"])
+VARIANTS = pytest.mark.parametrize(
+    "variant", [G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE],
+    ids=["split", "fused", "fused_face"])
 
 
 def _check_bitwise(gs, orc, prim, time=0.0, order=2):
@@ -113,7 +114,7 @@ def test_fvtd_periodic_delaunay_bitwise(flux, variant):
 @pytest.mark.parametrize("order,lim", [(1, 1), (2, 1), (2, 2)])
 @VARIANTS
 def test_fvtd_all_bc_types_bitwise(order, lim, variant):
-    if order == 1 and variant == G.KV_FUSED:
+    if order == 1 and variant in (G.KV_FUSED, G.KV_FUSED_FACE):
         pytest.skip("the fused operator is 2nd order only")
     mesh, _ = cases.mixed_bc_mesh()
     gs, orc = _pair(mesh, 5, order, lim, k=5.0, variant=variant)
@@ -130,9 +131,10 @@ def test_fused_rejected_where_it_does_not_apply():
     prm = G.params(flux=2, order=2, limiter=2, navierstokes=1, mu=0.01)
     gs = G.Solver(mesh, prm)
     assert gs.variant() == G.KV_SPLIT
-    gs.set_variant(G.KV_FUSED)
-    with pytest.raises(G.GpuError):
-        gs.variant()
+    for v in (G.KV_FUSED, G.KV_FUSED_FACE):
+        gs.set_variant(v)
+        with pytest.raises(G.GpuError):
+            gs.variant()
     gs.close()
 
 
@@ -224,18 +226,18 @@ def test_fused_equals_split_large_delaunay(n, periodic):
     boundaries when not periodic."""
     mesh = cases.delaunay_mesh(n, seed=7, periodic=periodic)
     prm = G.params(flux=5, order=2, limiter=2, venk_k=5.0, nrk=4, torder=2)
-    a = G.Solver(mesh, prm)
-    b = G.Solver(mesh, prm)
-    a.set_variant(G.KV_SPLIT)
-    b.set_variant(G.KV_FUSED)
-    assert b.variant() == G.KV_FUSED
     prim = M.smooth_state(mesh.bary, seed=9)
-    a.set_prim(prim)
-    b.set_prim(prim)
-    ta, ra = a.advance(3, 0.0, 1e9, 1e9)
-    tb, rb = b.advance(3, 0.0, 1e9, 1e9)
-    assert ta == tb
-    assert np.array_equal(a.cons(), b.cons())
-    assert np.array_equal(a.ut(), b.ut())
-    a.close()
-    b.close()
+    out = []
+    for v in (G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE):
+        a = G.Solver(mesh, prm)
+        a.set_variant(v)
+        assert a.variant() == v
+        a.set_prim(prim)
+        t, r = a.advance(3, 0.0, 1e9, 1e9)
+        out.append((t, r, a.cons(), a.ut()))
+        a.close()
+    for t, r, cons, ut in out[1:]:
+        assert t == out[0][0]
+        assert np.array_equal(r, out[0][1])
+        assert np.array_equal(cons, out[0][2])
+        assert np.array_equal(ut, out[0][3])
