@@ -152,8 +152,7 @@ __device__ inline BCDev op_bc(const OpArgs<T>& a, int id) {
 // SC: sides per element known at compile time (3: triangles, 4: quads /
 // mixed with padding) so all neighbour loads of an element issue together
 template <typename T, int LIM, int SC>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
-void recon_kernel(OpArgs<T> a) {
+__global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
   const BlockPos bp = block_pos(a);
   if ((int)threadIdx.x >= bp.n) return;
   const int i = bp.base + threadIdx.x;
