@@ -460,6 +460,7 @@ static void launch_step(ccfd_batch* b) {
     a.U = b->Ubuf[uNew];
     a.prim_out = (T*)b->prim[1 - b->cur];
     a.write_ut = (k == b->nst) ? 1 : 0;
+    a.write_U = (k == b->nst) ? 1 : 0;
     a.res_part = (k == b->nst) ? b->part : nullptr;
     if (a.order == 2) {
       if (b->ghostHigh > 0)

@@ -727,6 +727,7 @@ OpArgs<T> ccfd::op_args(ccfd_gpu_ctx* c) {
   a.ghostP = (T*)c->ghostP;
   a.ut = (T*)c->ut;
   a.write_ut = 1;
+  a.write_U = 1;
   Eq<double> qd = eqd(c);
   a.eq.gam = T(qd.gam); a.eq.gam1 = T(qd.gam1); a.eq.gam1q = T(qd.gam1q);
   a.eq.mu = T(qd.mu); a.eq.Pr = T(qd.Pr);
@@ -920,6 +921,7 @@ static void step_t(ccfd_gpu_ctx* c, double t, double dt, bool devDt, bool res) {
     // dU/dt of the last stage is kept for host readback; earlier stages
     // feed the update only
     a.write_ut = (k == nst) ? 1 : 0;
+    a.write_U = (k == nst) ? 1 : 0;
     a.res_part = (res && k == nst) ? c->part : nullptr;
     if (!c->keepGradients && !a.ns) { a.ux = nullptr; a.uy = nullptr; }
     launch_operator<T>(c, a);

@@ -135,42 +135,6 @@ __global__ void bc_side_recon_kernel(OpArgs<T> a) {
   for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
 }
 
-// dU/dt of element i from its gathered side fluxes (src/finiteVolume.c:
-// 106-119), then the optional RK stage update (src/timeDiscretization.c:
-// 380-397) and the residual contribution area * ut^2 (src/analyze.c:521-539)
-template <typename T>
-__device__ inline void stage_epilogue(const OpArgs<T>& a, int i,
-                                      const T acc[4], double r2[4]) {
-  const int Np = a.Np;
-  const T aq = a.areaq[i];
-  T ut[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
-  if (a.write_ut) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
-  }
-  if (a.rk) {
-    const double c = a.rkc * stage_dt(a, i);
-    double Uc[4], pn[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      Uc[v] = a.U0[v * Np + i] + c * double(ut[v]);
-      a.U[v * Np + i] = Uc[v];
-    }
-    Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
-                     double(a.eq.mu), double(a.eq.Pr)};
-    cons_prim<double>(qd, Uc, pn);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
-  }
-  if (a.res_part) {
-    const double ar = a.area[i];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
-  }
-}
-
 // LDS column of neighbour element j: its offset in the block, or kBlock +
 // its position in the block's ascending halo list
 __device__ inline int tile_column(int j, int base, const int* hcell, int nh) {
@@ -222,6 +186,9 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
   const int Np = a.Np;
   const int h0 = f.haloStart[base / kBlock];
   const int nh = f.haloEnd[base / kBlock] - h0;
+
+  StagePre<T> pre{};
+  if (active) pre = stage_preload(a, i);
 
   // phase 1: owned, then halo side states into LDS
   int lk[S];
@@ -277,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
         acc[v] += fv[v];
       }
     }
-    stage_epilogue<T>(a, i, acc, r2);
+    stage_epilogue<T>(a, i, acc, r2, pre);
   }
   if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
 }
@@ -312,7 +279,9 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
   // and which face fluxes this block evaluates for it
   int lk[S];
   unsigned ownMask = 0, faceMask = 0;
+  StagePre<T> pre{};
   if (active) {
+    pre = stage_preload(a, i);
     fused_recon<T, LIM, S>(a, i, threadIdx.x, tile, lds, lk);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -394,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
         acc[v] += owner ? g : -g;
       }
     }
-    stage_epilogue<T>(a, i, acc, r2);
+    stage_epilogue<T>(a, i, acc, r2, pre);
   }
   if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
 }

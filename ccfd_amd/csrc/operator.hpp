@@ -78,6 +78,7 @@ struct OpArgs {
   double* __restrict__ U;
   T* __restrict__ prim_out;
   int write_ut;
+  int write_U;          // store U (last stage); intermediate stages do not
   // residual partial sums [4][nblocks] of area * ut^2
   double* res_part;
   Eq<T> eq;
@@ -298,6 +299,71 @@ __device__ inline void block_sum4(double v[4], double* res_part, int lb,
   }
 }
 
+// dU/dt of element i from its gathered side fluxes (src/finiteVolume.c:
+// 106-119), then the optional RK stage update (src/timeDiscretization.c:
+// 380-397; the conservative state is stored only where it is needed after
+// the stage, its primitive form always) and the residual contribution
+// area * ut^2 (src/analyze.c:521-539)
+// element data the epilogue reads that does not depend on the fluxes; the
+// fused kernels load it before their first barrier so its latency overlaps
+// the reconstruction
+template <typename T>
+struct StagePre {
+  double u0[4];
+  T aq;
+};
+
+template <typename T>
+__device__ inline StagePre<T> stage_preload(const OpArgs<T>& a, int i) {
+  StagePre<T> p;
+  p.aq = a.areaq[i];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) p.u0[v] = a.rk ? a.U0[v * a.Np + i] : 0.0;
+  return p;
+}
+
+template <typename T>
+__device__ inline void stage_epilogue(const OpArgs<T>& a, int i,
+                                      const T acc[4], double r2[4],
+                                      const StagePre<T>& pre) {
+  const int Np = a.Np;
+  const T aq = pre.aq;
+  T ut[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
+  if (a.write_ut) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
+  }
+  if (a.rk) {
+    const double c = a.rkc * stage_dt(a, i);
+    double Uc[4], pn[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Uc[v] = pre.u0[v] + c * double(ut[v]);
+    // intermediate stages only feed the next stage's primitive state
+    if (a.write_U) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a.U[v * Np + i] = Uc[v];
+    }
+    Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
+                     double(a.eq.mu), double(a.eq.Pr)};
+    cons_prim<double>(qd, Uc, pn);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
+  }
+  if (a.res_part) {
+    const double ar = a.area[i];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
+  }
+}
+
+template <typename T>
+__device__ inline void stage_epilogue(const OpArgs<T>& a, int i,
+                                      const T acc[4], double r2[4]) {
+  stage_epilogue<T>(a, i, acc, r2, stage_preload(a, i));
+}
+
 template <int CODE, typename T, int ORDER, int NS>
 __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
   const BlockPos bp = block_pos(a);
@@ -413,33 +479,7 @@ __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
         acc[v] += f[v];
       }
     }
-    const T aq = a.areaq[i];
-    T ut[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
-    if (a.write_ut) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];
-    }
-    if (a.rk) {
-      const double c = a.rkc * stage_dt(a, i);
-      double Uc[4], pn[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        Uc[v] = a.U0[v * Np + i] + c * double(ut[v]);
-        a.U[v * Np + i] = Uc[v];
-      }
-      Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
-                       double(a.eq.mu), double(a.eq.Pr)};
-      cons_prim<double>(qd, Uc, pn);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
-    }
-    if (a.res_part) {
-      const double ar = a.area[i];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) r2[v] = ar * double(ut[v]) * double(ut[v]);
-    }
+    stage_epilogue<T>(a, i, acc, r2);
   }
   if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
 }

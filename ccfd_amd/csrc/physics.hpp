@@ -832,23 +832,25 @@ CCFD_HD void bc_ghost(const Eq<T>& q, const BCDev& b, const ExactCfg& ex,
 // per-side limiter value for one variable (src/reconstruction.c:59-69,
 // 143-160); d = grad . GP
 template <int LIM, typename T>
+//
+// The two signs of d use the same expression on different bounds, so the
+// bound is selected first and the expression evaluated once: the same
+// operations on the same operands as the branches of the reference, without
+// divergent paths in a wavefront. d == 0 (or NaN) gives 1, as in the
+// reference; the expression's value there is discarded.
 CCFD_HD T limiter_phi(T d, T dMax, T dMin, T dMax2, T dMin2, T eps) {
-  T pl = T(1.0);
+  const bool pos = d > T(0.0);
+  const T dm = pos ? dMax : dMin;
+  T pl;
   if (LIM == 1) {
-    if (d > T(0.0))
-      pl = tmin(T(1.0), dMax / d);
-    else if (d < T(0.0))
-      pl = tmin(T(1.0), dMin / d);
+    pl = tmin(T(1.0), dm / d);
   } else {
-    T d2 = d * d;
-    if (d > T(0.0))
-      pl = T(1.0) / d * (((dMax2 + eps) * d + T(2.0) * d2 * dMax) /
-                         (dMax2 + T(2.0) * d2 + d * dMax + eps));
-    else if (d < T(0.0))
-      pl = T(1.0) / d * (((dMin2 + eps) * d + T(2.0) * d2 * dMin) /
-                         (dMin2 + T(2.0) * d2 + d * dMin + eps));
+    const T dm2 = pos ? dMax2 : dMin2;
+    const T d2 = d * d;
+    pl = T(1.0) / d * (((dm2 + eps) * d + T(2.0) * d2 * dm) /
+                       (dm2 + T(2.0) * d2 + d * dm + eps));
   }
-  return pl;
+  return (d > T(0.0) || d < T(0.0)) ? pl : T(1.0);
 }
 
 }  // namespace ccfd
