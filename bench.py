@@ -167,28 +167,41 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
     out = {"requests": n_req, "cells_range": list(cells),
            "arrival_rate_per_s": rate, "max_steps": max_steps,
            "target_drop": 1e-6,
-           "scheme": "AUSMD, VK K=5, RK4 local time steps, FP32 flux / FP64 "
-                     "state; baseline FP64 one solve at a time"}
+           "scheme": "AUSMD, VK K=5, RK4 local time steps at CFL 2; served: "
+                     "FP32 flux / FP64 state until a 1e-4 drop or a stall, "
+                     "then FP64 to the target; baseline: FP64 one solve at a "
+                     "time"}
     b = BT.Batch(scheme(G.FP32_FLUX), max_sides=4, local_dt=True,
-                 hbm_bytes=hbm_gb << 30)
+                 hbm_bytes=(3 * hbm_gb << 30) // 4)
+    pol = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
+                   hbm_bytes=(hbm_gb << 30) // 4)
     srv = SV.Server(b, target_drop=1e-6, max_steps=max_steps,
-                    steps_per_poll=100, keep_state=baseline)
+                    steps_per_poll=100, keep_state=baseline, polish=pol,
+                    switch_drop=1e-4, log=log)
     res = srv.run(requests(arr))
     b.close()
+    pol.close()
     out["served"] = SV.summary(res)
+    log("served: %s" % json.dumps(out["served"]))
     if baseline:
         b64 = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
                        hbm_bytes=hbm_gb << 30)
         srv64 = SV.Server(b64, target_drop=1e-6, max_steps=max_steps,
-                          steps_per_poll=100, keep_state=True, max_active=1)
+                          steps_per_poll=100, keep_state=True, max_active=1,
+                          log=log)
         res64 = srv64.run(requests(np.zeros(n_req)))
         b64.close()
         out["one_at_a_time_fp64"] = SV.summary(res64)
         errs = [_density_err(r["prim"], q["prim"]) for r, q in zip(res, res64)]
+        both = [e for e, r, q in zip(errs, res, res64)
+                if r["converged"] and q["converged"]]
         out["density_err_vs_fp64"] = {
             "l2_max": max(e[0] for e in errs),
             "l2_median": float(np.median([e[0] for e in errs])),
-            "linf_max": max(e[1] for e in errs)}
+            "linf_max": max(e[1] for e in errs),
+            "both_converged": len(both),
+            "both_converged_l2_max": max([e[0] for e in both] or [0.0]),
+            "both_converged_linf_max": max([e[1] for e in both] or [0.0])}
         out["speedup_solves_per_s"] = (out["served"]["solves_per_s"] /
                                        out["one_at_a_time_fp64"]["solves_per_s"])
     return out

@@ -74,6 +74,7 @@ struct RangeAlloc {
 
 struct Inst {
   bool used = false;
+  bool wide = false;   // block halo larger than the fused tile
   int N = 0, nbk = 0, b0 = 0, g0 = 0, ng = 0, h0 = 0, nh = 0, bc0 = 0, nbc = 0;
   std::vector<int> perm;
   double totalArea_q = 0.0;
@@ -128,6 +129,7 @@ struct ccfd_batch {
   bool dirty = true;
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   int variant = CCFD_KV_AUTO;
+  int wideHalo = 0;   // instances whose block halo exceeds the LDS tile
   std::vector<void*> allocs;
 };
 
@@ -388,8 +390,11 @@ static bool fused_ok(const ccfd_batch* b) {
   return b->cfg.scheme.spatialOrder == 2;
 }
 
+// the fused kernels while every instance's halo fits the LDS tile, the
+// split pair otherwise (same results)
 static int active_variant(const ccfd_batch* b) {
-  if (b->variant == CCFD_KV_SPLIT || !fused_ok(b)) return CCFD_KV_SPLIT;
+  if (b->variant == CCFD_KV_SPLIT || !fused_ok(b) || b->wideHalo > 0)
+    return CCFD_KV_SPLIT;
   return b->variant == CCFD_KV_AUTO ? CCFD_KV_FUSED_FACE : b->variant;
 }
 
@@ -568,10 +573,15 @@ extern "C" int ccfd_batch_create(ccfd_batch** out, int device,
   b->S = cfg->maxSides;
   b->ts = p.precision == CCFD_FP64 ? 8 : 4;
   b->maxInst = cfg->maxInstances > 0 ? cfg->maxInstances : 4096;
-  const int maxHalo = cfg->maxHalo > 0 ? cfg->maxHalo : 192;
-  b->tile = kBlock + maxHalo;
   int ldsMax = 65536;
   hipDeviceGetAttribute(&ldsMax, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  // largest halo whose tile fits the device's LDS (at most 512 columns)
+  int fitHalo = 0;
+  while (fitHalo + 16 <= 512 &&
+         fused_lds_bytes(b->ts, b->S, kBlock + fitHalo + 16) + 256 <= (size_t)ldsMax)
+    fitHalo += 16;
+  const int maxHalo = cfg->maxHalo > 0 ? cfg->maxHalo : fitHalo;
+  b->tile = kBlock + maxHalo;
   if (p.spatialOrder == 2 &&
       fused_lds_bytes(b->ts, b->S, b->tile) + 256 > (size_t)ldsMax) {
     delete b;
@@ -741,19 +751,20 @@ extern "C" int ccfd_batch_add(ccfd_batch* b, const ccfd_mesh* m,
   }
   MeshSoA h;
   build_soa(m, b->S, b0 * kBlock, g0, bc0, h);
-  const int tileNeed = kBlock + h.maxHalo;
   const int h0 = b->halos.alloc((int)h.haloCells.size());
-  if (h0 < 0 || (fused_ok(b) && tileNeed > b->tile)) {
-    if (h0 >= 0) b->halos.release(h0, (int)h.haloCells.size());
+  if (h0 < 0) {
     b->blocks.release(b0, nbk);
     b->ghosts.release(g0, G);
     b->bcs.release(bc0, m->nBC);
-    if (h0 < 0) return fail(CCFD_ENOMEM, "no room for the halo lists");
-    return fail(CCFD_ENOTSUP, "block halo of %d exceeds the batch's %d",
-                h.maxHalo, b->tile - kBlock);
+    return fail(CCFD_ENOMEM, "no room for the halo lists");
   }
   Inst& I = b->inst[k];
   I.used = true;
+  I.wide = kBlock + h.maxHalo > b->tile;
+  if (I.wide) {
+    ++b->wideHalo;
+    free_graphs(b);
+  }
   I.N = N;
   I.nbk = nbk;
   I.b0 = b0;
@@ -865,6 +876,10 @@ extern "C" int ccfd_batch_remove(ccfd_batch* b, int id) {
   b->ghosts.release(I.g0, I.ng);
   b->halos.release(I.h0, I.nh);
   b->bcs.release(I.bc0, I.nbc);
+  if (I.wide) {
+    --b->wideHalo;
+    free_graphs(b);
+  }
   I = Inst();
   return rebuild_tables(b);
 }
@@ -1074,7 +1089,7 @@ extern "C" int ccfd_batch_autotune(ccfd_batch* b, int nSteps, int* variant,
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   for (int c : cand) {
-    if (c != CCFD_KV_SPLIT && !fused_ok(b)) continue;
+    if (c != CCFD_KV_SPLIT && (!fused_ok(b) || b->wideHalo > 0)) continue;
     b->variant = c;
     free_graphs(b);
     ccfd_batch_step(b, 2);   // warm-up and graph capture

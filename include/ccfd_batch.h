@@ -44,7 +44,10 @@ typedef struct ccfd_batch_cfg {
 	                           free memory */
 	int maxInstances;       /* instance slots; 0: 4096 */
 	int maxHalo;            /* fused-operator halo columns per block (LDS);
-	                           0: 192 */
+	                           0: the most the device's LDS holds (at most
+	                           512). While an instance with a larger block
+	                           halo is admitted, the batch runs the split
+	                           operator (same results). */
 } ccfd_batch_cfg;
 
 typedef struct ccfd_batch_case {

@@ -202,3 +202,23 @@ def test_local_time_stepping_reaches_a_steady_state():
           (r[0] / r0[0]))
     assert r[0] / r0[0] < 1e-5
     b.close()
+
+
+def test_wide_halo_instance_switches_to_the_split_operator():
+    """An instance whose block halo exceeds the LDS tile is admitted; the
+    batch runs the split operator while it is present, with unchanged
+    results."""
+    sch = _scheme(G.FP64)
+    cs = _cases((16, 24))
+    b = BT.Batch(sch, hbm_bytes=32 * MB, max_halo=16)
+    ref = BT.Batch(sch, hbm_bytes=32 * MB)
+    ids = [b.add(m, p) for m, p in cs]
+    rids = [ref.add(m, p) for m, p in cs]
+    assert b.variant() == G.KV_SPLIT
+    assert ref.variant() == G.KV_FUSED_FACE
+    b.step(4)
+    ref.step(4)
+    for k, r in zip(ids, rids):
+        assert np.array_equal(b.cons(k), ref.cons(r))
+    b.close()
+    ref.close()

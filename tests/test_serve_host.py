@@ -150,3 +150,73 @@ def test_one_at_a_time_mode_never_overlaps():
     order = sorted(res, key=lambda r: r["admitted"])
     for a, b in zip(order, order[1:]):
         assert b["admitted"] >= a["done"]
+
+
+class FloorBatch(FakeBatch):
+    """main arena whose residual floors (like FP32 flux arithmetic)"""
+
+    def __init__(self, cap, clock, floor):
+        super().__init__(cap, clock)
+        self.floor = floor
+
+    def poll(self):
+        out = super().poll()
+        for k, i in self.inst.items():
+            out[k]["res"][0] = max(out[k]["res"][0], self.floor)
+        return out
+
+    def cons(self, k):
+        i = self.inst[k]
+        tau = 20.0 + i["n"] / 50.0
+        i["mesh"].level = max(math.exp(-i["steps"] / tau), self.floor)
+        return np.ones((2, 4))
+
+    def add(self, mesh, prim, cfl=0.9):
+        k = super().add(mesh, prim, cfl)
+        self.inst[k]["mesh"] = mesh
+        return k
+
+
+class PolishBatch(FakeBatch):
+    """FP64 arena: continues from the level the main arena handed over"""
+
+    def add(self, mesh, prim, cfl=0.9):
+        k = super().add(mesh, prim, cfl)
+        self.inst[k]["level"] = mesh.level
+        return k
+
+    def poll(self):
+        out = {}
+        for k, i in self.inst.items():
+            r = i["level"] * math.exp(-i["steps"] / 30.0)
+            out[k] = {"steps": i["steps"], "res": np.array([r, 0, 0, 0]),
+                      "res0": np.array([i["level"], 0, 0, 0]),
+                      "nonfinite": False}
+        return out
+
+
+def test_precision_promotion_reaches_the_target_below_the_floor():
+    clock = Clock()
+    main = FloorBatch(cap=20000, clock=clock, floor=3e-5)
+    pol = PolishBatch(cap=8000, clock=clock)
+    srv = S.Server(main, target_drop=1e-6, max_steps=100000,
+                   steps_per_poll=25, prep_workers=2, polish=pol,
+                   switch_drop=1e-4, stall_polls=4, clock=clock,
+                   sleep=clock.sleep)
+    res = srv.run(_requests(20, rate=200.0, seed=11))
+    assert len(res) == 20
+    for r in res:
+        assert r["converged"] and r["drop"] <= 1e-6
+        assert 0 < r["polish_steps"] < r["steps"]
+    assert main.used == 0 and pol.used == 0
+    assert pol.max_used <= pol.cap
+
+
+def test_without_polish_the_floor_caps_the_solves():
+    clock = Clock()
+    main = FloorBatch(cap=20000, clock=clock, floor=3e-5)
+    srv = S.Server(main, target_drop=1e-6, max_steps=3000,
+                   steps_per_poll=25, prep_workers=2, clock=clock,
+                   sleep=clock.sleep)
+    res = srv.run(_requests(6, rate=200.0, seed=12))
+    assert all(not r["converged"] and r["steps"] >= 3000 for r in res)
