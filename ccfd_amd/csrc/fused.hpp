@@ -52,7 +52,7 @@ __device__ inline void limited_gradient(const OpArgs<T>& a, int i, T p[4],
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (lk[s] == kNoSide) continue;
+    if (no_side<S>(lk[s])) continue;
     T q[4];
     if (lk[s] >= 0) {
       const int j = lk[s] >> 2;
@@ -85,7 +85,7 @@ __device__ inline void limited_gradient(const OpArgs<T>& a, int i, T p[4],
     T phi = T(1.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (lk[s] == kNoSide) continue;
+      if (no_side<S>(lk[s])) continue;
       const T d = gx[v] * GPx[s] + gy[v] * GPy[s];
       phi = tmin(phi, limiter_phi<LIM, T>(d, dmax, dmin, dmax * dmax,
                                           dmin * dmin, eps));
@@ -193,7 +193,9 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
   // phase 1: owned, then halo side states into LDS
   int lk[S];
   if (active) fused_recon<T, LIM, S>(a, i, threadIdx.x, tile, lds, lk);
-  for (int h = threadIdx.x; h < nh; h += kBlock) {
+  // halo elements on the last threads: the first threads take the extra
+  // list entries of the flux phase
+  for (int h = kBlock - 1 - (int)threadIdx.x; h < nh; h += kBlock) {
     const int e = f.haloCells[h0 + h];
     hcell[h] = e;
     int lkh[S];
@@ -208,26 +210,32 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
 #pragma unroll 1
     for (int s = 0; s < S; ++s) {
       const int l = lk[s];
-      if (l == kNoSide) continue;
-      T own[4], oth[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) own[v] = lds[(v * S + s) * tile + threadIdx.x];
+      if (no_side<S>(l)) continue;
       const T nx = a.fn[s * Np + i], ny = a.fn[(S + s) * Np + i];
       const T fl = a.flen[s * Np + i];
       const bool owner = fl > T(0.0);
       const T len = owner ? fl : -fl;
+      const int c = threadIdx.x;
+      T L[4], R[4];   // owning side left
       if (l >= 0) {
         const int col = tile_column(l >> 2, base, hcell, nh), ps = l & 3;
+        const int cL = owner ? c : col, sL = owner ? s : ps;
+        const int cR = owner ? col : c, sR = owner ? ps : s;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) oth[v] = lds[(v * S + ps) * tile + col];
+        for (int v = 0; v < 4; ++v) {
+          L[v] = lds[(v * S + sL) * tile + cL];
+          R[v] = lds[(v * S + sR) * tile + cR];
+        }
       } else {
-        // ghost side state from bc_side_recon_kernel
+        // ghost side state from bc_side_recon_kernel; the boundary side
+        // owns its face
         const int g = -l - 1;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) oth[v] = a.ghostS[v * a.G + g];
+        for (int v = 0; v < 4; ++v) {
+          L[v] = lds[(v * S + s) * tile + c];
+          R[v] = a.ghostS[v * a.G + g];
+        }
       }
-      const T* L = owner ? own : oth;
-      const T* R = owner ? oth : own;
       T Lr[4] = {L[0], nx * L[1] + ny * L[2], -ny * L[1] + nx * L[2], L[3]};
       T Rr[4] = {R[0], nx * R[1] + ny * R[2], -ny * R[1] + nx * R[2], R[3]};
       T F[4];
@@ -285,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
     fused_recon<T, LIM, S>(a, i, threadIdx.x, tile, lds, lk);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (lk[s] == kNoSide) continue;
+      if (no_side<S>(lk[s])) continue;
       const bool owner = a.flen[s * Np + i] > T(0.0);
       const int jc = (lk[s] >> 2) - base;
       if (owner) ownMask |= 1u << s;
@@ -293,7 +301,9 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
         faceMask |= 1u << s;
     }
   }
-  for (int h = threadIdx.x; h < nh; h += kBlock) {
+  // halo elements on the last threads: the first threads take the extra
+  // list entries of the flux phase
+  for (int h = kBlock - 1 - (int)threadIdx.x; h < nh; h += kBlock) {
     const int e = f.haloCells[h0 + h];
     hcell[h] = e;
     int lkh[S];
@@ -311,24 +321,29 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
     const int c = ent[e] >> 2, s = ent[e] & 3;
     const int ie = base + c;
     const int l = a.link[s * Np + ie];
-    T own[4], oth[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) own[v] = lds[(v * S + s) * tile + c];
     const T nx = a.fn[s * Np + ie], ny = a.fn[(S + s) * Np + ie];
     const T fl = a.flen[s * Np + ie];
     const bool owner = fl > T(0.0);
     const T len = owner ? fl : -fl;
+    T L[4], R[4];   // owning side left
     if (l >= 0) {
       const int col = tile_column(l >> 2, base, hcell, nh), ps = l & 3;
+      const int cL = owner ? c : col, sL = owner ? s : ps;
+      const int cR = owner ? col : c, sR = owner ? ps : s;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) oth[v] = lds[(v * S + ps) * tile + col];
+      for (int v = 0; v < 4; ++v) {
+        L[v] = lds[(v * S + sL) * tile + cL];
+        R[v] = lds[(v * S + sR) * tile + cR];
+      }
     } else {
+      // a boundary side owns its face; the ghost state is on the right
       const int g = -l - 1;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) oth[v] = a.ghostS[v * a.G + g];
+      for (int v = 0; v < 4; ++v) {
+        L[v] = lds[(v * S + s) * tile + c];
+        R[v] = a.ghostS[v * a.G + g];
+      }
     }
-    const T* L = owner ? own : oth;
-    const T* R = owner ? oth : own;
     T Lr[4] = {L[0], nx * L[1] + ny * L[2], -ny * L[1] + nx * L[2], L[3]};
     T Rr[4] = {R[0], nx * R[1] + ny * R[2], -ny * R[1] + nx * R[2], R[3]};
     T F[4];
@@ -349,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
     T acc[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (lk[s] == kNoSide) continue;
+      if (no_side<S>(lk[s])) continue;
       int slot = s, col = threadIdx.x;
       if (!(faceMask & (1u << s))) {
         // evaluated by the owning side, an element of this block

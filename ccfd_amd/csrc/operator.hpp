@@ -107,6 +107,14 @@ __device__ inline int logical_block() {
   return (b % 8) * (nb8 / 8) + b / 8;
 }
 
+// can slot s of an element with link l be absent? Only in meshes with
+// quadrilaterals: when every element has S = 3 sides all slots exist, and
+// the checks compile away
+template <int S>
+__device__ inline bool no_side(int l) {
+  return S != 3 && l == kNoSide;
+}
+
 // launched block -> first element and element count
 struct BlockPos {
   int lb, base, n;
@@ -173,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
   // least-squares gradient and neighbour extrema, sides in reference order
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (lk[s] == kNoSide) continue;
+    if (no_side<S>(lk[s])) continue;
     T q[4];
     if (lk[s] >= 0) {
       const int j = lk[s] >> 2;
@@ -209,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
     T phi = T(1.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (lk[s] == kNoSide) continue;
+      if (no_side<S>(lk[s])) continue;
       const T d = gx[v] * GPx[s] + gy[v] * GPy[s];
       phi = tmin(phi, limiter_phi<LIM, T>(d, dmax, dmin, dmax * dmax,
                                           dmin * dmin, eps));
@@ -226,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void recon_kernel(OpArgs<T> a) {
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (lk[s] == kNoSide) continue;
+    if (no_side<S>(lk[s])) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       a.side[(v * S + s) * Np + i] = p[v] + GPx[s] * gx[v] + GPy[s] * gy[v];
