@@ -39,12 +39,14 @@ B_FLUX_FP32 = 186.0   # same with FP32 operator data (FP64 U0 / U)
 B_STAGE_FP64 = 572.0
 B_STAGE_FP32 = 330.0
 # what the fused kernel itself must move per element and stage (each array
-# once, side states in LDS): state 32 + links 12 + LSQ weights 48 + GP 48 +
-# eps^2 8 + side normals 48 + signed lengths 24 + 1/area 8 + U0 32 + U 32 +
-# primitive out 32 = 324 B (+32 B dU/dt on the last stage) in FP64; halo
-# re-reads come on top
-B_FUSED_MIN_FP64 = 324.0
-B_FUSED_MIN_FP32 = 200.0
+# once, side states in LDS), FP64: reads state 32 + links 12 + LSQ weights
+# 48 + GP 48 + eps^2 8 + side normals 48 + signed lengths 24 + 1/area 8 +
+# U0 32 = 260, writes the primitive state 32; the last stage also writes U
+# 32 and dU/dt 32. FP32 operator data: 152 + 16, last stage + 32 + 16.
+# Halo re-reads come on top.
+def fused_min_bytes(fp64, stages):
+    per, last = (292.0, 64.0) if fp64 else (168.0, 48.0)
+    return per + last / stages
 
 
 def log(*a):
@@ -409,7 +411,7 @@ def main():
             "kernel_ms": round(ms["flux"], 4),
             "bytes_per_cell": bker}
     if fused:
-        bmin = B_FUSED_MIN_FP64 if prec == G.FP64 else B_FUSED_MIN_FP32
+        bmin = fused_min_bytes(prec == G.FP64, 4)
         roof["compulsory_bytes_per_cell"] = bmin
         roof["compulsory_gbs"] = round(bmin * N / (ms["flux"] * 1e-3) / 1e9, 1)
 
