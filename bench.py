@@ -280,6 +280,60 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
     return out
 
 
+def implicit_run(steps, cfl, ni=512, nj=256):
+    """Implicit Navier-Stokes around a NACA 2412 (m=0.02, p=0.4, t=0.12) on
+    an ni x nj O-grid (outer radius 20 chords, wall spacing 1e-4): Mach 0.5,
+    AoA 2 deg, Re 5000 (mu = rho u c / Re), Pr 0.72, Roe flux, 2nd order
+    with VK (K=5), adiabatic no-slip wall, characteristic far field.
+    Backward-Euler pseudo-time steps with local steps (CFL ramped to the
+    given value), one Newton iteration per step, GMRES (Krylov dimension
+    30) preconditioned by multicolour LU-SGS, in FP64 and in FP32-flux mode
+    (FP32 Krylov basis, Jacobian and sweeps; FP64 Newton residual, Frechet
+    difference, Arnoldi and state)."""
+    import math
+    import numpy as np
+    from ccfd_amd import gpu as G
+    from ccfd_amd import mesh as M
+    mach, aoa, re = 0.5, 2.0, 5000.0
+    t0 = time.time()
+    raw = M.naca_ogrid(ni, nj, wall_bc=201, far_bc=501)
+    mesh = M.Mesh(raw, [M.BC(201), M.BC(501, rho=1.0, mach=mach, alpha=aoa,
+                                         pressure=1.0)])
+    c = math.sqrt(1.4)
+    a = math.radians(aoa)
+    prim = np.tile([1.0, mach * c * math.cos(a), mach * c * math.sin(a), 1.0],
+                   (mesh.n_elems, 1))
+    mu = mach * c / re
+    log("implicit: %d cells, mesh %.1f s" % (mesh.n_elems, time.time() - t0))
+    out = {"cells": mesh.n_elems, "steps": steps, "cfl": cfl, "mu": mu,
+           "scheme": "NS, Roe, 2nd order VK, backward Euler with local "
+                     "pseudo-time steps, 1 Newton iteration / step, GMRES "
+                     "nKdim 30 (tolerance 1e-2), multicolour LU-SGS"}
+    states = {}
+    for name, prec in (("fp64", G.FP64), ("fp32_flux", G.FP32_FLUX)):
+        prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0, navierstokes=1,
+                       mu=mu, Pr=0.72, precision=prec)
+        s = G.Solver(mesh, prm)
+        s.set_prim(prim)
+        cfg = G.ImplicitCfg(30, 1, 2, 1e-3, 1e-2, 0.9)
+        s.sync()
+        w0 = time.perf_counter()
+        r, r0, st, nn, ng = s.implicit_steady(cfg, cfl, steps, 1e-8)
+        s.sync()
+        el = time.perf_counter() - w0
+        states[name] = s.prim()
+        out[name] = {"seconds": el, "ms_per_step": 1e3 * el / st,
+                     "steps": st, "gmres_per_step": ng / st,
+                     "residual_drop": r[0] / r0[0],
+                     "reached_1e-8": bool(r[0] <= 1e-8 * r0[0])}
+        log("implicit %s: %s" % (name, json.dumps(out[name])))
+        s.close()
+    d = states["fp32_flux"][:, 0] - states["fp64"][:, 0]
+    out["fp32_vs_fp64_density_rel_l2"] = float(
+        np.linalg.norm(d) / np.linalg.norm(states["fp64"][:, 0]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,7 +353,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 1 GiB streaming copy (reachable HBM "
                          "bandwidth; counter calibration under rocprofv3)")
-    ap.add_argument("--workload", choices=["operator", "batched", "served"],
+    ap.add_argument("--workload", choices=["operator", "batched", "served",
+                                           "implicit"],
                     default="operator",
                     help="operator: the 4M-triangle stage benchmark (plus a "
                          "short served sample unless --no-served); batched: "
@@ -339,6 +394,21 @@ def main():
                "config": {"workload": "%d instances, %d cells, %d steps"
                           % (r["instances"], r["cells"], r["steps"])},
                "batched": r}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "implicit":
+        r = implicit_run(args.batch_steps if args.batch_steps != 200 else 400,
+                         5.0)
+        out = {"metric": "implicit pseudo-time steps/s (NACA O-grid NS, "
+                         "JFNK-GMRES + LU-SGS)",
+               "value": 1e3 / r["fp32_flux"]["ms_per_step"],
+               "unit": "steps/s", "n_gpus": 1, "steps": r["steps"],
+               "warmup": 0, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32 flux / f64 state",
+               "data": "synthetic (analytic NACA 2412 O-grid, free stream)",
+               "config": {"workload": "512x256 O-grid NS Re 5000, %d steps"
+                          % r["steps"]},
+               "implicit": r}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "served":

@@ -999,7 +999,11 @@ __global__ __launch_bounds__(kBlock) void k_res_partial(int N, int Np,
 extern "C" int ccfd_gpu_residual(ccfd_gpu_ctx* c, double res[4]) {
   if (!c || !c->hasMesh || !res) return fail(CCFD_EINVAL, "no mesh / null res");
   hipSetDevice(c->device);
-  if (c->prm.precision == CCFD_FP64)
+  return residual_of(c, c->prm.precision == CCFD_FP64, res);
+}
+
+int ccfd::residual_of(ccfd_gpu_ctx* c, bool fp64, double res[4]) {
+  if (fp64)
     hipLaunchKernelGGL(k_res_partial<double>, dim3(c->nb), dim3(kBlock), 0, c->st,
                        c->N, c->Np, (const double*)c->ut, c->area, c->part);
   else

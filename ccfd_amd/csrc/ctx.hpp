@@ -97,6 +97,8 @@ int check_params(const ccfd_gpu_params* p, double rk[6]);
 int apply_params(ccfd_gpu_ctx* c, const ccfd_gpu_params* p);
 int upload_venk(ccfd_gpu_ctx* c);
 int refresh_prim(ccfd_gpu_ctx* c);
+// globalResidual of c->ut read as FP64 or as FP32 values
+int residual_of(ccfd_gpu_ctx* c, bool fp64, double res[4]);
 Eq<double> eqd(const ccfd_gpu_ctx* c);
 template <typename T>
 OpArgs<T> op_args(ccfd_gpu_ctx* c);

@@ -55,8 +55,15 @@ struct ImplicitState {
   int* lvlF = nullptr;          // cells of forward levels (CSR items)
   int* lvlB = nullptr;          // backward
   std::vector<int> lvlFStart, lvlBStart;
+  // multicolour sweeps (precond = 2): greedy distance-1 colouring; a cell's
+  // lower neighbours are those of smaller colour
+  bool haveMC = false;
+  int* mcRank = nullptr;        // [Np] colour of each element
+  int* mcCells = nullptr;       // cells by colour (CSR items)
+  std::vector<int> mcStart;
   double* xs = nullptr;         // [4][Np] forward-sweep result
   int* flag = nullptr;          // singular-block flag
+  double* dtLoc = nullptr;      // [Np] local pseudo-time steps (steady solves)
   std::vector<void*> allocs;
 };
 
@@ -98,17 +105,26 @@ __global__ void k_load_state(int N, int Np, const double* __restrict__ X,
 
 // F = U - Q - alpha*dt*ut (src/timeDiscretization.c:441-465, 524-537);
 // optionally also XK = U, RXK = ut
+// alpha*dt of element i: the global step, or alpha times the element's own
+// pseudo-time step in steady solves
+__device__ inline double elem_adt(double adt, const double* dtv, double alpha,
+                                  int i) {
+  return dtv ? alpha * dtv[i] : adt;
+}
+
 template <typename T>
 __global__ void k_newton_residual(int N, int Np, const double* __restrict__ U,
                                   const double* __restrict__ Q,
                                   const T* __restrict__ ut, double adt,
+                                  const double* __restrict__ dtv, double alpha,
                                   double* __restrict__ F, double* __restrict__ XK,
                                   double* __restrict__ RXK) {
   FOR_CELLS
+  const double a = elem_adt(adt, dtv, alpha, i);
   for (int v = 0; v < 4; ++v) {
     int k = v * Np + i;
     double u = double(ut[k]);
-    F[k] = U[k] - Q[k] - adt * u;
+    F[k] = U[k] - Q[k] - a * u;
     if (XK) XK[k] = U[k];
     if (RXK) RXK[k] = u;
   }
@@ -203,12 +219,14 @@ template <typename T, typename K>
 __global__ void k_jv_diff(int N, int Np, const K* __restrict__ v,
                           const T* __restrict__ ut, const double* __restrict__ RXK,
                           const double* __restrict__ epsp, double adt,
+                          const double* __restrict__ dtv, double alpha,
                           double* __restrict__ res) {
   FOR_CELLS
   const double eps = *epsp;
+  const double a = elem_adt(adt, dtv, alpha, i);
   for (int k = 0; k < 4; ++k) {
     int o = k * Np + i;
-    res[o] = double(v[o]) - adt * (double(ut[o]) - RXK[o]) / eps;
+    res[o] = double(v[o]) - a * (double(ut[o]) - RXK[o]) / eps;
   }
 }
 
@@ -247,6 +265,24 @@ __global__ void k_fd_perturb(int N, int Np, const double* __restrict__ base,
   }
 }
 
+// prim = consPrim(XK) with the conservative variable iv of the elements of
+// colour c perturbed (Jacobian with respect to the conservative state, which
+// the LU-SGS sweeps apply it to; the multicolour preconditioner uses it)
+template <typename T>
+__global__ void k_fd_perturb_cons(int N, int Np, const double* __restrict__ XK,
+                                  const int* __restrict__ color, int c, int iv,
+                                  double reps, T* __restrict__ prim,
+                                  Eq<double> q) {
+  FOR_CELLS
+  double u[4], p[4];
+  for (int v = 0; v < 4; ++v) {
+    u[v] = XK[v * Np + i];
+    if (v == iv && color[i] == c) u[v] += reps;
+  }
+  cons_prim<double>(q, u, p);
+  for (int v = 0; v < 4; ++v) prim[v * Np + i] = T(p[v]);
+}
+
 // record (ut_j - RXK_j) / sqrt(eps) into the diagonal block (j perturbed) or
 // the block of the side of j facing the perturbed neighbour
 // (src/linearSolver.c:203-222)
@@ -278,9 +314,11 @@ __global__ void k_fd_record(int N, int Np, int S, const int* __restrict__ link,
 // (src/linearSolver.c:226-251, 136-179)
 template <typename M>
 __global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
-                            double dt, M* __restrict__ Dg, M* __restrict__ Blk,
+                            double dt0, const double* __restrict__ dtv,
+                            M* __restrict__ Dg, M* __restrict__ Blk,
                             M* __restrict__ Dinv, int* flag) {
   FOR_CELLS
+  const double dt = dtv ? dtv[i] : dt0;
   double A[4][4];
   for (int a = 0; a < 4; ++a)
     for (int b = 0; b < 4; ++b) {
@@ -386,6 +424,28 @@ __global__ void k_lusgs_bwd(int n, const int* __restrict__ cells, int Np, int S,
     for (int c = 0; c < 4; ++c) t2 += Dinv[(4 * a + c) * Np + i] * t1[c];
     x[a * Np + i] = K(xs[a * Np + i] - t2);
   }
+}
+
+// local pseudo-time step of every element: the CFL (and, with viscosity,
+// diffusive) limit of calcTimeStep (src/timeDiscretization.c:253-308) per
+// element instead of its minimum
+template <typename T>
+__global__ void k_local_dt(int N, int Np, const T* __restrict__ prim,
+                           const double* __restrict__ area,
+                           const double* __restrict__ sx,
+                           const double* __restrict__ sy, double gam, double mu,
+                           double gamPrMax, double cfl, double dfl,
+                           double* __restrict__ dtv) {
+  FOR_CELLS
+  const double r = double(prim[i]), u = double(prim[Np + i]);
+  const double v = double(prim[2 * Np + i]), p = double(prim[3 * Np + i]);
+  const double a = sqrt(gam * p / r);
+  double dt = cfl * area[i] / ((fabs(u) + a) * sx[i] + (fabs(v) + a) * sy[i]);
+  if (mu > 1e-10) {
+    const double ssv = gamPrMax * mu * sx[i] * sx[i] + gamPrMax * mu * sy[i] * sy[i];
+    dt = fmin(dt, dfl * r * r * area[i] * area[i] / (4.0 * ssv));
+  }
+  dtv[i] = dt;
 }
 
 // --------------------------------------------------------------- host side
@@ -540,6 +600,53 @@ static int prepare_lusgs(ccfd_gpu_ctx* c, int order) {
   return CCFD_OK;
 }
 
+// multicolour sweep schedule (precond = 2): greedy distance-1 colouring in
+// element order; the sweeps run colour by colour, lower neighbours being
+// those of smaller colour
+static int prepare_multicolor(ccfd_gpu_ctx* c) {
+  ImplicitState* s = c->imp;
+  if (s->haveMC) return CCFD_OK;
+  const int N = c->N, Np = c->Np, S = c->S;
+  std::vector<int> link((size_t)S * Np);
+  if (hipMemcpyAsync(link.data(), c->link, link.size() * 4,
+                     hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "link download failed");
+  std::vector<int> col(N, -1);
+  std::vector<char> used;
+  int ncol = 0;
+  for (int i = 0; i < N; ++i) {
+    used.assign(ncol + 1, 0);
+    for (int k = 0; k < S; ++k) {
+      const int lk = link[(size_t)k * Np + i];
+      if (lk >= 0 && col[lk >> 2] >= 0) used[col[lk >> 2]] = 1;
+    }
+    int cc = 0;
+    while (cc < ncol && used[cc]) ++cc;
+    col[i] = cc;
+    if (cc == ncol) ++ncol;
+  }
+  s->mcStart.assign(ncol + 1, 0);
+  for (int i = 0; i < N; ++i) s->mcStart[col[i] + 1]++;
+  for (int k = 0; k < ncol; ++k) s->mcStart[k + 1] += s->mcStart[k];
+  std::vector<int> items(N), pos(s->mcStart.begin(), s->mcStart.end() - 1);
+  for (int i = 0; i < N; ++i) items[pos[col[i]]++] = i;
+  std::vector<int> rank(Np, 0);
+  for (int i = 0; i < N; ++i) rank[i] = col[i];
+  s->mcRank = (int*)imalloc(c, s, Np * 4);
+  s->mcCells = (int*)imalloc(c, s, (size_t)std::max(N, 1) * 4);
+  if (!s->mcRank || !s->mcCells)
+    return fail(CCFD_ENOMEM, "multicolour schedule allocation failed");
+  if (hipMemcpyAsync(s->mcRank, rank.data(), Np * 4, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipMemcpyAsync(s->mcCells, items.data(), N * 4, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "multicolour schedule upload failed");
+  s->haveMC = true;
+  return CCFD_OK;
+}
+
 template <typename T>
 static void operator_at(ccfd_gpu_ctx* c, double time) {
   OpArgs<T> a = op_args<T>(c);
@@ -561,8 +668,11 @@ static double dot_host(ccfd_gpu_ctx* c, const double* a, const double* b) {
 }
 
 // buildMatrix by coloured finite differences at the primitive state of XK
+// consJac: perturb the conservative state (precond = 2) instead of the
+// primitive state as buildMatrix does (src/linearSolver.c:199-201)
 template <typename T, typename M>
-static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt) {
+static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
+                        const double* dtv, bool consJac) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np, S = c->S;
   const double reps = c->prm.precision == CCFD_FP64 ? sqrt(DBL_EPSILON)
@@ -577,9 +687,14 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt) {
                      N, Np, s->XK, c->U, s->primBase, eqd(c));
   for (int col = 0; col < s->nColors; ++col) {
     for (int iv = 0; iv < 4; ++iv) {
-      hipLaunchKernelGGL(k_fd_perturb<T>, cells_grid(c), dim3(256), 0, c->st, N,
-                         Np, s->primBase, s->color, col, iv, reps,
-                         (T*)c->prim[c->cur]);
+      if (consJac)
+        hipLaunchKernelGGL(k_fd_perturb_cons<T>, cells_grid(c), dim3(256), 0,
+                           c->st, N, Np, s->XK, s->color, col, iv, reps,
+                           (T*)c->prim[c->cur], eqd(c));
+      else
+        hipLaunchKernelGGL(k_fd_perturb<T>, cells_grid(c), dim3(256), 0, c->st,
+                           N, Np, s->primBase, s->color, col, iv, reps,
+                           (T*)c->prim[c->cur]);
       operator_at<T>(c, tGlobal);
       hipLaunchKernelGGL((k_fd_record<T, M>), cells_grid(c), dim3(256), 0,
                          c->st, N, Np, S, c->link, s->color, col, iv,
@@ -587,7 +702,8 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt) {
     }
   }
   hipLaunchKernelGGL(k_fd_finish<M>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                     S, c->link, dt, (M*)s->Dg, (M*)s->Blk, (M*)s->Dinv, s->flag);
+                     S, c->link, dt, dtv, (M*)s->Dg, (M*)s->Blk, (M*)s->Dinv,
+                     s->flag);
   int flag = 0;
   hipMemcpyAsync(&flag, s->flag, 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
@@ -598,9 +714,27 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt) {
 }
 
 template <typename M, typename K>
-static void lusgs(ccfd_gpu_ctx* c, const K* b, K* x) {
+static void lusgs(ccfd_gpu_ctx* c, int mode, const K* b, K* x) {
   ImplicitState* s = c->imp;
   const int Np = c->Np, S = c->S;
+  if (mode == 2) {
+    const int nc = (int)s->mcStart.size() - 1;
+    for (int l = 0; l < nc; ++l) {
+      const int n = s->mcStart[l + 1] - s->mcStart[l];
+      hipLaunchKernelGGL((k_lusgs_fwd<M, K>), dim3((n + 127) / 128), dim3(128), 0,
+                         c->st, n, s->mcCells + s->mcStart[l], Np, S, c->link,
+                         s->mcRank, (const M*)s->Blk, (const M*)s->Dinv, b,
+                         (M*)s->xs);
+    }
+    for (int l = nc - 1; l >= 0; --l) {
+      const int n = s->mcStart[l + 1] - s->mcStart[l];
+      hipLaunchKernelGGL((k_lusgs_bwd<M, K>), dim3((n + 127) / 128), dim3(128), 0,
+                         c->st, n, s->mcCells + s->mcStart[l], Np, S, c->link,
+                         s->mcRank, (const M*)s->Blk, (const M*)s->Dinv,
+                         (const M*)s->xs, x);
+    }
+    return;
+  }
   for (size_t l = 0; l + 1 < s->lvlFStart.size(); ++l) {
     int n = s->lvlFStart[l + 1] - s->lvlFStart[l];
     hipLaunchKernelGGL((k_lusgs_fwd<M, K>), dim3((n + 127) / 128), dim3(128), 0,
@@ -620,8 +754,8 @@ static void lusgs(ccfd_gpu_ctx* c, const K* b, K* x) {
 // GMRES_M (src/linearSolver.c:385-510) with B = s->B, result in s->dX
 template <typename T, typename K>
 static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
-                 double tGlobal, double time, double dt, double alpha,
-                 double normB, int* nIter) {
+                 double tGlobal, double time, double dt, const double* dtv,
+                 double alpha, double normB, int* nIter) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np, nK = cfg->nKdim;
   const size_t vs = 4 * (size_t)Np;
@@ -637,9 +771,10 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
       Cg(nK + 1, 0.0), Sg(nK + 1, 0.0), alp(nK + 1, 0.0);
   gm[0] = normB;
   if (cfg->precond) {
+    const bool consJac = cfg->precond == 2;
     int rc = c->prm.precision == CCFD_FP64
-                 ? build_matrix<T, double>(c, tGlobal, dt)
-                 : build_matrix<T, float>(c, tGlobal, dt);
+                 ? build_matrix<T, double>(c, tGlobal, dt, dtv, consJac)
+                 : build_matrix<T, float>(c, tGlobal, dt, dtv, consJac);
     if (rc) return rc;
   }
 #define HM(i, j) H[(i) * (nK + 1) + (j)]
@@ -649,9 +784,9 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     K* Zm = Z + m * vs;
     if (cfg->precond) {
       if (c->prm.precision == CCFD_FP64)
-        lusgs<double, K>(c, Vm, Zm);
+        lusgs<double, K>(c, cfg->precond, Vm, Zm);
       else
-        lusgs<float, K>(c, Vm, Zm);
+        lusgs<float, K>(c, cfg->precond, Vm, Zm);
     } else {
       hipMemcpyAsync(Zm, Vm, vs * sizeof(K), hipMemcpyDeviceToDevice, c->st);
     }
@@ -666,7 +801,8 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
                        epsFD, eqd(c));
     operator_at<T>(c, time);
     hipLaunchKernelGGL((k_jv_diff<T, K>), cells_grid(c), dim3(256), 0, c->st, N,
-                       Np, Zm, (const T*)c->ut, s->RXK, epsFD, alpha * dt, s->W);
+                       Np, Zm, (const T*)c->ut, s->RXK, epsFD, alpha * dt, dtv,
+                       alpha, s->W);
     // modified Gram-Schmidt (src/linearSolver.c:440-450)
     for (int n = 0; n <= m; ++n) {
       K* Vn = V + n * vs;
@@ -719,30 +855,61 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
 
 template <typename T>
 static int gmres_any(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
-                     double tGlobal, double time, double dt, double alpha,
-                     double normB, int* nIter) {
+                     double tGlobal, double time, double dt, const double* dtv,
+                     double alpha, double normB, int* nIter) {
   if (c->prm.precision == CCFD_FP64)
-    return gmres<T, double>(c, cfg, tGlobal, time, dt, alpha, normB, nIter);
-  return gmres<T, float>(c, cfg, tGlobal, time, dt, alpha, normB, nIter);
+    return gmres<T, double>(c, cfg, tGlobal, time, dt, dtv, alpha, normB, nIter);
+  return gmres<T, float>(c, cfg, tGlobal, time, dt, dtv, alpha, normB, nIter);
 }
 
+// Newton residual F(X) = X - Q - alpha*dt*R(X) in FP64 (the outer residual
+// of the defect correction: the converged state is the FP64 operator's
+// whatever the precision of the Krylov / preconditioner work). U <- X,
+// XK <- X when XKout, RXK <- R(X); ut keeps R(X) in FP64.
+static void outer_residual(ccfd_gpu_ctx* c, const double* X, double time,
+                           double adt, const double* dtv, double alpha,
+                           double* XKout) {
+  ImplicitState* s = c->imp;
+  hipLaunchKernelGGL(k_load_state<double>, cells_grid(c), dim3(256), 0, c->st,
+                     c->N, c->Np, X, c->U, (double*)c->prim[c->cur], eqd(c));
+  operator_at<double>(c, time);
+  hipLaunchKernelGGL(k_newton_residual<double>, cells_grid(c), dim3(256), 0,
+                     c->st, c->N, c->Np, c->U, s->Q, (const double*)c->ut, adt,
+                     dtv, alpha, s->FXK, XKout, s->RXK);
+}
+
+// FP32-flux mode: R(XK) with the FP32 operator as the base of the Frechet
+// differences and the finite-difference Jacobian (differences of one
+// arithmetic), from U = XK
+static int base_residual_fp32(ccfd_gpu_ctx* c, double time) {
+  ImplicitState* s = c->imp;
+  int rc = refresh_prim(c);
+  if (rc) return rc;
+  operator_at<float>(c, time);
+  const long n = 4 * (long)c->Np;
+  hipLaunchKernelGGL((k_copy_cvt<float, double>), dim3((n + 255) / 256),
+                     dim3(256), 0, c->st, n, (const float*)c->ut, s->RXK);
+  return CCFD_OK;
+}
+
+// dtv (may be null): per-element pseudo-time steps instead of dt; with
+// steady set, a Newton loop that stops at nNewtonIter is accepted (one
+// Newton iteration per pseudo-time step is the usual steady iteration).
+// The Newton residual is FP64 in both precisions; in FP32-flux mode the
+// GMRES operator evaluations, the Jacobian and the LU-SGS sweeps are FP32.
 template <typename T>
 static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
-                           double t, double dt, double res[4], int* nNewton,
-                           int* nGMRES) {
+                           double t, double dt, const double* dtv, bool steady,
+                           double res[4], int* nNewton, int* nGMRES) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np;
   const size_t vs = 4 * (size_t)Np;
   const double alpha = 1.0, beta = 1.0;
+  const bool mixed = sizeof(T) == 4;
   // Q = U; prim = consPrim(U) (src/timeDiscretization.c:418-428)
   hipMemcpyAsync(s->Q, c->U, vs * 8, hipMemcpyDeviceToDevice, c->st);
-  hipLaunchKernelGGL(k_load_state<T>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                     s->Q, c->U, (T*)c->prim[c->cur], eqd(c));
   const double time = t + beta * dt;
-  operator_at<T>(c, time);
-  hipLaunchKernelGGL(k_newton_residual<T>, cells_grid(c), dim3(256), 0, c->st, N,
-                     Np, c->U, s->Q, (const T*)c->ut, alpha * dt, s->FXK, s->XK,
-                     s->RXK);
+  outer_residual(c, s->Q, time, alpha * dt, dtv, alpha, s->XK);
   const double n2F0 = dot_host(c, s->FXK, s->FXK);
   const double eps2 = cfg->epsNewton * cfg->epsNewton;
   const double eps2loc = fmax(1e-8 * 1e-8 * N * eps2 / n2F0, eps2);
@@ -750,34 +917,36 @@ static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   int nIn = 0;
   if (cfg->precond) {
     int rc = prepare_lusgs(c, c->prm.spatialOrder);
+    if (!rc && cfg->precond == 2) rc = prepare_multicolor(c);
     if (rc) return rc;
   }
   while ((n2FK > eps2loc * n2F0) && (nIn < cfg->nNewtonIter)) {
     nIn++;
+    if (mixed) {
+      int rc = base_residual_fp32(c, time);
+      if (rc) return rc;
+    }
     hipMemcpyAsync(s->B, s->FXK, vs * 8, hipMemcpyDeviceToDevice, c->st);
-    int rc = gmres_any<T>(c, cfg, t, time, dt, alpha, sqrt(n2FK), nGMRES);
+    int rc = gmres_any<T>(c, cfg, t, time, dt, dtv, alpha, sqrt(n2FK), nGMRES);
     if (rc) return rc;
     // XK += dX; U = XK; prim = consPrim(U) (src/timeDiscretization.c:505-520)
     hipLaunchKernelGGL(k_axpy, cells_grid(c), dim3(256), 0, c->st, N, Np, s->XK,
                        s->dX);
-    hipLaunchKernelGGL(k_load_state<T>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                       s->XK, c->U, (T*)c->prim[c->cur], eqd(c));
-    operator_at<T>(c, time);
-    hipLaunchKernelGGL(k_newton_residual<T>, cells_grid(c), dim3(256), 0, c->st, N,
-                       Np, c->U, s->Q, (const T*)c->ut, alpha * dt, s->FXK,
-                       (double*)nullptr, s->RXK);
+    outer_residual(c, s->XK, time, alpha * dt, dtv, alpha, nullptr);
     n2FK = dot_host(c, s->FXK, s->FXK);
   }
   *nNewton += nIn;
-  if (nIn == cfg->nNewtonIter)
+  if (nIn == cfg->nNewtonIter && !steady)
     return fail(CCFD_ENOCONV,
                 "Newton NOT converged with %d Newton iterations (|F|^2/|F0|^2 = %g)",
                 nIn, n2FK / n2F0);
+  // globalResidual of the FP64 dU/dt of the new state (ut holds it)
   if (res) {
-    int rc = ccfd_gpu_residual(c, res);
+    int rc = residual_of(c, true, res);
     if (rc) return rc;
   }
-  return CCFD_OK;
+  // leave the primitive state in the context's arithmetic
+  return mixed ? refresh_prim(c) : CCFD_OK;
 }
 
 extern "C" int ccfd_gpu_implicit_step(ccfd_gpu_ctx* c,
@@ -792,8 +961,66 @@ extern "C" int ccfd_gpu_implicit_step(ccfd_gpu_ctx* c,
   int rc = ensure(c, cfg->nKdim);
   if (rc) return rc;
   return c->prm.precision == CCFD_FP64
-             ? implicit_step_t<double>(c, cfg, t, dt, res, nNewton, nGMRES)
-             : implicit_step_t<float>(c, cfg, t, dt, res, nNewton, nGMRES);
+             ? implicit_step_t<double>(c, cfg, t, dt, nullptr, false, res, nNewton,
+                                       nGMRES)
+             : implicit_step_t<float>(c, cfg, t, dt, nullptr, false, res, nNewton,
+                                      nGMRES);
+}
+
+template <typename T>
+static int steady_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
+                    double cfl, int maxSteps, double drop, double res[4],
+                    double res0[4], int* steps, int* nNewton, int* nGMRES) {
+  ImplicitState* s = c->imp;
+  if (!s->dtLoc) {
+    s->dtLoc = (double*)imalloc(c, s, (size_t)c->Np * 8);
+    if (!s->dtLoc) return fail(CCFD_ENOMEM, "local step allocation failed");
+  }
+  const Eq<double> q = eqd(c);
+  const double gamPrMax = fmax(4.0 / 3.0, q.gam / q.Pr);
+  const double mu = c->prm.navierstokes ? q.mu : 0.0;
+  double cflk = fmin(cfl, 1.0);
+  for (int k = 0; k < maxSteps; ++k) {
+    // pseudo-time steps from the current state; the CFL number ramps up
+    // from 1 by 20% per step
+    int rc = refresh_prim(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_local_dt<T>, cells_grid(c), dim3(256), 0, c->st, c->N,
+                       c->Np, (const T*)c->prim[c->cur], c->area, c->sx, c->sy,
+                       q.gam, mu, gamPrMax, cflk, cflk, s->dtLoc);
+    cflk = fmin(cfl, 1.2 * cflk);
+    rc = implicit_step_t<T>(c, cfg, 0.0, 0.0, s->dtLoc, true, res, nNewton,
+                            nGMRES);
+    if (rc) return rc;
+    *steps = k + 1;
+    if (k == 0)
+      for (int v = 0; v < 4; ++v) res0[v] = res[v];
+    if (!(res[0] == res[0])) return fail(CCFD_ENONFINITE, "non-finite residual");
+    if (res[0] <= drop * res0[0]) break;
+  }
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_implicit_steady(ccfd_gpu_ctx* c,
+                                        const ccfd_gpu_implicit_cfg* cfg,
+                                        double cfl, int maxSteps, double drop,
+                                        double res[4], double res0[4],
+                                        int* steps, int* nNewton, int* nGMRES) {
+  if (!c || !c->hasMesh || !cfg || !res || !res0 || !steps || !nNewton ||
+      !nGMRES)
+    return fail(CCFD_EINVAL, "bad steady-solve arguments");
+  if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->nNewtonIter < 1 ||
+      cfg->precond < 0 || cfg->precond > 2 || !(cfl > 0.0) || maxSteps < 1)
+    return fail(CCFD_EINVAL, "bad steady-solve settings");
+  hipSetDevice(c->device);
+  int rc = ensure(c, cfg->nKdim);
+  if (rc) return rc;
+  *steps = 0;
+  return c->prm.precision == CCFD_FP64
+             ? steady_t<double>(c, cfg, cfl, maxSteps, drop, res, res0, steps,
+                                nNewton, nGMRES)
+             : steady_t<float>(c, cfg, cfl, maxSteps, drop, res, res0, steps,
+                               nNewton, nGMRES);
 }
 
 extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
@@ -818,7 +1045,7 @@ extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     operator_at<T>(c, time);
     hipLaunchKernelGGL((k_copy_cvt<T, double>), dim3((4 * Np + 255) / 256),
                        dim3(256), 0, c->st, (long)4 * Np, (const T*)c->ut, s->RXK);
-    return gmres_any<T>(c, cfg, tGlobal, time, dt, alpha, normB, nIter);
+    return gmres_any<T>(c, cfg, tGlobal, time, dt, nullptr, alpha, normB, nIter);
   };
   rc = c->prm.precision == CCFD_FP64 ? run(double(0)) : run(float(0));
   if (rc) return rc;

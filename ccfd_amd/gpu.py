@@ -90,6 +90,10 @@ def lib():
                                              C.c_long, dp, dp, dp]),
         "ccfd_gpu_synchronize": (C.c_int, [vp]),
         "ccfd_gpu_set_variant": (C.c_int, [vp, C.c_int]),
+        "ccfd_gpu_implicit_steady": (C.c_int, [vp, C.POINTER(ImplicitCfg),
+                                               C.c_double, C.c_int,
+                                               C.c_double, dp, dp, ip, ip,
+                                               ip]),
         "ccfd_gpu_get_variant": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
                                              C.c_double, dp, ip]),
@@ -230,6 +234,16 @@ class Solver:
                                             N.dptr(r), C.byref(nn),
                                             C.byref(ng)))
         return r, nn.value, ng.value
+
+    def implicit_steady(self, cfg, cfl, max_steps, drop):
+        """implicit pseudo-time steps with local steps until the density
+        residual drops by `drop`; returns (res, res0, steps, newton, gmres)"""
+        r, r0 = np.zeros(4), np.zeros(4)
+        st, nn, ng = C.c_int(0), C.c_int(0), C.c_int(0)
+        check(self.L.ccfd_gpu_implicit_steady(
+            self.h, C.byref(cfg), cfl, max_steps, drop, N.dptr(r), N.dptr(r0),
+            C.byref(st), C.byref(nn), C.byref(ng)))
+        return r, r0, st.value, nn.value, ng.value
 
     def sync(self):
         check(self.L.ccfd_gpu_synchronize(self.h))

@@ -134,7 +134,14 @@ int ccfd_gpu_residual(ccfd_gpu_ctx *ctx, double res[4]);
 typedef struct ccfd_gpu_implicit_cfg {
 	int nKdim;              /* Krylov dimension (single cycle, as GMRES_M) */
 	int nNewtonIter;        /* Newton iteration limit */
-	int precond;            /* 1: LU-SGS preconditioner */
+	int precond;            /* 1: LU-SGS preconditioner, sweeps in the
+	                           reference's element order on buildMatrix's
+	                           Jacobian; 2: LU-SGS with multicolour sweeps
+	                           (greedy colouring of the face graph, a few
+	                           launches per sweep instead of one per
+	                           dependency level) on the Jacobian with
+	                           respect to the conservative state: a
+	                           different, stronger preconditioner */
 	double epsNewton;       /* relative Newton tolerance */
 	double epsGMRES;        /* relative GMRES tolerance */
 	double gammaEW;         /* Eisenstat-Walker gamma */
@@ -144,6 +151,22 @@ typedef struct ccfd_gpu_implicit_cfg {
  * matrixVector, buildMatrix and LUSGS (src/linearSolver.c:136-510) */
 int ccfd_gpu_implicit_step(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
 		double t, double dt, double res[4], int *nNewton, int *nGMRES);
+
+/*
+ * Steady solve by implicit pseudo-time stepping: backward-Euler steps as
+ * ccfd_gpu_implicit_step, each element with its own pseudo-time step (the
+ * CFL and diffusive limits of calcTimeStep per element; the CFL number
+ * starts at min(cfl, 1) and grows by 20% per step up to cfl), from the
+ * resident state until the density residual drops by `drop` relative to
+ * the first step, or maxSteps. A step stops its Newton loop at
+ * nNewtonIter without failing (one Newton iteration per step is the usual
+ * choice). res / res0: residual of the last / first step; *steps: steps
+ * taken. Local pseudo-time steps are not part of the reference, which
+ * steps with one global time step.
+ */
+int ccfd_gpu_implicit_steady(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
+		double cfl, int maxSteps, double drop, double res[4],
+		double res0[4], int *steps, int *nNewton, int *nGMRES);
 
 /*
  * GMRES_M (src/linearSolver.c:385-510) on a host right-hand side B (4 doubles

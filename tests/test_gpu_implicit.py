@@ -109,3 +109,54 @@ def test_newton_failure_is_reported():
     with pytest.raises(G.GpuError, match="Newton NOT converged"):
         gs.implicit_step(G.ImplicitCfg(5, 1, 0, 1e-12, 1e-3, 0.9), 0.0, dt)
     gs.close()
+
+
+def test_multicolour_lusgs_converges_to_the_same_step():
+    """precond = 2 (multicolour sweeps, conservative Jacobian) is another
+    preconditioner: the Newton solution of each backward-Euler step is the
+    same to the Newton tolerance."""
+    mesh, cfg, prim, c = TD.setup_case(SOD_IMPLICIT)
+    prm = G.params(gamma=c["gam"], flux=c["flux"], order=c["order"],
+                   limiter=c["limiter"], venk_k=c["venk_k"], cfl=c["cfl"],
+                   ts1d=int(c["ts1d"]), nrk=c["nRK"], torder=c["timeOrder"])
+    out = []
+    for pc in (1, 2):
+        gs = G.Solver(mesh, prm)
+        gs.set_prim(prim)
+        t, ng = 0.0, 0
+        for _ in range(5):
+            dt, _ = gs.calc_dt(t, c["IOtimeInterval"], c["tEnd"])
+            _, nn, gg = gs.implicit_step(G.ImplicitCfg(c["nKdim"], 25, pc, 1e-9,
+                                                       1e-3, 0.9), t, dt)
+            ng += gg
+            t += dt
+        out.append((gs.cons(), ng))
+        gs.close()
+    err = cases.rel_err(out[1][0], out[0][0])
+    print("reference order vs multicolour: rel err %.2e, GMRES %d / %d"
+          % (err, out[0][1], out[1][1]))
+    assert err < 1e-7
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32_FLUX])
+def test_implicit_steady_reduces_the_residual(precision):
+    """local pseudo-time steps on an Euler O-grid around a NACA 2412 at
+    Mach 0.5: the residual falls by orders of magnitude, and the FP32-flux
+    mode (FP64 Newton residual) follows the FP64 solve"""
+    raw = M.naca_ogrid(48, 16, wall_bc=101, far_bc=501)
+    bcs = [M.BC(101), M.BC(501, rho=1.0, mach=0.5, alpha=2.0, pressure=1.0)]
+    mesh = M.Mesh(raw, bcs)
+    c = np.sqrt(1.4)
+    a = np.radians(2.0)
+    prim = np.tile([1.0, 0.5 * c * np.cos(a), 0.5 * c * np.sin(a), 1.0],
+                   (mesh.n_elems, 1))
+    prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0, precision=precision)
+    gs = G.Solver(mesh, prm)
+    gs.set_prim(prim)
+    r, r0, steps, nn, ng = gs.implicit_steady(G.ImplicitCfg(20, 1, 2, 1e-3,
+                                                           1e-2, 0.9),
+                                              20.0, 200, 1e-10)
+    print("steps %d drop %.2e GMRES %d" % (steps, r[0] / r0[0], ng))
+    assert steps == 200 and nn == 200
+    assert r[0] / r0[0] < 1e-2
+    gs.close()
