@@ -3,11 +3,15 @@ by tests/golden/make_golden.py from the reference's compiled sources).
 Bit-for-bit: every flux function on 1010 state pairs, and every
 fvTimeDerivative output (dU/dt, limited gradients, side states, side
 fluxes) on 14 meshes / settings."""
+import os
+
 import numpy as np
 import pytest
 
 from oracle import bindings as B
 from tests import golden_io
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 CASES = golden_io.operator_cases()
 
@@ -49,3 +53,36 @@ def test_fixture_sanity():
     for case in CASES.values():
         codes |= set(int(r[0]) // 100 for r in case["bcs"])
     assert {1, 2, 3, 4, 5, 6, 7, 8} <= codes
+
+
+def test_reference_harness_keeps_its_own_fvtimederivative():
+    """The drop-in library exports fvTimeDerivative too; with both loaded in
+    one process (as bench.py's CPU baseline does) the reference harness must
+    still call its own. Run in a child process: the drop-in's version would
+    exit(1)."""
+    import subprocess
+    import sys
+    from oracle import bindings as B
+    if not B.ref_available(False):
+        pytest.skip("reference harness not built")
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from ccfd_amd import mesh as M\n"
+        "from oracle import bindings as B\n"
+        "raw, bcs = M.jittered_delaunay(12, 0.3, 2, periodic=True)\n"
+        "mesh = M.Mesh(raw, bcs)\n"
+        "B.write_ini('/tmp/ccfd_sym_%%d.ini' %% __import__('os').getpid(), "
+        "[('fluxFunction', 5), ('nBC', 4), ('BCtype', 701), "
+        "('connection', (0.0, 1.0)), ('BCtype', 702), "
+        "('connection', (0.0, 0.0)), ('BCtype', 711), "
+        "('connection', (1.0, 0.0)), ('BCtype', 712), "
+        "('connection', (0.0, 0.0)), ('spatialOrder', 2), ('limiter', 2), "
+        "('venk_k', 5.0)])\n"
+        "ref = B.Ref(omp=True)\n"
+        "ref.setup('/tmp/ccfd_sym_%%d.ini' %% __import__('os').getpid(), mesh)\n"
+        "ref.set_prim(M.smooth_state(mesh.bary, seed=2))\n"
+        "ref.fvtd_only(0.0)\n"
+        "print('REF-OK')\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0 and "REF-OK" in r.stdout, r.stdout[-500:]
