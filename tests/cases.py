@@ -58,3 +58,25 @@ def rel_err(a, b):
     scale (the scale of dU/dt differs by variable)"""
     scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
     return float((np.abs(a - b) / scale).max())
+
+
+def split_quads(raw, seed=0, frac=0.5):
+    """a mixed mesh: a seeded random subset of the quadrilaterals of `raw`
+    split into two triangles along a diagonal (orientation kept)"""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    q = raw.quad
+    pick = rng.uniform(size=len(q)) < frac
+    keep = q[~pick]
+    sq = q[pick]
+    alt = rng.uniform(size=len(sq)) < 0.5
+    t1 = np.where(alt[:, None], sq[:, [0, 1, 2, 4]], sq[:, [0, 1, 3, 4]])
+    t2 = np.where(alt[:, None], sq[:, [0, 2, 3, 4]], sq[:, [1, 2, 3, 4]])
+    tri = np.concatenate([t1, t2])
+    return M.Raw(raw.xy, tri, keep, raw.bcedge)
+
+
+def mixed_mesh(seed=3):
+    """the mixed-BC Cartesian mesh with about half its quadrilaterals split
+    into triangles: elements with 3 and 4 sides side by side"""
+    base, bcs = mixed_bc_mesh()
+    return M.Mesh(split_quads(base.raw, seed=seed), bcs), bcs

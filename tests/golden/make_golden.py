@@ -186,6 +186,13 @@ def main():
     run_case("mixedbc_o2_vk5_roe", mix, mbcs, p_mx, 2, 2, 2, 5.0, out=out)
     run_case("mixedbc_o2_bj_ausmd", mix, mbcs, p_mx, 9, 2, 1, out=out)
 
+    # mixed triangles and quadrilaterals (absent slots in the device layout)
+    mixq = cases.split_quads(mix, seed=3)
+    mqm = M.Mesh(mixq, mbcs)
+    p_mq = cases.random_state(mqm.n_elems, seed=13, mach=0.5)
+    run_case("mixedmesh_o2_vk5_hllc", mixq, mbcs, p_mq, 5, 2, 2, 5.0, out=out)
+    run_case("mixedmesh_o1_roe", mixq, mbcs, p_mq, 2, 1, out=out)
+
     # exact-function boundaries (Sod Riemann problem at t = 0.1)
     ex = M.cartesian(50, 1, (0.0, 0.0), (1.0, 0.02), (101, 601, 101, 601))
     ebcs = [BCspec(101), BCspec(601, exact_func=5)]

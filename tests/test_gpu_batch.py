@@ -61,6 +61,38 @@ def test_instances_equal_single_contexts(prec, variant):
     b.close()
 
 
+@pytest.mark.parametrize("variant", [G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE],
+                         ids=["split", "fused", "fused_face"])
+def test_four_sided_arena_mixes_triangle_and_mixed_meshes(variant):
+    """a maxSides=4 arena holds a mixed triangle/quadrilateral mesh next to
+    an all-triangle one (padded to four slots); both stay bit-identical to
+    their single-mesh contexts, which lay the triangle mesh out with three
+    slots"""
+    sch = _scheme(G.FP64)
+    mixed, _ = cases.mixed_mesh()
+    tm, tp = _cases((16,))[0]
+    mp = cases.random_state(mixed.n_elems, seed=23, mach=0.5)
+    b = BT.Batch(sch, max_sides=4, hbm_bytes=64 * MB)
+    b.set_variant(variant)
+    assert b.variant() == variant
+    ids = [b.add(mixed, mp), b.add(tm, tp)]
+    b.step(5)
+    b.sync()
+    for k, (m, p) in zip(ids, [(mixed, mp), (tm, tp)]):
+        s = G.Solver(m, sch)
+        s.set_variant(variant)
+        s.set_prim(p)
+        t, r = s.advance(5, 0.0, 1e300, 1e300)
+        inf = b.info(k)
+        print("instance %d: %d cells t %.6e" % (k, m.n_elems, inf["t"]),
+              flush=True)
+        assert inf["t"] == t
+        assert np.array_equal(b.cons(k), s.cons())
+        assert np.array_equal(inf["res"], r)
+        s.close()
+    b.close()
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_first_and_second_order_against_oracle(order):
     """FP64 batch instances against the oracle's explicit steps."""
