@@ -285,11 +285,12 @@ def implicit_run(steps, cfl, ni=512, nj=256):
     an ni x nj O-grid (outer radius 20 chords, wall spacing 1e-4): Mach 0.5,
     AoA 2 deg, Re 5000 (mu = rho u c / Re), Pr 0.72, Roe flux, 2nd order
     with VK (K=5), adiabatic no-slip wall, characteristic far field.
-    Backward-Euler pseudo-time steps with local steps (CFL ramped to the
-    given value), one Newton iteration per step, GMRES (Krylov dimension
-    30) preconditioned by multicolour LU-SGS, in FP64 and in FP32-flux mode
-    (FP32 Krylov basis, Jacobian and sweeps; FP64 Newton residual, Frechet
-    difference, Arnoldi and state)."""
+    Backward-Euler pseudo-time steps with local steps (CFL driven by the
+    GMRES convergence, up to the given value), one Newton iteration per
+    step, GMRES (Krylov dimension 30) preconditioned by 4 multicolour
+    Gauss-Seidel sweeps on the first-order Jacobian, in FP64 and in
+    FP32-flux mode (FP32 Krylov basis, Jacobian and sweeps; FP64 Newton
+    residual, Frechet difference, Arnoldi and state)."""
     import math
     import numpy as np
     from ccfd_amd import gpu as G
@@ -305,17 +306,19 @@ def implicit_run(steps, cfl, ni=512, nj=256):
                    (mesh.n_elems, 1))
     mu = mach * c / re
     log("implicit: %d cells, mesh %.1f s" % (mesh.n_elems, time.time() - t0))
-    out = {"cells": mesh.n_elems, "steps": steps, "cfl": cfl, "mu": mu,
+    out = {"cells": mesh.n_elems, "steps": steps, "cfl_max": cfl, "mu": mu,
            "scheme": "NS, Roe, 2nd order VK, backward Euler with local "
-                     "pseudo-time steps, 1 Newton iteration / step, GMRES "
-                     "nKdim 30 (tolerance 1e-2), multicolour LU-SGS"}
+                     "pseudo-time steps (CFL driven by GMRES convergence), "
+                     "1 Newton iteration / step, GMRES nKdim 30 (tolerance "
+                     "1e-2), 4 multicolour Gauss-Seidel sweeps on the "
+                     "first-order Jacobian"}
     states = {}
     for name, prec in (("fp64", G.FP64), ("fp32_flux", G.FP32_FLUX)):
         prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0, navierstokes=1,
                        mu=mu, Pr=0.72, precision=prec)
         s = G.Solver(mesh, prm)
         s.set_prim(prim)
-        cfg = G.ImplicitCfg(30, 1, 2, 1e-3, 1e-2, 0.9)
+        cfg = G.ImplicitCfg(30, 1, 2, 1e-3, 1e-2, 0.9, 1, 4, 1)
         s.sync()
         w0 = time.perf_counter()
         r, r0, st, nn, ng = s.implicit_steady(cfg, cfl, steps, 1e-8)
@@ -397,8 +400,8 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.workload == "implicit":
-        r = implicit_run(args.batch_steps if args.batch_steps != 200 else 400,
-                         5.0)
+        r = implicit_run(args.batch_steps if args.batch_steps != 200 else 1000,
+                         1e4)
         out = {"metric": "implicit pseudo-time steps/s (NACA O-grid NS, "
                          "JFNK-GMRES + LU-SGS)",
                "value": 1e3 / r["fp32_flux"]["ms_per_step"],

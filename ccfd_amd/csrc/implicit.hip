@@ -55,15 +55,15 @@ struct ImplicitState {
   int* lvlF = nullptr;          // cells of forward levels (CSR items)
   int* lvlB = nullptr;          // backward
   std::vector<int> lvlFStart, lvlBStart;
-  // multicolour sweeps (precond = 2): greedy distance-1 colouring; a cell's
-  // lower neighbours are those of smaller colour
+  // multicolour Gauss-Seidel sweeps (precond = 2): greedy distance-1
+  // colouring; elements of one colour are updated together
   bool haveMC = false;
-  int* mcRank = nullptr;        // [Np] colour of each element
   int* mcCells = nullptr;       // cells by colour (CSR items)
   std::vector<int> mcStart;
   double* xs = nullptr;         // [4][Np] forward-sweep result
   int* flag = nullptr;          // singular-block flag
   double* dtLoc = nullptr;      // [Np] local pseudo-time steps (steady solves)
+  double gmresRel = 0.0;        // |residual| / |rhs| the last GMRES call reached
   std::vector<void*> allocs;
 };
 
@@ -117,14 +117,17 @@ __global__ void k_newton_residual(int N, int Np, const double* __restrict__ U,
                                   const double* __restrict__ Q,
                                   const T* __restrict__ ut, double adt,
                                   const double* __restrict__ dtv, double alpha,
+                                  const double* __restrict__ area,
                                   double* __restrict__ F, double* __restrict__ XK,
                                   double* __restrict__ RXK) {
   FOR_CELLS
   const double a = elem_adt(adt, dtv, alpha, i);
+  const double w = dtv ? sqrt(area[i]) : 1.0;
   for (int v = 0; v < 4; ++v) {
     int k = v * Np + i;
     double u = double(ut[k]);
-    F[k] = U[k] - Q[k] - a * u;
+    // local steps: rows scaled as in k_jv_diff
+    F[k] = dtv ? w * ((U[k] - Q[k]) / a - u) : U[k] - Q[k] - a * u;
     if (XK) XK[k] = U[k];
     if (RXK) RXK[k] = u;
   }
@@ -214,19 +217,27 @@ __global__ void k_jv_perturb(int N, int Np, const double* __restrict__ vv,
   for (int k = 0; k < 4; ++k) prim[k * Np + i] = T(p[k]);
 }
 
-// res = v - alpha*dt*(ut - RXK)/epsFD (src/linearSolver.c:364-372)
+// res = v - alpha*dt*(ut - RXK)/epsFD (src/linearSolver.c:364-372). With
+// local steps (dtv) every row is divided by its alpha*dt_i and multiplied
+// by sqrt(area_i): sqrt(A_i) (v/(alpha dt_i) - (ut - RXK)/epsFD). The
+// 2-norm GMRES minimises is then the area-weighted norm of dU/dt that
+// globalResidual measures, not one dominated by the elements with the
+// largest steps or the smallest areas.
 template <typename T, typename K>
 __global__ void k_jv_diff(int N, int Np, const K* __restrict__ v,
                           const T* __restrict__ ut, const double* __restrict__ RXK,
                           const double* __restrict__ epsp, double adt,
                           const double* __restrict__ dtv, double alpha,
+                          const double* __restrict__ area,
                           double* __restrict__ res) {
   FOR_CELLS
   const double eps = *epsp;
   const double a = elem_adt(adt, dtv, alpha, i);
+  const double w = dtv ? sqrt(area[i]) : 1.0;
   for (int k = 0; k < 4; ++k) {
     int o = k * Np + i;
-    res[o] = double(v[o]) - a * (double(ut[o]) - RXK[o]) / eps;
+    res[o] = dtv ? w * (double(v[o]) / a - (double(ut[o]) - RXK[o]) / eps)
+                 : double(v[o]) - a * (double(ut[o]) - RXK[o]) / eps;
   }
 }
 
@@ -245,9 +256,53 @@ __global__ void k_update_dx(int N, int Np, int m, const double* __restrict__ alp
 }
 
 __global__ void k_axpy(int N, int Np, double* __restrict__ x,
-                       const double* __restrict__ d) {
+                       const double* __restrict__ d, double w) {
   FOR_CELLS
-  for (int v = 0; v < 4; ++v) x[v * Np + i] += d[v * Np + i];
+  if (w == 1.0)
+    for (int v = 0; v < 4; ++v) x[v * Np + i] += d[v * Np + i];
+  else
+    for (int v = 0; v < 4; ++v) x[v * Np + i] += w * d[v * Np + i];
+}
+
+// largest relative change of density and pressure that the update d makes
+// to the conservative state x, per block (non-finite changes count as
+// 1e300)
+__global__ __launch_bounds__(256) void k_relchange_partial(
+    int N, int Np, const double* __restrict__ x, const double* __restrict__ d,
+    double gam1, double* __restrict__ part) {
+  __shared__ double sm[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double r = 0.0;
+  if (i < N) {
+    const double r0 = x[i], mx = x[Np + i], my = x[2 * Np + i];
+    const double p0 = gam1 * (x[3 * Np + i] - 0.5 * (mx * mx + my * my) / r0);
+    const double r1 = r0 + d[i], nx = mx + d[Np + i], ny = my + d[2 * Np + i];
+    const double p1 =
+        gam1 * (x[3 * Np + i] + d[3 * Np + i] - 0.5 * (nx * nx + ny * ny) / r1);
+    r = fmax(fabs(r1 - r0) / fabs(r0), fabs(p1 - p0) / fabs(p0));
+    if (!(r <= 1e300)) r = 1e300;
+  }
+  sm[threadIdx.x] = r;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void k_max_final(int nb, const double* __restrict__ part,
+                            double* out) {
+  __shared__ double sm[256];
+  double r = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) r = fmax(r, part[k]);
+  sm[threadIdx.x] = r;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sm[0];
 }
 
 // ---- LU-SGS assembly (coloured finite differences) ----
@@ -310,29 +365,10 @@ __global__ void k_fd_record(int N, int Np, int S, const int* __restrict__ link,
   }
 }
 
-// D = I - dt J_ii, neighbour blocks -dt J_ij, Dinv = calcDinv(D)
-// (src/linearSolver.c:226-251, 136-179)
-template <typename M>
-__global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
-                            double dt0, const double* __restrict__ dtv,
-                            M* __restrict__ Dg, M* __restrict__ Blk,
-                            M* __restrict__ Dinv, int* flag) {
-  FOR_CELLS
-  const double dt = dtv ? dtv[i] : dt0;
-  double A[4][4];
-  for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) {
-      double x = double(Dg[(4 * a + b) * Np + i]) * -dt;
-      if (a == b) x += 1.0;
-      A[a][b] = x;
-    }
-  for (int s = 0; s < S; ++s) {
-    if (link[s * Np + i] == kNoSide) continue;
-    for (int k = 0; k < 16; ++k) {
-      size_t o = ((size_t)k * S + s) * Np + i;
-      Blk[o] = M(double(Blk[o]) * -dt);
-    }
-  }
+// determinant and inverse of a 4x4 block by cofactors (calcDinv,
+// src/linearSolver.c:136-179); returns the determinant, Ainv only when it
+// is not below DBL_EPSILON in magnitude
+__device__ inline double inv4(const double A[4][4], double Ainv[4][4]) {
   double det = A[0][0]*(A[1][1]*(A[2][2]*A[3][3]-A[2][3]*A[3][2])
     +A[1][2]*(A[2][3]*A[3][1]-A[2][1]*A[3][3])+A[1][3]*(A[2][1]
     *A[3][2]-A[2][2]*A[3][1]))-A[0][1]*(A[1][0]*(A[2][2]*A[3][3]
@@ -343,10 +379,7 @@ __global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
     -A[0][3]*(A[1][0]*(A[2][1]*A[3][2]-A[2][2]*A[3][1])+A[1][1]
     *(A[2][2]*A[3][0]-A[2][0]*A[3][2])+A[1][2]*(A[2][0]*A[3][1]
     -A[2][1]*A[3][0]));
-  if (fabs(det) <= DBL_EPSILON) {
-    *flag = 1;
-    return;
-  }
+  if (fabs(det) <= DBL_EPSILON) return det;
   double C[4][4];
   C[0][0] = A[1][1]*(A[2][2]*A[3][3]-A[2][3]*A[3][2])+A[1][2]*(A[2][3]*A[3][1]-A[2][1]*A[3][3])+A[1][3]*(A[2][1]*A[3][2]-A[2][2]*A[3][1]);
   C[0][1] = A[1][0]*(A[2][3]*A[3][2]-A[2][2]*A[3][3])+A[1][2]*(A[2][0]*A[3][3]-A[2][3]*A[3][0])+A[1][3]*(A[2][2]*A[3][0]-A[2][0]*A[3][2]);
@@ -365,8 +398,46 @@ __global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
   C[3][2] = A[0][0]*(A[1][3]*A[2][1]-A[1][1]*A[2][3])+A[0][1]*(A[1][0]*A[2][3]-A[1][3]*A[2][0])+A[0][3]*(A[1][1]*A[2][0]-A[1][0]*A[2][1]);
   C[3][3] = A[0][0]*(A[1][1]*A[2][2]-A[1][2]*A[2][1])+A[0][1]*(A[1][2]*A[2][0]-A[1][0]*A[2][2])+A[0][2]*(A[1][0]*A[2][1]-A[1][1]*A[2][0]);
   for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b)
-      Dinv[(4 * a + b) * Np + i] = M(C[b][a] / det);
+    for (int b = 0; b < 4; ++b) Ainv[a][b] = C[b][a] / det;
+  return det;
+}
+
+// D = I - dt J_ii, neighbour blocks -dt J_ij, Dinv = calcDinv(D)
+// (src/linearSolver.c:226-251, 136-179)
+template <typename M>
+__global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
+                            double dt0, const double* __restrict__ dtv,
+                            const double* __restrict__ area,
+                            M* __restrict__ Dg, M* __restrict__ Blk,
+                            M* __restrict__ Dinv, int* flag) {
+  FOR_CELLS
+  // local steps (dtv): rows scaled as in k_jv_diff,
+  // D = sqrt(A_i) (I/dt_i - J_ii), blocks -sqrt(A_i) J_ij
+  const double w = dtv ? sqrt(area[i]) : 1.0;
+  const double dt = dtv ? w : dt0;
+  const double diag = dtv ? w / dtv[i] : 1.0;
+  double A[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      double x = double(Dg[(4 * a + b) * Np + i]) * -dt;
+      if (a == b) x += diag;
+      A[a][b] = x;
+    }
+  for (int s = 0; s < S; ++s) {
+    if (link[s * Np + i] == kNoSide) continue;
+    for (int k = 0; k < 16; ++k) {
+      size_t o = ((size_t)k * S + s) * Np + i;
+      Blk[o] = M(double(Blk[o]) * -dt);
+    }
+  }
+  double E[4][4];
+  const double det = inv4(A, E);
+  if (fabs(det) <= DBL_EPSILON) {
+    *flag = 1;
+    return;
+  }
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) Dinv[(4 * a + b) * Np + i] = M(E[a][b]);
 }
 
 // LUSGS forward sweep over the cells of one level (src/linearSolver.c:271-301)
@@ -423,6 +494,37 @@ __global__ void k_lusgs_bwd(int n, const int* __restrict__ cells, int Np, int S,
     M t2 = M(0);
     for (int c = 0; c < 4; ++c) t2 += Dinv[(4 * a + c) * Np + i] * t1[c];
     x[a * Np + i] = K(xs[a * Np + i] - t2);
+  }
+}
+
+// one Gauss-Seidel update of the elements of one colour:
+// x_i = D_i^-1 (b_i - sum over the face neighbours j of B_ij x_j), with the
+// current values of x (elements of one colour are not neighbours)
+template <typename M, typename K>
+__global__ __launch_bounds__(128) void k_gs_cells(
+    int n, const int* __restrict__ cells, int Np, int S,
+    const int* __restrict__ link, const M* __restrict__ Blk,
+    const M* __restrict__ Dinv, const K* __restrict__ b, K* __restrict__ x) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = cells[t];
+  double r[4];
+  for (int v = 0; v < 4; ++v) r[v] = double(b[v * Np + i]);
+  for (int s = 0; s < S; ++s) {
+    const int lk = link[s * Np + i];
+    if (lk < 0) continue;
+    const int j = lk >> 2;
+    if (j == i) continue;
+    double xj[4];
+    for (int v = 0; v < 4; ++v) xj[v] = double(x[v * Np + j]);
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 4; ++c)
+        r[a] -= double(Blk[((size_t)(4 * a + c) * S + s) * Np + i]) * xj[c];
+  }
+  for (int a = 0; a < 4; ++a) {
+    double u = 0.0;
+    for (int c = 0; c < 4; ++c) u += double(Dinv[(4 * a + c) * Np + i]) * r[c];
+    x[a * Np + i] = K(u);
   }
 }
 
@@ -631,20 +733,21 @@ static int prepare_multicolor(ccfd_gpu_ctx* c) {
   for (int k = 0; k < ncol; ++k) s->mcStart[k + 1] += s->mcStart[k];
   std::vector<int> items(N), pos(s->mcStart.begin(), s->mcStart.end() - 1);
   for (int i = 0; i < N; ++i) items[pos[col[i]]++] = i;
-  std::vector<int> rank(Np, 0);
-  for (int i = 0; i < N; ++i) rank[i] = col[i];
-  s->mcRank = (int*)imalloc(c, s, Np * 4);
   s->mcCells = (int*)imalloc(c, s, (size_t)std::max(N, 1) * 4);
-  if (!s->mcRank || !s->mcCells)
+  if (!s->mcCells)
     return fail(CCFD_ENOMEM, "multicolour schedule allocation failed");
-  if (hipMemcpyAsync(s->mcRank, rank.data(), Np * 4, hipMemcpyHostToDevice,
-                     c->st) != hipSuccess ||
-      hipMemcpyAsync(s->mcCells, items.data(), N * 4, hipMemcpyHostToDevice,
+  if (hipMemcpyAsync(s->mcCells, items.data(), N * 4, hipMemcpyHostToDevice,
                      c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
     return fail(CCFD_EHIP, "multicolour schedule upload failed");
   s->haveMC = true;
   return CCFD_OK;
+}
+
+// order of the operator whose Jacobian the LU-SGS blocks hold
+static int jacobian_order(const ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg) {
+  const int o = c->prm.spatialOrder;
+  return cfg->jacobianOrder > 0 && cfg->jacobianOrder < o ? cfg->jacobianOrder : o;
 }
 
 template <typename T>
@@ -670,9 +773,11 @@ static double dot_host(ccfd_gpu_ctx* c, const double* a, const double* b) {
 // buildMatrix by coloured finite differences at the primitive state of XK
 // consJac: perturb the conservative state (precond = 2) instead of the
 // primitive state as buildMatrix does (src/linearSolver.c:199-201)
+// jacOrder 1 on a 2nd-order context: the blocks are the Jacobian of the
+// first-order operator (differences against its own base residual)
 template <typename T, typename M>
 static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
-                        const double* dtv, bool consJac) {
+                        const double* dtv, bool consJac, int jacOrder) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np, S = c->S;
   const double reps = c->prm.precision == CCFD_FP64 ? sqrt(DBL_EPSILON)
@@ -685,25 +790,40 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
   // base primitive state: consPrim(XK), kept in FP64
   hipLaunchKernelGGL(k_load_state<double>, cells_grid(c), dim3(256), 0, c->st,
                      N, Np, s->XK, c->U, s->primBase, eqd(c));
+  auto perturb = [&](int col, int iv) {
+    if (consJac)
+      hipLaunchKernelGGL(k_fd_perturb_cons<T>, cells_grid(c), dim3(256), 0,
+                         c->st, N, Np, s->XK, s->color, col, iv, reps,
+                         (T*)c->prim[c->cur], eqd(c));
+    else
+      hipLaunchKernelGGL(k_fd_perturb<T>, cells_grid(c), dim3(256), 0, c->st,
+                         N, Np, s->primBase, s->color, col, iv, reps,
+                         (T*)c->prim[c->cur]);
+  };
+  const int opOrder = c->prm.spatialOrder;
+  const double* base = s->RXK;
+  if (jacOrder != opOrder) {
+    // the first-order operator (split kernels, zero gradients) at XK
+    c->prm.spatialOrder = jacOrder;
+    perturb(-1, 0);
+    operator_at<T>(c, tGlobal);
+    hipLaunchKernelGGL((k_copy_cvt<T, double>), dim3((4 * Np + 255) / 256),
+                       dim3(256), 0, c->st, 4 * (long)Np, (const T*)c->ut, s->W);
+    base = s->W;
+  }
   for (int col = 0; col < s->nColors; ++col) {
     for (int iv = 0; iv < 4; ++iv) {
-      if (consJac)
-        hipLaunchKernelGGL(k_fd_perturb_cons<T>, cells_grid(c), dim3(256), 0,
-                           c->st, N, Np, s->XK, s->color, col, iv, reps,
-                           (T*)c->prim[c->cur], eqd(c));
-      else
-        hipLaunchKernelGGL(k_fd_perturb<T>, cells_grid(c), dim3(256), 0, c->st,
-                           N, Np, s->primBase, s->color, col, iv, reps,
-                           (T*)c->prim[c->cur]);
+      perturb(col, iv);
       operator_at<T>(c, tGlobal);
       hipLaunchKernelGGL((k_fd_record<T, M>), cells_grid(c), dim3(256), 0,
                          c->st, N, Np, S, c->link, s->color, col, iv,
-                         (const T*)c->ut, s->RXK, srEps, (M*)s->Dg, (M*)s->Blk);
+                         (const T*)c->ut, base, srEps, (M*)s->Dg, (M*)s->Blk);
     }
   }
+  c->prm.spatialOrder = opOrder;
   hipLaunchKernelGGL(k_fd_finish<M>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                     S, c->link, dt, dtv, (M*)s->Dg, (M*)s->Blk, (M*)s->Dinv,
-                     s->flag);
+                     S, c->link, dt, dtv, c->area, (M*)s->Dg, (M*)s->Blk,
+                     (M*)s->Dinv, s->flag);
   int flag = 0;
   hipMemcpyAsync(&flag, s->flag, 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
@@ -713,25 +833,30 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
   return CCFD_OK;
 }
 
+// precond 2: `sweeps` symmetric Gauss-Seidel sweeps by colour from x = 0
+// (colours 0..nc-1, then back down; the first sweep is the LU-SGS forward
+// and backward sweep); precond 1: the reference's LU-SGS in its element
+// order, by dependency levels
 template <typename M, typename K>
-static void lusgs(ccfd_gpu_ctx* c, int mode, const K* b, K* x) {
+static void lusgs(ccfd_gpu_ctx* c, int mode, int sweeps, const K* b, K* x) {
   ImplicitState* s = c->imp;
   const int Np = c->Np, S = c->S;
   if (mode == 2) {
     const int nc = (int)s->mcStart.size() - 1;
-    for (int l = 0; l < nc; ++l) {
-      const int n = s->mcStart[l + 1] - s->mcStart[l];
-      hipLaunchKernelGGL((k_lusgs_fwd<M, K>), dim3((n + 127) / 128), dim3(128), 0,
-                         c->st, n, s->mcCells + s->mcStart[l], Np, S, c->link,
-                         s->mcRank, (const M*)s->Blk, (const M*)s->Dinv, b,
-                         (M*)s->xs);
-    }
-    for (int l = nc - 1; l >= 0; --l) {
-      const int n = s->mcStart[l + 1] - s->mcStart[l];
-      hipLaunchKernelGGL((k_lusgs_bwd<M, K>), dim3((n + 127) / 128), dim3(128), 0,
-                         c->st, n, s->mcCells + s->mcStart[l], Np, S, c->link,
-                         s->mcRank, (const M*)s->Blk, (const M*)s->Dinv,
-                         (const M*)s->xs, x);
+    hipMemsetAsync(x, 0, 4 * (size_t)Np * sizeof(K), c->st);
+    auto update = [&](int k) {
+      const int n = s->mcStart[k + 1] - s->mcStart[k];
+      hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
+                         c->st, n, s->mcCells + s->mcStart[k], Np, S, c->link,
+                         (const M*)s->Blk, (const M*)s->Dinv, b, x);
+    };
+    // a colour updated twice in a row gives the same values: skip repeats
+    int last = -1;
+    for (int sw = 0; sw < std::max(sweeps, 1); ++sw) {
+      for (int k = 0; k < nc; ++k)
+        if (k != last) update(last = k);
+      for (int k = nc - 1; k >= 0; --k)
+        if (k != last) update(last = k);
     }
     return;
   }
@@ -751,7 +876,9 @@ static void lusgs(ccfd_gpu_ctx* c, int mode, const K* b, K* x) {
   }
 }
 
-// GMRES_M (src/linearSolver.c:385-510) with B = s->B, result in s->dX
+// GMRES_M (src/linearSolver.c:385-510) with B = s->B, result in s->dX.
+// cfg->cycles > 1 restarts it (each cycle from the true residual
+// -B - J dX) until the tolerance or the cycle count; one cycle is GMRES_M.
 template <typename T, typename K>
 static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
                  double tGlobal, double time, double dt, const double* dtv,
@@ -764,90 +891,118 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   const double abortCrit = cfg->epsGMRES * normB;
   const double sqeps = c->prm.precision == CCFD_FP64 ? sqrt(DBL_EPSILON)
                                                      : sqrt((double)FLT_EPSILON);
+  const int cycles = std::max(cfg->cycles, 1);
   hipMemsetAsync(s->dX, 0, vs * 8, c->st);
-  hipLaunchKernelGGL(k_neg_div<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                     s->B, V, normB);
   std::vector<double> H((nK + 1) * (nK + 1), 0.0), gm(nK + 1, 0.0),
       Cg(nK + 1, 0.0), Sg(nK + 1, 0.0), alp(nK + 1, 0.0);
-  gm[0] = normB;
   if (cfg->precond) {
     const bool consJac = cfg->precond == 2;
+    const int jo = jacobian_order(c, cfg);
     int rc = c->prm.precision == CCFD_FP64
-                 ? build_matrix<T, double>(c, tGlobal, dt, dtv, consJac)
-                 : build_matrix<T, float>(c, tGlobal, dt, dtv, consJac);
+                 ? build_matrix<T, double>(c, tGlobal, dt, dtv, consJac, jo)
+                 : build_matrix<T, float>(c, tGlobal, dt, dtv, consJac, jo);
     if (rc) return rc;
   }
-#define HM(i, j) H[(i) * (nK + 1) + (j)]
-  for (int m = 0; m < nK; ++m) {
-    (*nIter)++;
-    K* Vm = V + m * vs;
-    K* Zm = Z + m * vs;
-    if (cfg->precond) {
-      if (c->prm.precision == CCFD_FP64)
-        lusgs<double, K>(c, cfg->precond, Vm, Zm);
-      else
-        lusgs<float, K>(c, cfg->precond, Vm, Zm);
-    } else {
-      hipMemcpyAsync(Zm, Vm, vs * sizeof(K), hipMemcpyDeviceToDevice, c->st);
-    }
-    // matrixVector(Z_m) -> W; |Z_m|^2 in FP64 (Z_m copied to W first)
-    double* vv = s->H + nK + 4;
-    double* epsFD = s->H + nK + 5;
+  // matrixVector (src/linearSolver.c:343-373) of v into s->W; |v|^2 in FP64
+  // (v copied to W first)
+  double* vv = s->H + nK + 4;
+  double* epsFD = s->H + nK + 5;
+  auto jv = [&](const K* v) {
     hipLaunchKernelGGL((k_copy_cvt<K, double>), dim3((vs + 255) / 256), dim3(256),
-                       0, c->st, (long)vs, Zm, s->W);
+                       0, c->st, (long)vs, v, s->W);
     dot_dev(c, s->W, s->W, vv);
     hipLaunchKernelGGL((k_jv_perturb<T, K>), cells_grid(c), dim3(256), 0, c->st,
-                       N, Np, vv, sqeps, s->XK, Zm, c->U, (T*)c->prim[c->cur],
+                       N, Np, vv, sqeps, s->XK, v, c->U, (T*)c->prim[c->cur],
                        epsFD, eqd(c));
     operator_at<T>(c, time);
     hipLaunchKernelGGL((k_jv_diff<T, K>), cells_grid(c), dim3(256), 0, c->st, N,
-                       Np, Zm, (const T*)c->ut, s->RXK, epsFD, alpha * dt, dtv,
-                       alpha, s->W);
-    // modified Gram-Schmidt (src/linearSolver.c:440-450)
-    for (int n = 0; n <= m; ++n) {
-      K* Vn = V + n * vs;
-      hipLaunchKernelGGL(k_dotk_partial<K>, dim3(c->nb), dim3(256), 0, c->st, N,
-                         Np, Vn, s->W, s->part);
-      hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, c->st, c->nb, s->part,
-                         s->H + n, 0);
-      hipLaunchKernelGGL(k_axpy_dev<K>, cells_grid(c), dim3(256), 0, c->st, N,
-                         Np, s->H + n, Vn, s->W);
+                       Np, v, (const T*)c->ut, s->RXK, epsFD, alpha * dt, dtv,
+                       alpha, c->area, s->W);
+  };
+#define HM(i, j) H[(i) * (nK + 1) + (j)]
+  for (int cyc = 0; cyc < cycles; ++cyc) {
+    double beta = normB;
+    if (cyc == 0) {
+      hipLaunchKernelGGL(k_neg_div<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
+                         s->B, V, normB);
+    } else {
+      // r = -B - J dX; V_0 = r / |r| (dX staged in V_0 for the product)
+      hipLaunchKernelGGL((k_copy_cvt<double, K>), dim3((vs + 255) / 256),
+                         dim3(256), 0, c->st, (long)vs, s->dX, V);
+      jv(V);
+      hipLaunchKernelGGL(k_axpy, cells_grid(c), dim3(256), 0, c->st, N, Np, s->W,
+                         s->B, 1.0);
+      beta = sqrt(dot_host(c, s->W, s->W));
+      if (!(beta > abortCrit)) break;
+      hipLaunchKernelGGL(k_neg_div<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
+                         s->W, V, beta);
     }
-    dot_dev(c, s->W, s->W, s->H + m + 1);
-    std::vector<double> col(m + 2);
-    hipMemcpyAsync(col.data(), s->H, (m + 2) * 8, hipMemcpyDeviceToHost, c->st);
-    if (hipStreamSynchronize(c->st) != hipSuccess)
-      return fail(CCFD_EHIP, "GMRES iteration failed: %s",
-                  hipGetErrorString(hipGetLastError()));
-    for (int n = 0; n <= m; ++n) HM(n, m) = col[n];
-    HM(m + 1, m) = sqrt(col[m + 1]);
-    // Givens rotations (src/linearSolver.c:456-468)
-    for (int n = 0; n <= m - 1; ++n) {
-      double tmp = Cg[n] * HM(n, m) + Sg[n] * HM(n + 1, m);
-      HM(n + 1, m) = -Sg[n] * HM(n, m) + Cg[n] * HM(n + 1, m);
-      HM(n, m) = tmp;
-    }
-    double bet = sqrt(HM(m, m) * HM(m, m) + HM(m + 1, m) * HM(m + 1, m));
-    Sg[m] = HM(m + 1, m) / bet;
-    Cg[m] = HM(m, m) / bet;
-    HM(m, m) = bet;
-    gm[m + 1] = -Sg[m] * gm[m];
-    gm[m] = Cg[m] * gm[m];
-    if ((fabs(gm[m + 1]) <= abortCrit) || (m == nK - 1)) {
-      for (int n = m; n >= 0; --n) {
-        alp[n] = gm[n];
-        for (int o = n + 1; o <= m; ++o) alp[n] -= HM(n, o) * alp[o];
-        alp[n] /= HM(n, n);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(gm.begin(), gm.end(), 0.0);
+    gm[0] = beta;
+    for (int m = 0; m < nK; ++m) {
+      (*nIter)++;
+      K* Vm = V + m * vs;
+      K* Zm = Z + m * vs;
+      if (cfg->precond) {
+        if (c->prm.precision == CCFD_FP64)
+          lusgs<double, K>(c, cfg->precond, cfg->sweeps, Vm, Zm);
+        else
+          lusgs<float, K>(c, cfg->precond, cfg->sweeps, Vm, Zm);
+      } else {
+        hipMemcpyAsync(Zm, Vm, vs * sizeof(K), hipMemcpyDeviceToDevice, c->st);
       }
-      hipMemcpyAsync(s->alp, alp.data(), (m + 1) * 8, hipMemcpyHostToDevice, c->st);
-      hipLaunchKernelGGL(k_update_dx<K>, cells_grid(c), dim3(256), 0, c->st, N,
-                         Np, m, s->alp, Z, vs, s->dX);
-      return hipStreamSynchronize(c->st) == hipSuccess
-                 ? CCFD_OK
-                 : fail(CCFD_EHIP, "GMRES update failed");
+      jv(Zm);
+      // modified Gram-Schmidt (src/linearSolver.c:440-450)
+      for (int n = 0; n <= m; ++n) {
+        K* Vn = V + n * vs;
+        hipLaunchKernelGGL(k_dotk_partial<K>, dim3(c->nb), dim3(256), 0, c->st, N,
+                           Np, Vn, s->W, s->part);
+        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, c->st, c->nb, s->part,
+                           s->H + n, 0);
+        hipLaunchKernelGGL(k_axpy_dev<K>, cells_grid(c), dim3(256), 0, c->st, N,
+                           Np, s->H + n, Vn, s->W);
+      }
+      dot_dev(c, s->W, s->W, s->H + m + 1);
+      std::vector<double> col(m + 2);
+      hipMemcpyAsync(col.data(), s->H, (m + 2) * 8, hipMemcpyDeviceToHost, c->st);
+      if (hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(CCFD_EHIP, "GMRES iteration failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+      for (int n = 0; n <= m; ++n) HM(n, m) = col[n];
+      HM(m + 1, m) = sqrt(col[m + 1]);
+      // Givens rotations (src/linearSolver.c:456-468)
+      for (int n = 0; n <= m - 1; ++n) {
+        double tmp = Cg[n] * HM(n, m) + Sg[n] * HM(n + 1, m);
+        HM(n + 1, m) = -Sg[n] * HM(n, m) + Cg[n] * HM(n + 1, m);
+        HM(n, m) = tmp;
+      }
+      double bet = sqrt(HM(m, m) * HM(m, m) + HM(m + 1, m) * HM(m + 1, m));
+      Sg[m] = HM(m + 1, m) / bet;
+      Cg[m] = HM(m, m) / bet;
+      HM(m, m) = bet;
+      gm[m + 1] = -Sg[m] * gm[m];
+      gm[m] = Cg[m] * gm[m];
+      const bool done = fabs(gm[m + 1]) <= abortCrit;
+      if (done || m == nK - 1) {
+        s->gmresRel = normB > 0.0 ? fabs(gm[m + 1]) / normB : 0.0;
+        for (int n = m; n >= 0; --n) {
+          alp[n] = gm[n];
+          for (int o = n + 1; o <= m; ++o) alp[n] -= HM(n, o) * alp[o];
+          alp[n] /= HM(n, n);
+        }
+        hipMemcpyAsync(s->alp, alp.data(), (m + 1) * 8, hipMemcpyHostToDevice,
+                       c->st);
+        hipLaunchKernelGGL(k_update_dx<K>, cells_grid(c), dim3(256), 0, c->st, N,
+                           Np, m, s->alp, Z, vs, s->dX);
+        if (hipStreamSynchronize(c->st) != hipSuccess)
+          return fail(CCFD_EHIP, "GMRES update failed");
+        if (done) return CCFD_OK;
+        break;
+      }
+      hipLaunchKernelGGL(k_scale<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
+                         s->W, V + (m + 1) * vs, HM(m + 1, m));
     }
-    hipLaunchKernelGGL(k_scale<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                       s->W, V + (m + 1) * vs, HM(m + 1, m));
   }
 #undef HM
   return CCFD_OK;
@@ -875,7 +1030,7 @@ static void outer_residual(ccfd_gpu_ctx* c, const double* X, double time,
   operator_at<double>(c, time);
   hipLaunchKernelGGL(k_newton_residual<double>, cells_grid(c), dim3(256), 0,
                      c->st, c->N, c->Np, c->U, s->Q, (const double*)c->ut, adt,
-                     dtv, alpha, s->FXK, XKout, s->RXK);
+                     dtv, alpha, c->area, s->FXK, XKout, s->RXK);
 }
 
 // FP32-flux mode: R(XK) with the FP32 operator as the base of the Frechet
@@ -897,10 +1052,14 @@ static int base_residual_fp32(ccfd_gpu_ctx* c, double time) {
 // Newton iteration per pseudo-time step is the usual steady iteration).
 // The Newton residual is FP64 in both precisions; in FP32-flux mode the
 // GMRES operator evaluations, the Jacobian and the LU-SGS sweeps are FP32.
+// maxChange > 0 (steady solves only): every Newton update is scaled so that
+// no element's density or pressure changes by more than that fraction;
+// *relax (may be null) receives the smallest scale used.
 template <typename T>
 static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
                            double t, double dt, const double* dtv, bool steady,
-                           double res[4], int* nNewton, int* nGMRES) {
+                           double res[4], int* nNewton, int* nGMRES,
+                           double maxChange = 0.0, double* relax = nullptr) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np;
   const size_t vs = 4 * (size_t)Np;
@@ -916,7 +1075,7 @@ static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   double n2FK = n2F0;
   int nIn = 0;
   if (cfg->precond) {
-    int rc = prepare_lusgs(c, c->prm.spatialOrder);
+    int rc = prepare_lusgs(c, jacobian_order(c, cfg));
     if (!rc && cfg->precond == 2) rc = prepare_multicolor(c);
     if (rc) return rc;
   }
@@ -929,9 +1088,22 @@ static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     hipMemcpyAsync(s->B, s->FXK, vs * 8, hipMemcpyDeviceToDevice, c->st);
     int rc = gmres_any<T>(c, cfg, t, time, dt, dtv, alpha, sqrt(n2FK), nGMRES);
     if (rc) return rc;
+    double w = 1.0;
+    if (maxChange > 0.0) {
+      hipLaunchKernelGGL(k_relchange_partial, dim3(c->nb), dim3(256), 0, c->st,
+                         N, Np, s->XK, s->dX, c->prm.gam - 1.0, s->part);
+      hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, c->st, c->nb,
+                         s->part, s->H + cfg->nKdim + 6);
+      double mx = 0.0;
+      hipMemcpyAsync(&mx, s->H + cfg->nKdim + 6, 8, hipMemcpyDeviceToHost, c->st);
+      if (hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(CCFD_EHIP, "update bound failed");
+      if (mx > maxChange) w = maxChange / mx;
+      if (relax) *relax = fmin(*relax, w);
+    }
     // XK += dX; U = XK; prim = consPrim(U) (src/timeDiscretization.c:505-520)
     hipLaunchKernelGGL(k_axpy, cells_grid(c), dim3(256), 0, c->st, N, Np, s->XK,
-                       s->dX);
+                       s->dX, w);
     outer_residual(c, s->XK, time, alpha * dt, dtv, alpha, nullptr);
     n2FK = dot_host(c, s->FXK, s->FXK);
   }
@@ -955,8 +1127,13 @@ extern "C" int ccfd_gpu_implicit_step(ccfd_gpu_ctx* c,
                                       int* nGMRES) {
   if (!c || !c->hasMesh || !cfg || !nNewton || !nGMRES)
     return fail(CCFD_EINVAL, "bad implicit-step arguments");
-  if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->nNewtonIter < 1)
-    return fail(CCFD_EINVAL, "nKdim must be in 1..200, nNewtonIter >= 1");
+  if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->nNewtonIter < 1 ||
+      cfg->precond < 0 || cfg->precond > 2 || cfg->jacobianOrder < 0 ||
+      cfg->jacobianOrder > 2 || cfg->sweeps < 0 || cfg->sweeps > 64 ||
+      cfg->cycles < 0 || cfg->cycles > 100)
+    return fail(CCFD_EINVAL, "nKdim must be in 1..200, nNewtonIter >= 1, "
+                             "precond 0..2, jacobianOrder 0..2, sweeps 0..64, "
+                             "cycles 0..100");
   hipSetDevice(c->device);
   int rc = ensure(c, cfg->nKdim);
   if (rc) return rc;
@@ -981,17 +1158,27 @@ static int steady_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   const double mu = c->prm.navierstokes ? q.mu : 0.0;
   double cflk = fmin(cfl, 1.0);
   for (int k = 0; k < maxSteps; ++k) {
-    // pseudo-time steps from the current state; the CFL number ramps up
-    // from 1 by 20% per step
+    // pseudo-time steps from the current state. The CFL number starts at
+    // 1 and follows the linear solver: it grows by 20% after a step whose
+    // GMRES reduced its residual tenfold, stays after one that reduced it
+    // by more than half, and is cut by 30% after one that did not (at large
+    // steps the preconditioned Krylov solve stops converging and the Newton
+    // update is of no use) or whose update had to be scaled down (more than
+    // a 20% density or pressure change somewhere).
     int rc = refresh_prim(c);
     if (rc) return rc;
     hipLaunchKernelGGL(k_local_dt<T>, cells_grid(c), dim3(256), 0, c->st, c->N,
                        c->Np, (const T*)c->prim[c->cur], c->area, c->sx, c->sy,
                        q.gam, mu, gamPrMax, cflk, cflk, s->dtLoc);
-    cflk = fmin(cfl, 1.2 * cflk);
+    double relax = 1.0;
     rc = implicit_step_t<T>(c, cfg, 0.0, 0.0, s->dtLoc, true, res, nNewton,
-                            nGMRES);
+                            nGMRES, 0.2, &relax);
     if (rc) return rc;
+    const double lo = fmin(cfl, 1.0), rel = s->gmresRel;
+    if (relax < 1.0 || rel > 0.5)
+      cflk = fmax(lo, 0.7 * cflk);
+    else if (rel <= 0.1)
+      cflk = fmin(cfl, 1.2 * cflk);
     *steps = k + 1;
     if (k == 0)
       for (int v = 0; v < 4; ++v) res0[v] = res[v];
@@ -1010,7 +1197,10 @@ extern "C" int ccfd_gpu_implicit_steady(ccfd_gpu_ctx* c,
       !nGMRES)
     return fail(CCFD_EINVAL, "bad steady-solve arguments");
   if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->nNewtonIter < 1 ||
-      cfg->precond < 0 || cfg->precond > 2 || !(cfl > 0.0) || maxSteps < 1)
+      cfg->precond < 0 || cfg->precond > 2 || cfg->jacobianOrder < 0 ||
+      cfg->jacobianOrder > 2 || cfg->sweeps < 0 || cfg->sweeps > 64 ||
+      cfg->cycles < 0 || cfg->cycles > 100 ||
+      !(cfl > 0.0) || maxSteps < 1)
     return fail(CCFD_EINVAL, "bad steady-solve settings");
   hipSetDevice(c->device);
   int rc = ensure(c, cfg->nKdim);
@@ -1029,6 +1219,12 @@ extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
                               double* dX, int* nIter) {
   if (!c || !c->hasMesh || !cfg || !B || !dX || !nIter)
     return fail(CCFD_EINVAL, "bad GMRES arguments");
+  if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->precond < 0 ||
+      cfg->precond > 2 || cfg->jacobianOrder < 0 || cfg->jacobianOrder > 2 ||
+      cfg->sweeps < 0 || cfg->sweeps > 64 ||
+      cfg->cycles < 0 || cfg->cycles > 100)
+    return fail(CCFD_EINVAL, "nKdim must be in 1..200, precond 0..2, "
+                             "jacobianOrder 0..2, sweeps 0..64, cycles 0..100");
   hipSetDevice(c->device);
   int rc = ensure(c, cfg->nKdim);
   if (rc) return rc;
@@ -1037,7 +1233,8 @@ extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   // linearisation point: the resident state
   hipMemcpyAsync(s->XK, c->U, 4 * (size_t)Np * 8, hipMemcpyDeviceToDevice, c->st);
   if ((rc = upload_vec(c, B, s->B))) return rc;
-  if (cfg->precond && (rc = prepare_lusgs(c, c->prm.spatialOrder))) return rc;
+  if (cfg->precond && (rc = prepare_lusgs(c, jacobian_order(c, cfg)))) return rc;
+  if (cfg->precond == 2 && (rc = prepare_multicolor(c))) return rc;
   auto run = [&](auto tag) -> int {
     using T = decltype(tag);
     hipLaunchKernelGGL(k_load_state<T>, cells_grid(c), dim3(256), 0, c->st, N, Np,

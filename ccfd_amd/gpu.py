@@ -37,7 +37,9 @@ class ImplicitCfg(C.Structure):
     """struct ccfd_gpu_implicit_cfg"""
     _fields_ = [("nKdim", C.c_int), ("nNewtonIter", C.c_int),
                 ("precond", C.c_int), ("epsNewton", C.c_double),
-                ("epsGMRES", C.c_double), ("gammaEW", C.c_double)]
+                ("epsGMRES", C.c_double), ("gammaEW", C.c_double),
+                ("jacobianOrder", C.c_int), ("sweeps", C.c_int),
+                ("cycles", C.c_int)]
 
 
 def params(gamma=1.4, mu=0.0, Pr=0.72, navierstokes=0, flux=5, order=2,

@@ -145,6 +145,22 @@ typedef struct ccfd_gpu_implicit_cfg {
 	double epsNewton;       /* relative Newton tolerance */
 	double epsGMRES;        /* relative GMRES tolerance */
 	double gammaEW;         /* Eisenstat-Walker gamma */
+	int jacobianOrder;      /* 0: the LU-SGS blocks are the Jacobian of the
+	                           operator itself (buildMatrix); 1: of the
+	                           first-order operator, also for 2nd-order
+	                           solves (cheaper to build, diagonally
+	                           dominant). Not in the reference. */
+	int sweeps;             /* precond 2: symmetric Gauss-Seidel sweeps
+	                           per preconditioner application, from zero
+	                           (0 or 1: one, which is LU-SGS; 1..64). More
+	                           than one pays on a diagonally dominant
+	                           Jacobian (jacobianOrder 1); on the truncated
+	                           2nd-order one the sweeps can diverge. Not
+	                           in the reference. */
+	int cycles;             /* GMRES cycles of nKdim iterations, each
+	                           restarted from the true residual, until
+	                           epsGMRES is met (0 or 1: one, as GMRES_M;
+	                           1..100). Not in the reference. */
 } ccfd_gpu_implicit_cfg;
 
 /* implicitTimeStep (src/timeDiscretization.c:412-552) with GMRES_M,
@@ -155,14 +171,20 @@ int ccfd_gpu_implicit_step(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
 /*
  * Steady solve by implicit pseudo-time stepping: backward-Euler steps as
  * ccfd_gpu_implicit_step, each element with its own pseudo-time step (the
- * CFL and diffusive limits of calcTimeStep per element; the CFL number
- * starts at min(cfl, 1) and grows by 20% per step up to cfl), from the
- * resident state until the density residual drops by `drop` relative to
- * the first step, or maxSteps. A step stops its Newton loop at
- * nNewtonIter without failing (one Newton iteration per step is the usual
- * choice). res / res0: residual of the last / first step; *steps: steps
- * taken. Local pseudo-time steps are not part of the reference, which
- * steps with one global time step.
+ * CFL and diffusive limits of calcTimeStep per element), from the resident
+ * state until the density residual drops by `drop` relative to the first
+ * step, or maxSteps. The Newton system is solved with each row divided by
+ * its element's step. The CFL number starts at min(cfl, 1) and follows the
+ * linear solver, up to cfl: +20% after a step whose GMRES reduced its
+ * residual tenfold, unchanged after one that halved it, -30% after one
+ * that did not. Each Newton update is scaled down where needed so that no
+ * element's density or pressure changes by more than 20%; after a step
+ * that needed this the CFL number is cut by 30% as well. A
+ * step stops its Newton loop at nNewtonIter without failing (one Newton
+ * iteration per step is the usual choice). res / res0: residual of the
+ * last / first step; *steps: steps taken. Local pseudo-time steps and the
+ * update bound are not part of the reference, which steps with one global
+ * time step.
  */
 int ccfd_gpu_implicit_steady(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
 		double cfl, int maxSteps, double drop, double res[4],
