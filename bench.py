@@ -275,7 +275,9 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
         out["density_err_vs_fp64"] = {
             "l2_max": max(e[0] for e in errs),
             "l2_median": float(np.median([e[0] for e in errs])),
-            "linf_max": max(e[1] for e in errs)}
+            "linf_max": max(e[1] for e in errs),
+            "instances_l2_within_1e-6": int(sum(e[0] <= 1e-6 for e in errs)),
+            "instances": len(errs)}
     b.close()
     return out
 
