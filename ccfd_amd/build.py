@@ -22,7 +22,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 
 HOST_SRC = ["mesh.c", "ini.c", "driver.c"]
 GPU_SRC = ["ccfd_gpu.hip", "implicit.hip", "batch.hip"]
-HOST_CXX_SRC = ["soa.cpp"]   # host C++ linked into libccfd_gpu.so
+HOST_CXX_SRC = ["soa.cpp", "agglomerate.cpp"]   # host C++ in libccfd_gpu.so
 FLUX_CODES = list(range(1, 12))
 OBJ = os.path.join(ROOT, "_build")
 
@@ -114,6 +114,11 @@ def build_test_native():
         _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC",
               "-shared", "-I" + CSRC, "-o",
               os.path.join(d, "libphysics_host.so"), src])
+    src = os.path.join(d, "agglomerate_host.cpp")
+    if os.path.exists(src):
+        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + CSRC,
+              "-o", os.path.join(d, "libagglomerate_host.so"), src,
+              os.path.join(CSRC, "agglomerate.cpp")])
 
 
 def build_oracle():

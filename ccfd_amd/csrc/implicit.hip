@@ -26,6 +26,7 @@
 #include <cstring>
 #include <vector>
 
+#include "agglomerate.hpp"
 #include "ctx.hpp"
 
 using namespace ccfd;
@@ -60,6 +61,19 @@ struct ImplicitState {
   bool haveMC = false;
   int* mcCells = nullptr;       // cells by colour (CSR items)
   std::vector<int> mcStart;
+  // aggregation multigrid (precond = 3): coarse levels below the elements
+  struct MGLevel {
+    int n = 0, nnz = 0, nPrev = 0;
+    int *aggOf = nullptr, *memStart = nullptr, *mem = nullptr;
+    int *rowStart = nullptr, *col = nullptr, *colorCells = nullptr;
+    int64_t *dSrcStart = nullptr, *dSrc = nullptr;
+    int64_t *oSrcStart = nullptr, *oSrc = nullptr;
+    std::vector<int> colorStart;
+    double *D = nullptr, *Dinv = nullptr, *O = nullptr;   // [16][n], [16][nnz]
+    double *b = nullptr, *x = nullptr, *r = nullptr;      // [4][n]
+  };
+  bool haveMG = false;
+  std::vector<MGLevel> mg;
   double* xs = nullptr;         // [4][Np] forward-sweep result
   int* flag = nullptr;          // singular-block flag
   double* dtLoc = nullptr;      // [Np] local pseudo-time steps (steady solves)
@@ -403,7 +417,8 @@ __device__ inline double inv4(const double A[4][4], double Ainv[4][4]) {
 }
 
 // D = I - dt J_ii, neighbour blocks -dt J_ij, Dinv = calcDinv(D)
-// (src/linearSolver.c:226-251, 136-179)
+// (src/linearSolver.c:226-251, 136-179); D itself is kept in Dg (the
+// multigrid residuals and coarse blocks use it)
 template <typename M>
 __global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
                             double dt0, const double* __restrict__ dtv,
@@ -437,7 +452,10 @@ __global__ void k_fd_finish(int N, int Np, int S, const int* __restrict__ link,
     return;
   }
   for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) Dinv[(4 * a + b) * Np + i] = M(E[a][b]);
+    for (int b = 0; b < 4; ++b) {
+      Dinv[(4 * a + b) * Np + i] = M(E[a][b]);
+      Dg[(4 * a + b) * Np + i] = M(A[a][b]);
+    }
 }
 
 // LUSGS forward sweep over the cells of one level (src/linearSolver.c:271-301)
@@ -526,6 +544,157 @@ __global__ __launch_bounds__(128) void k_gs_cells(
     for (int c = 0; c < 4; ++c) u += double(Dinv[(4 * a + c) * Np + i]) * r[c];
     x[a * Np + i] = K(u);
   }
+}
+
+// ---- aggregation multigrid (precond = 3) ----
+
+// coarse diagonal blocks: sums of the listed blocks of the level above
+// (code & 1: a diagonal block, else an off-diagonal one; index code >> 1),
+// then inverted
+template <typename M>
+__global__ void k_mg_diag(int n, const int64_t* __restrict__ start,
+                          const int64_t* __restrict__ src,
+                          const M* __restrict__ Dp, long dStride,
+                          const M* __restrict__ Op, long oStride,
+                          double* __restrict__ D, double* __restrict__ Dinv,
+                          int* flag) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double A[4][4] = {};
+  for (int64_t t = start[c]; t < start[c + 1]; ++t) {
+    const int64_t code = src[t], idx = code >> 1;
+    for (int k = 0; k < 16; ++k)
+      A[k >> 2][k & 3] += (code & 1) ? double(Dp[k * dStride + idx])
+                                     : double(Op[k * oStride + idx]);
+  }
+  double E[4][4];
+  if (fabs(inv4(A, E)) <= DBL_EPSILON) {
+    *flag = 1;
+    return;
+  }
+  for (int k = 0; k < 16; ++k) {
+    D[(size_t)k * n + c] = A[k >> 2][k & 3];
+    Dinv[(size_t)k * n + c] = E[k >> 2][k & 3];
+  }
+}
+
+// coarse off-diagonal blocks
+template <typename M>
+__global__ void k_mg_off(int nnz, const int64_t* __restrict__ start,
+                         const int64_t* __restrict__ src,
+                         const M* __restrict__ Dp, long dStride,
+                         const M* __restrict__ Op, long oStride,
+                         double* __restrict__ O) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nnz) return;
+  double a[16] = {};
+  for (int64_t t = start[z]; t < start[z + 1]; ++t) {
+    const int64_t code = src[t], idx = code >> 1;
+    for (int k = 0; k < 16; ++k)
+      a[k] += (code & 1) ? double(Dp[k * dStride + idx])
+                         : double(Op[k * oStride + idx]);
+  }
+  for (int k = 0; k < 16; ++k) O[(size_t)k * nnz + z] = a[k];
+}
+
+// Gauss-Seidel update of the coarse elements of one colour
+__global__ void k_mg_gs(int m, const int* __restrict__ cells, int n, int nnz,
+                        const int* __restrict__ rowStart,
+                        const int* __restrict__ col,
+                        const double* __restrict__ O,
+                        const double* __restrict__ Dinv,
+                        const double* __restrict__ b, double* __restrict__ x) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int i = cells[t];
+  double r[4];
+  for (int v = 0; v < 4; ++v) r[v] = b[(size_t)v * n + i];
+  for (int z = rowStart[i]; z < rowStart[i + 1]; ++z) {
+    const int j = col[z];
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 4; ++c)
+        r[a] -= O[(size_t)(4 * a + c) * nnz + z] * x[(size_t)c * n + j];
+  }
+  for (int a = 0; a < 4; ++a) {
+    double u = 0.0;
+    for (int c = 0; c < 4; ++c) u += Dinv[(size_t)(4 * a + c) * n + i] * r[c];
+    x[(size_t)a * n + i] = u;
+  }
+}
+
+// r = b - A x on a coarse level
+__global__ void k_mg_resid(int n, int nnz, const int* __restrict__ rowStart,
+                           const int* __restrict__ col,
+                           const double* __restrict__ D,
+                           const double* __restrict__ O,
+                           const double* __restrict__ b,
+                           const double* __restrict__ x,
+                           double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double t[4];
+  for (int a = 0; a < 4; ++a) {
+    t[a] = b[(size_t)a * n + i];
+    for (int c = 0; c < 4; ++c)
+      t[a] -= D[(size_t)(4 * a + c) * n + i] * x[(size_t)c * n + i];
+  }
+  for (int z = rowStart[i]; z < rowStart[i + 1]; ++z) {
+    const int j = col[z];
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 4; ++c)
+        t[a] -= O[(size_t)(4 * a + c) * nnz + z] * x[(size_t)c * n + j];
+  }
+  for (int a = 0; a < 4; ++a) r[(size_t)a * n + i] = t[a];
+}
+
+// r = b - A x on the elements (diagonal blocks D, side blocks Blk)
+template <typename M, typename K>
+__global__ void k_mg_resid0(int N, int Np, int S, const int* __restrict__ link,
+                            const M* __restrict__ D, const M* __restrict__ Blk,
+                            const K* __restrict__ b, const K* __restrict__ x,
+                            double* __restrict__ r) {
+  FOR_CELLS
+  double t[4];
+  for (int a = 0; a < 4; ++a) {
+    t[a] = double(b[a * Np + i]);
+    for (int c = 0; c < 4; ++c)
+      t[a] -= double(D[(4 * a + c) * Np + i]) * double(x[c * Np + i]);
+  }
+  for (int s = 0; s < S; ++s) {
+    const int lk = link[s * Np + i];
+    if (lk < 0 || (lk >> 2) == i) continue;
+    const int j = lk >> 2;
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 4; ++c)
+        t[a] -= double(Blk[((size_t)(4 * a + c) * S + s) * Np + i]) *
+                double(x[c * Np + j]);
+  }
+  for (int a = 0; a < 4; ++a) r[a * Np + i] = t[a];
+}
+
+// b_c = sum of r over the members of coarse element c
+__global__ void k_mg_restrict(int n, const int* __restrict__ memStart,
+                              const int* __restrict__ mem,
+                              const double* __restrict__ r, long rStride,
+                              double* __restrict__ b) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = memStart[c]; k < memStart[c + 1]; ++k)
+    for (int v = 0; v < 4; ++v) t[v] += r[v * rStride + mem[k]];
+  for (int v = 0; v < 4; ++v) b[(size_t)v * n + c] = t[v];
+}
+
+// x_p += x_c of p's coarse element
+template <typename K>
+__global__ void k_mg_prolong(int nPrev, const int* __restrict__ aggOf,
+                             const double* __restrict__ xc, int n,
+                             K* __restrict__ x, long xStride) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nPrev) return;
+  const int c = aggOf[p];
+  for (int v = 0; v < 4; ++v)
+    x[v * xStride + p] = K(double(x[v * xStride + p]) + xc[(size_t)v * n + c]);
 }
 
 // local pseudo-time step of every element: the CFL (and, with viscosity,
@@ -744,6 +913,111 @@ static int prepare_multicolor(ccfd_gpu_ctx* c) {
   return CCFD_OK;
 }
 
+// coarse levels of the multigrid preconditioner (precond = 3): groups of
+// elements by heavy-edge matching on face length / barycentre distance,
+// down to at most kMGMinCells elements
+constexpr int kMGMinCells = 256;
+constexpr int kMGMaxLevels = 12;
+
+template <typename X>
+static X* upload_new(ccfd_gpu_ctx* c, ImplicitState* s, const std::vector<X>& v,
+                     int& rc) {
+  X* d = (X*)imalloc(c, s, std::max<size_t>(v.size(), 1) * sizeof(X));
+  if (!d) {
+    rc = CCFD_ENOMEM;
+    return nullptr;
+  }
+  if (!v.empty() && hipMemcpyAsync(d, v.data(), v.size() * sizeof(X),
+                                   hipMemcpyHostToDevice, c->st) != hipSuccess)
+    rc = CCFD_EHIP;
+  return d;
+}
+
+static int prepare_mg(ccfd_gpu_ctx* c) {
+  ImplicitState* s = c->imp;
+  if (s->haveMG) return CCFD_OK;
+  const int N = c->N, Np = c->Np, S = c->S;
+  const size_t n = (size_t)S * Np;
+  std::vector<int> link(n);
+  std::vector<double> flen(n), bbd(n), w(n, 0.0);
+  if (hipMemcpyAsync(link.data(), c->link, n * 4, hipMemcpyDeviceToHost,
+                     c->st) != hipSuccess ||
+      hipMemcpyAsync(flen.data(), c->g64.flen, n * 8, hipMemcpyDeviceToHost,
+                     c->st) != hipSuccess ||
+      hipMemcpyAsync(bbd.data(), c->g64.bbd, n * 8, hipMemcpyDeviceToHost,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "multigrid geometry download failed");
+  for (size_t k = 0; k < n; ++k)
+    if (link[k] >= 0 && bbd[k] > 0.0) w[k] = fabs(flen[k]) / bbd[k];
+  std::vector<AggLevel> lv;
+  build_agglomeration(N, Np, S, link.data(), w.data(), kMGMinCells,
+                      kMGMaxLevels, lv);
+  int rc = CCFD_OK, nPrev = N;
+  s->mg.resize(lv.size());
+  for (size_t l = 0; l < lv.size(); ++l) {
+    const AggLevel& A = lv[l];
+    ImplicitState::MGLevel& G = s->mg[l];
+    G.n = A.n;
+    G.nnz = A.nnz;
+    G.nPrev = nPrev;
+    G.aggOf = upload_new(c, s, A.aggOf, rc);
+    G.memStart = upload_new(c, s, A.memStart, rc);
+    G.mem = upload_new(c, s, A.mem, rc);
+    G.rowStart = upload_new(c, s, A.rowStart, rc);
+    G.col = upload_new(c, s, A.col, rc);
+    G.colorCells = upload_new(c, s, A.colorCells, rc);
+    G.dSrcStart = upload_new(c, s, A.dSrcStart, rc);
+    G.dSrc = upload_new(c, s, A.dSrc, rc);
+    G.oSrcStart = upload_new(c, s, A.oSrcStart, rc);
+    G.oSrc = upload_new(c, s, A.oSrc, rc);
+    G.colorStart = A.colorStart;
+    G.D = (double*)imalloc(c, s, 16 * (size_t)A.n * 8);
+    G.Dinv = (double*)imalloc(c, s, 16 * (size_t)A.n * 8);
+    G.O = (double*)imalloc(c, s, 16 * std::max<size_t>(A.nnz, 1) * 8);
+    G.b = (double*)imalloc(c, s, 4 * (size_t)A.n * 8);
+    G.x = (double*)imalloc(c, s, 4 * (size_t)A.n * 8);
+    G.r = (double*)imalloc(c, s, 4 * (size_t)A.n * 8);
+    if (!G.D || !G.Dinv || !G.O || !G.b || !G.x || !G.r) rc = CCFD_ENOMEM;
+    if (rc) break;
+    nPrev = A.n;
+  }
+  if (rc == CCFD_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = CCFD_EHIP;
+  if (rc)
+    return fail(rc, "multigrid level setup failed");
+  s->haveMG = true;
+  return CCFD_OK;
+}
+
+// coarse blocks of every level from the element blocks (Dg holds D)
+template <typename M>
+static void mg_assemble(ccfd_gpu_ctx* c) {
+  ImplicitState* s = c->imp;
+  for (size_t l = 0; l < s->mg.size(); ++l) {
+    ImplicitState::MGLevel& G = s->mg[l];
+    const dim3 gd((G.n + 127) / 128), go((G.nnz + 127) / 128), b(128);
+    if (l == 0) {
+      hipLaunchKernelGGL(k_mg_diag<M>, gd, b, 0, c->st, G.n, G.dSrcStart,
+                         G.dSrc, (const M*)s->Dg, (long)c->Np,
+                         (const M*)s->Blk, (long)c->S * c->Np, G.D, G.Dinv,
+                         s->flag);
+      if (G.nnz)
+        hipLaunchKernelGGL(k_mg_off<M>, go, b, 0, c->st, G.nnz, G.oSrcStart,
+                           G.oSrc, (const M*)s->Dg, (long)c->Np,
+                           (const M*)s->Blk, (long)c->S * c->Np, G.O);
+    } else {
+      const ImplicitState::MGLevel& P = s->mg[l - 1];
+      hipLaunchKernelGGL(k_mg_diag<double>, gd, b, 0, c->st, G.n, G.dSrcStart,
+                         G.dSrc, (const double*)P.D, (long)P.n,
+                         (const double*)P.O, (long)P.nnz, G.D, G.Dinv, s->flag);
+      if (G.nnz)
+        hipLaunchKernelGGL(k_mg_off<double>, go, b, 0, c->st, G.nnz,
+                           G.oSrcStart, G.oSrc, (const double*)P.D, (long)P.n,
+                           (const double*)P.O, (long)P.nnz, G.O);
+    }
+  }
+}
+
 // order of the operator whose Jacobian the LU-SGS blocks hold
 static int jacobian_order(const ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg) {
   const int o = c->prm.spatialOrder;
@@ -777,7 +1051,8 @@ static double dot_host(ccfd_gpu_ctx* c, const double* a, const double* b) {
 // first-order operator (differences against its own base residual)
 template <typename T, typename M>
 static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
-                        const double* dtv, bool consJac, int jacOrder) {
+                        const double* dtv, bool consJac, int jacOrder,
+                        bool coarse) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np, S = c->S;
   const double reps = c->prm.precision == CCFD_FP64 ? sqrt(DBL_EPSILON)
@@ -824,6 +1099,7 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
   hipLaunchKernelGGL(k_fd_finish<M>, cells_grid(c), dim3(256), 0, c->st, N, Np,
                      S, c->link, dt, dtv, c->area, (M*)s->Dg, (M*)s->Blk,
                      (M*)s->Dinv, s->flag);
+  if (coarse) mg_assemble<M>(c);
   int flag = 0;
   hipMemcpyAsync(&flag, s->flag, 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
@@ -833,31 +1109,107 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
   return CCFD_OK;
 }
 
-// precond 2: `sweeps` symmetric Gauss-Seidel sweeps by colour from x = 0
-// (colours 0..nc-1, then back down; the first sweep is the LU-SGS forward
-// and backward sweep); precond 1: the reference's LU-SGS in its element
-// order, by dependency levels
+// symmetric Gauss-Seidel sweeps by colour on the elements (colours
+// 0..nc-1, then back down), from x = 0 when `zero` (the first sweep is then
+// the LU-SGS forward and backward sweep) or from the current x
+template <typename M, typename K>
+static void gs0(ccfd_gpu_ctx* c, int sweeps, bool zero, const K* b, K* x) {
+  ImplicitState* s = c->imp;
+  const int Np = c->Np, S = c->S;
+  const int nc = (int)s->mcStart.size() - 1;
+  if (zero) hipMemsetAsync(x, 0, 4 * (size_t)Np * sizeof(K), c->st);
+  auto update = [&](int k) {
+    const int n = s->mcStart[k + 1] - s->mcStart[k];
+    hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
+                       c->st, n, s->mcCells + s->mcStart[k], Np, S, c->link,
+                       (const M*)s->Blk, (const M*)s->Dinv, b, x);
+  };
+  // a colour updated twice in a row gives the same values: skip repeats
+  int last = -1;
+  for (int sw = 0; sw < std::max(sweeps, 1); ++sw) {
+    for (int k = 0; k < nc; ++k)
+      if (k != last) update(last = k);
+    for (int k = nc - 1; k >= 0; --k)
+      if (k != last) update(last = k);
+  }
+}
+
+// the same on a coarse multigrid level, from x = 0 when `zero`
+static void gs_level(ccfd_gpu_ctx* c, ImplicitState::MGLevel& G, int sweeps,
+                     bool zero) {
+  const int nc = (int)G.colorStart.size() - 1;
+  if (zero) hipMemsetAsync(G.x, 0, 4 * (size_t)G.n * 8, c->st);
+  auto update = [&](int k) {
+    const int m = G.colorStart[k + 1] - G.colorStart[k];
+    hipLaunchKernelGGL(k_mg_gs, dim3((m + 127) / 128), dim3(128), 0, c->st, m,
+                       G.colorCells + G.colorStart[k], G.n, G.nnz, G.rowStart,
+                       G.col, G.O, G.Dinv, G.b, G.x);
+  };
+  int last = -1;
+  for (int sw = 0; sw < std::max(sweeps, 1); ++sw) {
+    for (int k = 0; k < nc; ++k)
+      if (k != last) update(last = k);
+    for (int k = nc - 1; k >= 0; --k)
+      if (k != last) update(last = k);
+  }
+}
+
+// one multigrid V-cycle from x = 0: `sweeps` symmetric Gauss-Seidel sweeps
+// before and after the coarse correction on every level, kMGCoarseSweeps on
+// the coarsest
+constexpr int kMGCoarseSweeps = 8;
+
+template <typename M, typename K>
+static void mg_vcycle(ccfd_gpu_ctx* c, int sweeps, const K* b, K* x) {
+  ImplicitState* s = c->imp;
+  const int N = c->N, Np = c->Np, S = c->S, L = (int)s->mg.size();
+  gs0<M, K>(c, sweeps, true, b, x);
+  if (L == 0) return;
+  // restrict the element residual (s->W is free while GMRES preconditions)
+  hipLaunchKernelGGL((k_mg_resid0<M, K>), cells_grid(c), dim3(256), 0, c->st,
+                     N, Np, S, c->link, (const M*)s->Dg, (const M*)s->Blk, b, x,
+                     s->W);
+  hipLaunchKernelGGL(k_mg_restrict, dim3((s->mg[0].n + 127) / 128), dim3(128), 0,
+                     c->st, s->mg[0].n, s->mg[0].memStart, s->mg[0].mem, s->W,
+                     (long)Np, s->mg[0].b);
+  for (int l = 0; l < L; ++l) {
+    ImplicitState::MGLevel& G = s->mg[l];
+    if (l == L - 1) {
+      gs_level(c, G, kMGCoarseSweeps, true);
+      break;
+    }
+    gs_level(c, G, sweeps, true);
+    ImplicitState::MGLevel& H = s->mg[l + 1];
+    hipLaunchKernelGGL(k_mg_resid, dim3((G.n + 127) / 128), dim3(128), 0, c->st,
+                       G.n, G.nnz, G.rowStart, G.col, G.D, G.O, G.b, G.x, G.r);
+    hipLaunchKernelGGL(k_mg_restrict, dim3((H.n + 127) / 128), dim3(128), 0,
+                       c->st, H.n, H.memStart, H.mem, G.r, (long)G.n, H.b);
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    ImplicitState::MGLevel& G = s->mg[l];
+    const ImplicitState::MGLevel& H = s->mg[l + 1];
+    hipLaunchKernelGGL(k_mg_prolong<double>, dim3((G.n + 127) / 128), dim3(128),
+                       0, c->st, G.n, H.aggOf, H.x, H.n, G.x, (long)G.n);
+    gs_level(c, G, sweeps, false);
+  }
+  hipLaunchKernelGGL(k_mg_prolong<K>, cells_grid(c), dim3(256), 0, c->st, N,
+                     s->mg[0].aggOf, s->mg[0].x, s->mg[0].n, x, (long)Np);
+  gs0<M, K>(c, sweeps, false, b, x);
+}
+
+// precond 2: `sweeps` symmetric Gauss-Seidel sweeps by colour from x = 0;
+// precond 3: a multigrid V-cycle with them as smoother; precond 1: the
+// reference's LU-SGS in its element order, by dependency levels
 template <typename M, typename K>
 static void lusgs(ccfd_gpu_ctx* c, int mode, int sweeps, const K* b, K* x) {
   ImplicitState* s = c->imp;
   const int Np = c->Np, S = c->S;
   if (mode == 2) {
-    const int nc = (int)s->mcStart.size() - 1;
-    hipMemsetAsync(x, 0, 4 * (size_t)Np * sizeof(K), c->st);
-    auto update = [&](int k) {
-      const int n = s->mcStart[k + 1] - s->mcStart[k];
-      hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
-                         c->st, n, s->mcCells + s->mcStart[k], Np, S, c->link,
-                         (const M*)s->Blk, (const M*)s->Dinv, b, x);
-    };
-    // a colour updated twice in a row gives the same values: skip repeats
-    int last = -1;
-    for (int sw = 0; sw < std::max(sweeps, 1); ++sw) {
-      for (int k = 0; k < nc; ++k)
-        if (k != last) update(last = k);
-      for (int k = nc - 1; k >= 0; --k)
-        if (k != last) update(last = k);
-    }
+    gs0<M, K>(c, sweeps, true, b, x);
+    return;
+  }
+  if (mode == 3) {
+    mg_vcycle<M, K>(c, sweeps, b, x);
     return;
   }
   for (size_t l = 0; l + 1 < s->lvlFStart.size(); ++l) {
@@ -896,11 +1248,12 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   std::vector<double> H((nK + 1) * (nK + 1), 0.0), gm(nK + 1, 0.0),
       Cg(nK + 1, 0.0), Sg(nK + 1, 0.0), alp(nK + 1, 0.0);
   if (cfg->precond) {
-    const bool consJac = cfg->precond == 2;
+    const bool consJac = cfg->precond >= 2;
     const int jo = jacobian_order(c, cfg);
+    const bool coarse = cfg->precond == 3;
     int rc = c->prm.precision == CCFD_FP64
-                 ? build_matrix<T, double>(c, tGlobal, dt, dtv, consJac, jo)
-                 : build_matrix<T, float>(c, tGlobal, dt, dtv, consJac, jo);
+                 ? build_matrix<T, double>(c, tGlobal, dt, dtv, consJac, jo, coarse)
+                 : build_matrix<T, float>(c, tGlobal, dt, dtv, consJac, jo, coarse);
     if (rc) return rc;
   }
   // matrixVector (src/linearSolver.c:343-373) of v into s->W; |v|^2 in FP64
@@ -1076,7 +1429,8 @@ static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   int nIn = 0;
   if (cfg->precond) {
     int rc = prepare_lusgs(c, jacobian_order(c, cfg));
-    if (!rc && cfg->precond == 2) rc = prepare_multicolor(c);
+    if (!rc && cfg->precond >= 2) rc = prepare_multicolor(c);
+    if (!rc && cfg->precond == 3) rc = prepare_mg(c);
     if (rc) return rc;
   }
   while ((n2FK > eps2loc * n2F0) && (nIn < cfg->nNewtonIter)) {
@@ -1128,11 +1482,11 @@ extern "C" int ccfd_gpu_implicit_step(ccfd_gpu_ctx* c,
   if (!c || !c->hasMesh || !cfg || !nNewton || !nGMRES)
     return fail(CCFD_EINVAL, "bad implicit-step arguments");
   if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->nNewtonIter < 1 ||
-      cfg->precond < 0 || cfg->precond > 2 || cfg->jacobianOrder < 0 ||
+      cfg->precond < 0 || cfg->precond > 3 || cfg->jacobianOrder < 0 ||
       cfg->jacobianOrder > 2 || cfg->sweeps < 0 || cfg->sweeps > 64 ||
       cfg->cycles < 0 || cfg->cycles > 100)
     return fail(CCFD_EINVAL, "nKdim must be in 1..200, nNewtonIter >= 1, "
-                             "precond 0..2, jacobianOrder 0..2, sweeps 0..64, "
+                             "precond 0..3, jacobianOrder 0..2, sweeps 0..64, "
                              "cycles 0..100");
   hipSetDevice(c->device);
   int rc = ensure(c, cfg->nKdim);
@@ -1197,7 +1551,7 @@ extern "C" int ccfd_gpu_implicit_steady(ccfd_gpu_ctx* c,
       !nGMRES)
     return fail(CCFD_EINVAL, "bad steady-solve arguments");
   if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->nNewtonIter < 1 ||
-      cfg->precond < 0 || cfg->precond > 2 || cfg->jacobianOrder < 0 ||
+      cfg->precond < 0 || cfg->precond > 3 || cfg->jacobianOrder < 0 ||
       cfg->jacobianOrder > 2 || cfg->sweeps < 0 || cfg->sweeps > 64 ||
       cfg->cycles < 0 || cfg->cycles > 100 ||
       !(cfl > 0.0) || maxSteps < 1)
@@ -1220,10 +1574,10 @@ extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   if (!c || !c->hasMesh || !cfg || !B || !dX || !nIter)
     return fail(CCFD_EINVAL, "bad GMRES arguments");
   if (cfg->nKdim < 1 || cfg->nKdim > 200 || cfg->precond < 0 ||
-      cfg->precond > 2 || cfg->jacobianOrder < 0 || cfg->jacobianOrder > 2 ||
+      cfg->precond > 3 || cfg->jacobianOrder < 0 || cfg->jacobianOrder > 2 ||
       cfg->sweeps < 0 || cfg->sweeps > 64 ||
       cfg->cycles < 0 || cfg->cycles > 100)
-    return fail(CCFD_EINVAL, "nKdim must be in 1..200, precond 0..2, "
+    return fail(CCFD_EINVAL, "nKdim must be in 1..200, precond 0..3, "
                              "jacobianOrder 0..2, sweeps 0..64, cycles 0..100");
   hipSetDevice(c->device);
   int rc = ensure(c, cfg->nKdim);
@@ -1234,7 +1588,8 @@ extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   hipMemcpyAsync(s->XK, c->U, 4 * (size_t)Np * 8, hipMemcpyDeviceToDevice, c->st);
   if ((rc = upload_vec(c, B, s->B))) return rc;
   if (cfg->precond && (rc = prepare_lusgs(c, jacobian_order(c, cfg)))) return rc;
-  if (cfg->precond == 2 && (rc = prepare_multicolor(c))) return rc;
+  if (cfg->precond >= 2 && (rc = prepare_multicolor(c))) return rc;
+  if (cfg->precond == 3 && (rc = prepare_mg(c))) return rc;
   auto run = [&](auto tag) -> int {
     using T = decltype(tag);
     hipLaunchKernelGGL(k_load_state<T>, cells_grid(c), dim3(256), 0, c->st, N, Np,

@@ -100,6 +100,10 @@ def lib():
         "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
                                              C.c_double, dp, ip]),
         "ccfd_gpu_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
+        "ccfd_gpu_gmres": (C.c_int, [vp, C.POINTER(ImplicitCfg), C.c_double,
+                                     C.c_double, C.c_double, C.c_double, dp,
+                                     C.c_double, dp, ip]),
+        "ccfd_gpu_dot": (C.c_int, [vp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -236,6 +240,29 @@ class Solver:
                                             N.dptr(r), C.byref(nn),
                                             C.byref(ng)))
         return r, nn.value, ng.value
+
+    def gmres(self, cfg, t_global, time, dt, alpha, B, normB=None):
+        """GMRES_M about the resident state for the right-hand side B (N x 4,
+        reference order); returns (dX, Krylov iterations)"""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        assert B.shape == (self.N, 4)
+        if normB is None:
+            normB = float(np.sqrt(self.dot(B, B)))
+        dX = np.empty((self.N, 4))
+        it = C.c_int(0)
+        check(self.L.ccfd_gpu_gmres(self.h, C.byref(cfg), t_global, time, dt,
+                                    alpha, N.dptr(B), normB, N.dptr(dX),
+                                    C.byref(it)))
+        return dX, it.value
+
+    def dot(self, a, b):
+        """vectorDotProduct of two N x 4 host vectors, summed on the device"""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert a.shape == b.shape == (self.N, 4)
+        out = C.c_double(0.0)
+        check(self.L.ccfd_gpu_dot(self.h, N.dptr(a), N.dptr(b), C.byref(out)))
+        return out.value
 
     def implicit_steady(self, cfg, cfl, max_steps, drop):
         """implicit pseudo-time steps with local steps until the density

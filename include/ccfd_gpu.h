@@ -141,7 +141,11 @@ typedef struct ccfd_gpu_implicit_cfg {
 	                           launches per sweep instead of one per
 	                           dependency level) on the Jacobian with
 	                           respect to the conservative state: a
-	                           different, stronger preconditioner */
+	                           different, stronger preconditioner;
+	                           3: as 2 with a multigrid V-cycle around the
+	                           sweeps (coarse levels of agglomerated
+	                           elements, Galerkin blocks, Gauss-Seidel
+	                           smoothing on every level) */
 	double epsNewton;       /* relative Newton tolerance */
 	double epsGMRES;        /* relative GMRES tolerance */
 	double gammaEW;         /* Eisenstat-Walker gamma */
@@ -150,11 +154,13 @@ typedef struct ccfd_gpu_implicit_cfg {
 	                           first-order operator, also for 2nd-order
 	                           solves (cheaper to build, diagonally
 	                           dominant). Not in the reference. */
-	int sweeps;             /* precond 2: symmetric Gauss-Seidel sweeps
-	                           per preconditioner application, from zero
-	                           (0 or 1: one, which is LU-SGS; 1..64). More
-	                           than one pays on a diagonally dominant
-	                           Jacobian (jacobianOrder 1); on the truncated
+	int sweeps;             /* symmetric Gauss-Seidel sweeps, 0 or 1: one
+	                           (1..64). precond 2: per preconditioner
+	                           application, from zero (one is LU-SGS);
+	                           precond 3: before and after the coarse
+	                           correction on every level. More than one
+	                           pays on a diagonally dominant Jacobian
+	                           (jacobianOrder 1); on the truncated
 	                           2nd-order one the sweeps can diverge. Not
 	                           in the reference. */
 	int cycles;             /* GMRES cycles of nKdim iterations, each

@@ -159,6 +159,14 @@ class Oracle:
         return r
 
 
+def dot(a, b):
+    """vectorDotProduct (src/linearSolver.c:115-128) of two N x 4 vectors,
+    summed serially in element order"""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return oracle_lib().oracle_dot(len(a), _d(a), _d(b))
+
+
 def riemann(cfg, code, L, R):
     lib = oracle_lib()
     L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 4)

@@ -168,27 +168,31 @@ def _periodic_case():
     return mesh, prim
 
 
-@pytest.mark.parametrize("sweeps,jorder,cycles", [(1, 0, 1), (4, 1, 1),
-                                                  (1, 1, 1), (2, 1, 4)],
-                         ids=["lusgs", "sweeps4", "jacobian1", "restarted"])
-def test_preconditioner_options_converge_to_the_same_step(sweeps, jorder,
-                                                          cycles):
-    """Gauss-Seidel sweeps, a first-order Jacobian in the preconditioner and
-    restarted GMRES change how each Newton system is solved, not its
-    solution: three 2nd-order backward-Euler steps agree with the oracle's
-    (reference LU-SGS) to the Newton tolerance; on the (diagonally
-    dominant) first-order Jacobian more sweeps need fewer Krylov
-    iterations"""
+@pytest.mark.parametrize("precond,sweeps,jorder,cycles",
+                         [(2, 1, 0, 1), (2, 4, 1, 1), (2, 1, 1, 1),
+                          (2, 2, 1, 4), (3, 1, 1, 1), (3, 2, 0, 1)],
+                         ids=["lusgs", "sweeps4", "jacobian1", "restarted",
+                              "multigrid", "multigrid_jacobian2"])
+def test_preconditioner_options_converge_to_the_same_step(precond, sweeps,
+                                                          jorder, cycles):
+    """Gauss-Seidel sweeps, the multigrid V-cycle, a first-order Jacobian in
+    the preconditioner and restarted GMRES change how each Newton system is
+    solved, not its solution: three 2nd-order backward-Euler steps agree
+    with the oracle's (reference LU-SGS) to the Newton tolerance; on the
+    (diagonally dominant) first-order Jacobian more sweeps need fewer
+    Krylov iterations"""
     mesh, prim = _periodic_case()
     prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0, cfl=5.0)
     cfg = B.make_cfg(flux=2, order=2, limiter=2, venk_k=5.0, cfl=5.0)
     nk = 4 if cycles > 1 else 10
-    ig = G.ImplicitCfg(nk, 40, 2, 1e-10, 1e-3, 0.9, jorder, sweeps, cycles)
+    ig = G.ImplicitCfg(nk, 40, precond, 1e-10, 1e-3, 0.9, jorder, sweeps,
+                       cycles)
     io = B.ImplicitCfg(10, 40, 1, 1e-10, 1e-3, 0.9)
     ug, uo, ng, no = _run_pair(mesh, prm, cfg, prim, ig, io, 3)
     err = cases.rel_err(ug, uo)
-    print("sweeps %d jacobian order %d cycles %d: rel err %.2e, Newton %d "
-          "(oracle %d)" % (sweeps, jorder, cycles, err, ng, no))
+    print("precond %d sweeps %d jacobian order %d cycles %d: rel err %.2e, "
+          "Newton %d (oracle %d)" % (precond, sweeps, jorder, cycles, err, ng,
+                                     no))
     assert err < 1e-6
     if sweeps == 4:
         gs = G.Solver(mesh, prm)
@@ -210,7 +214,7 @@ def test_implicit_options_are_checked():
     gs.set_prim(prim)
     dt, _ = gs.calc_dt(0.0, 1e9, 1e9)
     for bad in (dict(jacobianOrder=3), dict(sweeps=65), dict(cycles=101),
-                dict(precond=3)):
+                dict(precond=4)):
         ic = G.ImplicitCfg(10, 5, 2, 1e-6, 1e-3, 0.9)
         for k, v in bad.items():
             setattr(ic, k, v)
@@ -252,3 +256,86 @@ def test_navier_stokes_steady_solve_converges():
     d = np.linalg.norm(out[1][:, 0] - out[0][:, 0]) / np.linalg.norm(out[0][:, 0])
     print("fp32 vs fp64 density rel L2 %.2e" % d)
     assert d < 1e-2
+
+
+def _frechet(gs, X, v, time, dt, alpha=1.0):
+    """J_F v = v - alpha dt (R(X + eps v) - R(X)) / eps, with R evaluated
+    by the device operator at host-set states (independent of the GMRES
+    code)"""
+    eps = 1e-7 * max(np.linalg.norm(X), 1e-300) / max(np.linalg.norm(v), 1e-300)
+    gs.set_cons(X)
+    gs.fvtd(time)
+    r0 = gs.ut()
+    gs.set_cons(X + eps * v)
+    gs.fvtd(time)
+    r1 = gs.ut()
+    gs.set_cons(X)
+    return v - alpha * dt * (r1 - r0) / eps
+
+
+def test_dot_matches_the_oracle():
+    mesh, prim = _periodic_case()
+    gs = G.Solver(mesh, G.params(flux=2, order=2, limiter=2))
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(mesh.n_elems, 4))
+    b = rng.normal(size=(mesh.n_elems, 4))
+    d, o = gs.dot(a, b), B.dot(a, b)
+    print("device %.17g oracle %.17g" % (d, o))
+    assert abs(d - o) <= 1e-13 * np.abs(a * b).sum()
+    assert gs.dot(a, a) == gs.dot(a, a)     # fixed summation order
+    gs.close()
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2, 3])
+def test_gmres_solves_the_linear_system(precond):
+    """GMRES_M through the C ABI: |B + J_F dX| <= epsGMRES |B| with J_F
+    applied by an independent finite difference, and dX -> -B as dt -> 0"""
+    mesh, prim = _periodic_case()
+    prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0, cfl=5.0)
+    gs = G.Solver(mesh, prm)
+    gs.set_prim(prim)
+    X = gs.cons()
+    dt, _ = gs.calc_dt(0.0, 1e9, 1e9)
+    gs.set_cons(X)
+    rng = np.random.default_rng(11)
+    Bv = rng.normal(size=X.shape) * 1e-3
+    cfg = G.ImplicitCfg(150, 1, precond, 1e-6, 1e-6, 0.9)
+    dX, it = gs.gmres(cfg, 0.0, 0.0, dt, 1.0, Bv)
+    res = np.linalg.norm(Bv + _frechet(gs, X, dX, 0.0, dt))
+    print("precond %d: %d iterations, |B + J dX| / |B| = %.2e"
+          % (precond, it, res / np.linalg.norm(Bv)))
+    assert 0 < it < 150       # converged before the Krylov space ran out
+    assert res <= 1e-5 * np.linalg.norm(Bv)
+    dX0, _ = gs.gmres(cfg, 0.0, 0.0, 1e-12 * dt, 1.0, Bv)
+    assert np.allclose(dX0, -Bv, rtol=1e-8, atol=1e-12)
+    gs.close()
+
+
+def test_multigrid_needs_fewer_krylov_iterations():
+    """a hard system, (I - dt J) with dt a thousand times the CFL step on a
+    128x64 NACA Navier-Stokes O-grid (three coarse levels): one V-cycle per
+    preconditioner application leaves GMRES fewer iterations to reach 1e-6
+    than the Gauss-Seidel sweeps it uses as smoother"""
+    raw = M.naca_ogrid(128, 64, wall_bc=201, far_bc=501, wall_spacing=1e-3,
+                       radius=10.0)
+    bcs = [M.BC(201), M.BC(501, rho=1.0, mach=0.5, alpha=2.0, pressure=1.0)]
+    mesh = M.Mesh(raw, bcs)
+    c = np.sqrt(1.4)
+    a = np.radians(2.0)
+    prim = np.tile([1.0, 0.5 * c * np.cos(a), 0.5 * c * np.sin(a), 1.0],
+                   (mesh.n_elems, 1))
+    prm = G.params(flux=2, order=1, navierstokes=1, mu=0.5 * c / 500.0,
+                   cfl=1.0)
+    rng = np.random.default_rng(5)
+    Bv = rng.normal(size=(mesh.n_elems, 4))
+    its = {}
+    for pc in (2, 3):
+        gs = G.Solver(mesh, prm)
+        gs.set_prim(prim)
+        dt, _ = gs.calc_dt(0.0, 1e9, 1e9)
+        _, its[pc] = gs.gmres(G.ImplicitCfg(200, 1, pc, 1e-6, 1e-6, 0.9),
+                              0.0, 0.0, 1e3 * dt, 1.0, Bv)
+        gs.close()
+    print("GMRES iterations to 1e-6: Gauss-Seidel %d, multigrid %d"
+          % (its[2], its[3]))
+    assert its[3] < its[2]
