@@ -78,6 +78,7 @@ struct ImplicitState {
   int* flag = nullptr;          // singular-block flag
   double* dtLoc = nullptr;      // [Np] local pseudo-time steps (steady solves)
   double gmresRel = 0.0;        // |residual| / |rhs| the last GMRES call reached
+  double steadyCfl = 0.0;       // CFL number a steady solve stopped at
   std::vector<void*> allocs;
 };
 
@@ -1510,7 +1511,11 @@ static int steady_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   const Eq<double> q = eqd(c);
   const double gamPrMax = fmax(4.0 / 3.0, q.gam / q.Pr);
   const double mu = c->prm.navierstokes ? q.mu : 0.0;
-  double cflk = fmin(cfl, 1.0);
+  // res0[0] > 0: continue a solve, measuring the drop against res0 and
+  // carrying on with the CFL number the previous call stopped at
+  const bool cont = res0[0] > 0.0;
+  double cflk = cont && s->steadyCfl > 0.0 ? fmin(cfl, s->steadyCfl)
+                                           : fmin(cfl, 1.0);
   for (int k = 0; k < maxSteps; ++k) {
     // pseudo-time steps from the current state. The CFL number starts at
     // 1 and follows the linear solver: it grows by 20% after a step whose
@@ -1534,7 +1539,8 @@ static int steady_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     else if (rel <= 0.1)
       cflk = fmin(cfl, 1.2 * cflk);
     *steps = k + 1;
-    if (k == 0)
+    s->steadyCfl = cflk;
+    if (k == 0 && !cont)
       for (int v = 0; v < 4; ++v) res0[v] = res[v];
     if (!(res[0] == res[0])) return fail(CCFD_ENONFINITE, "non-finite residual");
     if (res[0] <= drop * res0[0]) break;

@@ -264,10 +264,12 @@ class Solver:
         check(self.L.ccfd_gpu_dot(self.h, N.dptr(a), N.dptr(b), C.byref(out)))
         return out.value
 
-    def implicit_steady(self, cfg, cfl, max_steps, drop):
+    def implicit_steady(self, cfg, cfl, max_steps, drop, res0=None):
         """implicit pseudo-time steps with local steps until the density
-        residual drops by `drop`; returns (res, res0, steps, newton, gmres)"""
-        r, r0 = np.zeros(4), np.zeros(4)
+        residual drops by `drop`; returns (res, res0, steps, newton, gmres).
+        Passing the res0 of an earlier call continues that solve."""
+        r = np.zeros(4)
+        r0 = np.zeros(4) if res0 is None else np.array(res0, dtype=np.float64)
         st, nn, ng = C.c_int(0), C.c_int(0), C.c_int(0)
         check(self.L.ccfd_gpu_implicit_steady(
             self.h, C.byref(cfg), cfl, max_steps, drop, N.dptr(r), N.dptr(r0),

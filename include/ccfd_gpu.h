@@ -185,12 +185,15 @@ int ccfd_gpu_implicit_step(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
  * residual tenfold, unchanged after one that halved it, -30% after one
  * that did not. Each Newton update is scaled down where needed so that no
  * element's density or pressure changes by more than 20%; after a step
- * that needed this the CFL number is cut by 30% as well. A
- * step stops its Newton loop at nNewtonIter without failing (one Newton
- * iteration per step is the usual choice). res / res0: residual of the
- * last / first step; *steps: steps taken. Local pseudo-time steps and the
- * update bound are not part of the reference, which steps with one global
- * time step.
+ * that needed this the CFL number is cut by 30% as well. A step stops its
+ * Newton loop at nNewtonIter without failing (one Newton iteration per
+ * step is the usual choice). res / res0: residual of the last / first
+ * step; *steps: steps taken by this call. A solve continues over several
+ * calls when res0 comes back holding the first call's values (res0[0] > 0
+ * on entry): the drop is measured against it and the CFL number carries
+ * on where the previous call left it; res0[0] = 0 starts a new solve.
+ * Local pseudo-time steps, the CFL control and the update bound are not
+ * part of the reference, which steps with one global time step.
  */
 int ccfd_gpu_implicit_steady(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
 		double cfl, int maxSteps, double drop, double res[4],
