@@ -339,3 +339,34 @@ def test_multigrid_needs_fewer_krylov_iterations():
     print("GMRES iterations to 1e-6: Gauss-Seidel %d, multigrid %d"
           % (its[2], its[3]))
     assert its[3] < its[2]
+
+
+def test_steady_solve_continues_across_calls():
+    """a steady solve split into calls (res0 passed back) is the same solve:
+    20 + 20 steps give the state and residuals of 40 steps in one call"""
+    raw = M.naca_ogrid(48, 16, wall_bc=101, far_bc=501)
+    bcs = [M.BC(101), M.BC(501, rho=1.0, mach=0.5, alpha=2.0, pressure=1.0)]
+    mesh = M.Mesh(raw, bcs)
+    c = np.sqrt(1.4)
+    a = np.radians(2.0)
+    prim = np.tile([1.0, 0.5 * c * np.cos(a), 0.5 * c * np.sin(a), 1.0],
+                   (mesh.n_elems, 1))
+    prm = G.params(flux=2, order=2, limiter=2, venk_k=5.0)
+    ic = G.ImplicitCfg(20, 1, 2, 1e-3, 1e-2, 0.9, 1, 2, 1)
+    out = []
+    for split in (False, True):
+        gs = G.Solver(mesh, prm)
+        gs.set_prim(prim)
+        if split:
+            r, r0, n1, _, _ = gs.implicit_steady(ic, 100.0, 20, 1e-12)
+            r, r0b, n2, _, _ = gs.implicit_steady(ic, 100.0, 20, 1e-12, r0)
+            assert n1 == n2 == 20 and np.array_equal(r0, r0b)
+        else:
+            r, r0, n, _, _ = gs.implicit_steady(ic, 100.0, 40, 1e-12)
+            assert n == 40
+        out.append((gs.cons(), r, r0))
+        gs.close()
+    print("drop after 40 steps: %.3e" % (out[0][1][0] / out[0][2][0]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
