@@ -1,6 +1,6 @@
 """GPU parity of the spatial operator (fvTimeDerivative) against the oracle.
 
-FP64: bit-identical (the north-star bar is 1e-12 relative; the HIP kernels
+FP64: bit-identical (the drop-in bar is 1e-12 relative; the HIP kernels
 keep the reference's operation order, no FMA contraction, IEEE div/sqrt).
 FP32-flux: within 1e-4 relative of the FP64 oracle.
 """
