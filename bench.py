@@ -399,6 +399,9 @@ def main():
                     help="implicit workload: precisions to run")
     ap.add_argument("--save-state", default=None,
                     help="implicit workload: directory for the final states")
+    ap.add_argument("--no-baseline", action="store_true",
+                    help="batched workload: skip the one-at-a-time FP64 "
+                         "solves (profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -412,7 +415,10 @@ def main():
 
     if args.workload == "batched":
         cells = (args.min_cells or 2e3, args.max_cells or 5e5)
-        r = batched_run(args.instances, cells, args.batch_steps, args.hbm_gb)
+        if args.copy_probe:
+            G.copy_probe(1 << 30, 20)
+        r = batched_run(args.instances, cells, args.batch_steps, args.hbm_gb,
+                        baseline=not args.no_baseline)
         out = {"metric": "batched cell-stage updates/s (many meshes, one "
                          "launch per stage)",
                "value": r["cell_stage_updates_per_s"],
