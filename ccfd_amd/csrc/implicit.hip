@@ -61,6 +61,12 @@ struct ImplicitState {
   bool haveMC = false;
   int* mcCells = nullptr;       // cells by colour (CSR items)
   std::vector<int> mcStart;
+  // the sweeps' operands in colour order (row p is element mcCells[p]), so
+  // that a colour's sweep reads contiguous rows: neighbour ids [S][N],
+  // side blocks [16][S][N] and inverted diagonal blocks [16][N]
+  int* mcLink = nullptr;
+  void* mcBlk = nullptr;
+  void* mcDinv = nullptr;
   // aggregation multigrid (precond = 3): coarse levels below the elements
   struct MGLevel {
     int n = 0, nnz = 0, nPrev = 0;
@@ -516,34 +522,48 @@ __global__ void k_lusgs_bwd(int n, const int* __restrict__ cells, int Np, int S,
   }
 }
 
-// one Gauss-Seidel update of the elements of one colour:
+// one Gauss-Seidel update of the elements of one colour, rows p = start ..
+// start + n - 1 of the colour-ordered operands (element i = cells[p]):
 // x_i = D_i^-1 (b_i - sum over the face neighbours j of B_ij x_j), with the
 // current values of x (elements of one colour are not neighbours)
 template <typename M, typename K>
 __global__ __launch_bounds__(128) void k_gs_cells(
-    int n, const int* __restrict__ cells, int Np, int S,
-    const int* __restrict__ link, const M* __restrict__ Blk,
+    int n, int start, const int* __restrict__ cells, int N, int Np, int S,
+    const int* __restrict__ nbr, const M* __restrict__ Blk,
     const M* __restrict__ Dinv, const K* __restrict__ b, K* __restrict__ x) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int i = cells[t];
+  const int p = start + t, i = cells[p];
   double r[4];
   for (int v = 0; v < 4; ++v) r[v] = double(b[v * Np + i]);
   for (int s = 0; s < S; ++s) {
-    const int lk = link[s * Np + i];
-    if (lk < 0) continue;
-    const int j = lk >> 2;
-    if (j == i) continue;
+    const int j = nbr[(size_t)s * N + p];
+    if (j < 0) continue;
     double xj[4];
     for (int v = 0; v < 4; ++v) xj[v] = double(x[v * Np + j]);
     for (int a = 0; a < 4; ++a)
       for (int c = 0; c < 4; ++c)
-        r[a] -= double(Blk[((size_t)(4 * a + c) * S + s) * Np + i]) * xj[c];
+        r[a] -= double(Blk[((size_t)(4 * a + c) * S + s) * N + p]) * xj[c];
   }
   for (int a = 0; a < 4; ++a) {
     double u = 0.0;
-    for (int c = 0; c < 4; ++c) u += double(Dinv[(4 * a + c) * Np + i]) * r[c];
+    for (int c = 0; c < 4; ++c) u += double(Dinv[(size_t)(4 * a + c) * N + p]) * r[c];
     x[a * Np + i] = K(u);
+  }
+}
+
+// colour-ordered copies of the side blocks and inverted diagonal blocks
+template <typename M>
+__global__ void k_gs_order(int N, int Np, int S, const int* __restrict__ cells,
+                           const M* __restrict__ Blk, const M* __restrict__ Dinv,
+                           M* __restrict__ Bc, M* __restrict__ Dc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int i = cells[p];
+  for (int k = 0; k < 16; ++k) {
+    for (int s = 0; s < S; ++s)
+      Bc[((size_t)k * S + s) * N + p] = Blk[((size_t)k * S + s) * Np + i];
+    Dc[(size_t)k * N + p] = Dinv[(size_t)k * Np + i];
   }
 }
 
@@ -903,10 +923,24 @@ static int prepare_multicolor(ccfd_gpu_ctx* c) {
   for (int k = 0; k < ncol; ++k) s->mcStart[k + 1] += s->mcStart[k];
   std::vector<int> items(N), pos(s->mcStart.begin(), s->mcStart.end() - 1);
   for (int i = 0; i < N; ++i) items[pos[col[i]]++] = i;
+  // neighbour element of every side in colour order (-1: boundary, absent
+  // or a side onto the element itself)
+  std::vector<int> nb((size_t)S * std::max(N, 1), -1);
+  for (int p = 0; p < N; ++p)
+    for (int k = 0; k < S; ++k) {
+      const int lk = link[(size_t)k * Np + items[p]];
+      if (lk >= 0 && (lk >> 2) != items[p]) nb[(size_t)k * N + p] = lk >> 2;
+    }
+  const size_t kb = c->prm.precision == CCFD_FP64 ? 8 : 4;
   s->mcCells = (int*)imalloc(c, s, (size_t)std::max(N, 1) * 4);
-  if (!s->mcCells)
+  s->mcLink = (int*)imalloc(c, s, nb.size() * 4);
+  s->mcBlk = imalloc(c, s, 16 * (size_t)S * std::max(N, 1) * kb);
+  s->mcDinv = imalloc(c, s, 16 * (size_t)std::max(N, 1) * kb);
+  if (!s->mcCells || !s->mcLink || !s->mcBlk || !s->mcDinv)
     return fail(CCFD_ENOMEM, "multicolour schedule allocation failed");
   if (hipMemcpyAsync(s->mcCells, items.data(), N * 4, hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipMemcpyAsync(s->mcLink, nb.data(), nb.size() * 4, hipMemcpyHostToDevice,
                      c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
     return fail(CCFD_EHIP, "multicolour schedule upload failed");
@@ -1101,6 +1135,10 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
                      S, c->link, dt, dtv, c->area, (M*)s->Dg, (M*)s->Blk,
                      (M*)s->Dinv, s->flag);
   if (coarse) mg_assemble<M>(c);
+  if (s->haveMC)
+    hipLaunchKernelGGL(k_gs_order<M>, cells_grid(c), dim3(256), 0, c->st, N, Np,
+                       S, s->mcCells, (const M*)s->Blk, (const M*)s->Dinv,
+                       (M*)s->mcBlk, (M*)s->mcDinv);
   int flag = 0;
   hipMemcpyAsync(&flag, s->flag, 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
@@ -1122,8 +1160,9 @@ static void gs0(ccfd_gpu_ctx* c, int sweeps, bool zero, const K* b, K* x) {
   auto update = [&](int k) {
     const int n = s->mcStart[k + 1] - s->mcStart[k];
     hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
-                       c->st, n, s->mcCells + s->mcStart[k], Np, S, c->link,
-                       (const M*)s->Blk, (const M*)s->Dinv, b, x);
+                       c->st, n, s->mcStart[k], s->mcCells, c->N, Np, S,
+                       s->mcLink, (const M*)s->mcBlk, (const M*)s->mcDinv, b,
+                       x);
   };
   // a colour updated twice in a row gives the same values: skip repeats
   int last = -1;
