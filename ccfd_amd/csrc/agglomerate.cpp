@@ -166,12 +166,30 @@ void build_agglomeration(int N, int Np, int S, const int* link,
                      [](const auto& x, const auto& y) { return x.first < y.first; });
     to_csr(n2, dItems, L.dSrcStart, L.dSrc);
     to_csr(L.nnz, oItems, L.oSrcStart, L.oSrc);
-    // greedy distance-1 colouring for the Gauss-Seidel sweeps
+    // greedy distance-1 colouring for the Gauss-Seidel sweeps, in
+    // breadth-first order (few colours, two on a bipartite graph)
     {
+      std::vector<int> order;
+      order.reserve(n2);
+      std::vector<char> seen(n2, 0);
+      for (int r = 0; r < n2; ++r) {
+        if (seen[r]) continue;
+        seen[r] = 1;
+        size_t head = order.size();
+        order.push_back(r);
+        while (head < order.size()) {
+          const int u = order[head++];
+          for (int t = L.rowStart[u]; t < L.rowStart[u + 1]; ++t)
+            if (!seen[L.col[t]]) {
+              seen[L.col[t]] = 1;
+              order.push_back(L.col[t]);
+            }
+        }
+      }
       std::vector<int> color(n2, -1);
       std::vector<char> used;
       int ncol = 0;
-      for (int a = 0; a < n2; ++a) {
+      for (int a : order) {
         used.assign(ncol + 1, 0);
         for (int t = L.rowStart[a]; t < L.rowStart[a + 1]; ++t)
           if (color[L.col[t]] >= 0) used[color[L.col[t]]] = 1;

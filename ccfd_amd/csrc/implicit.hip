@@ -892,9 +892,8 @@ static int prepare_lusgs(ccfd_gpu_ctx* c, int order) {
   return CCFD_OK;
 }
 
-// multicolour sweep schedule (precond = 2): greedy distance-1 colouring in
-// element order; the sweeps run colour by colour, lower neighbours being
-// those of smaller colour
+// multicolour sweep schedule (precond = 2, 3): greedy distance-1
+// colouring; the sweeps run colour by colour
 static int prepare_multicolor(ccfd_gpu_ctx* c) {
   ImplicitState* s = c->imp;
   if (s->haveMC) return CCFD_OK;
@@ -904,10 +903,35 @@ static int prepare_multicolor(ccfd_gpu_ctx* c) {
                      hipMemcpyDeviceToHost, c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
     return fail(CCFD_EHIP, "link download failed");
+  // greedy colouring in breadth-first order over the face graph (from the
+  // lowest unvisited element, neighbours in side order): on a bipartite
+  // mesh such as a structured quadrilateral grid this gives two colours,
+  // where element order can give five (the extra small colours cost whole
+  // sweep launches)
+  std::vector<int> order;
+  order.reserve(N);
+  {
+    std::vector<char> seen(N, 0);
+    for (int r = 0; r < N; ++r) {
+      if (seen[r]) continue;
+      seen[r] = 1;
+      size_t head = order.size();
+      order.push_back(r);
+      while (head < order.size()) {
+        const int u = order[head++];
+        for (int k = 0; k < S; ++k) {
+          const int lk = link[(size_t)k * Np + u];
+          if (lk < 0 || seen[lk >> 2]) continue;
+          seen[lk >> 2] = 1;
+          order.push_back(lk >> 2);
+        }
+      }
+    }
+  }
   std::vector<int> col(N, -1);
   std::vector<char> used;
   int ncol = 0;
-  for (int i = 0; i < N; ++i) {
+  for (int i : order) {
     used.assign(ncol + 1, 0);
     for (int k = 0; k < S; ++k) {
       const int lk = link[(size_t)k * Np + i];
