@@ -478,7 +478,9 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   std::copy(h.ghostBary.begin(), h.ghostBary.end(), ghostBary.begin());
   std::vector<double> venk(Np, 0.0);
   const size_t sl = (size_t)S * Np;
+  c->hasExact = false;
   for (int b = 0; b < m->nBC; ++b) {
+    if (m->bc[b].type == 6) c->hasExact = true;
     c->bct.bc[b].type = m->bc[b].type;
     c->bct.bc[b].exactFunc = m->bc[b].exactFunc;
     for (int v = 0; v < 4; ++v) c->bct.bc[b].pVar[v] = m->bc[b].pVar[v];
@@ -643,6 +645,15 @@ extern "C" int ccfd_gpu_get_ut(ccfd_gpu_ctx* c, double* ut) {
   return download_aos(c, as_double(c, c->ut, 4 * (size_t)c->Np), ut);
 }
 
+// exact-function ghost states (side 0: barycentre, 1: side midpoint) after
+// the boundary kernel that left them out, where the mesh has any
+template <typename T>
+static void exact_ghosts(ccfd_gpu_ctx* c, const OpArgs<T>& a, int side) {
+  if (c->hasExact && c->G > 0)
+    hipLaunchKernelGGL(bc_exact_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
+                       0, c->st, a, side);
+}
+
 // limited gradients and side states of the current state at the time of
 // the last ccfd_gpu_fvtd call, into c->ux, c->uy, c->side (the fused
 // operator does not materialise them)
@@ -653,6 +664,7 @@ static void recompute_recon(ccfd_gpu_ctx* c) {
   if (c->G > 0)
     hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
                        0, c->st, a);
+  exact_ghosts<T>(c, a, 0);
   dim3 g(c->nb), b(kBlock);
   const int lim = c->prm.limiter;
   if (c->S == 3) {
@@ -781,6 +793,7 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
       tspan_begin(c, &eb);
       hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
                          0, c->st, a);
+      exact_ghosts<T>(c, a, 0);
       const dim3 gg((c->G + 127) / 128), gb(128);
       const int lim = c->prm.limiter;
       if (c->S == 3) {
@@ -790,6 +803,7 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
         if (lim == 1) hipLaunchKernelGGL((bc_side_recon_kernel<T, 1, 4>), gg, gb, 0, c->st, a);
         else hipLaunchKernelGGL((bc_side_recon_kernel<T, 2, 4>), gg, gb, 0, c->st, a);
       }
+      exact_ghosts<T>(c, a, 1);
       tspan_end(c, eb, KT_OTHER);
     }
     FusedArgs f{c->haloStart, c->haloStart + 1, c->haloCells, c->tile};
@@ -808,6 +822,7 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
       tspan_begin(c, &eb);
       hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
                          0, c->st, a);
+      exact_ghosts<T>(c, a, 0);
       tspan_end(c, eb, KT_OTHER);
     }
     tspan_begin(c, &e0);
@@ -825,6 +840,7 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
     tspan_begin(c, &es);
     hipLaunchKernelGGL(bc_side_kernel<T>, dim3((c->G + 127) / 128), dim3(128), 0,
                        c->st, a);
+    exact_ghosts<T>(c, a, 1);
     tspan_end(c, es, KT_OTHER);
   }
   tspan_begin(c, &e1);

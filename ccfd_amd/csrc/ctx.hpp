@@ -53,6 +53,7 @@ struct ccfd_gpu_ctx {
   ccfd::Geom<double> g64;
   ccfd::Geom<float> g32;
   ccfd::BCTable bct{};
+  bool hasExact = false;   // an exact-function boundary (type 6) is present
   ccfd::ExactCfg ex{};
   // state (buffers sized for double, used as the operator type T)
   void* prim[2] = {nullptr, nullptr};

@@ -129,7 +129,7 @@ __global__ void bc_side_recon_kernel(OpArgs<T> a) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) in[v] = p[v] + gpx * gx[v] + gpy * gy[v];
   T x[2] = {gpx + T(a.bary[i]), gpy + T(a.bary[Np + i])};
-  bc_ghost<T>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
+  bc_ghost<T, false>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
               a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];

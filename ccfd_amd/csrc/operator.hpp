@@ -254,7 +254,7 @@ __global__ void bc_bary_kernel(OpArgs<T> a) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) in[v] = a.prim[v * Np + i];
   T x[2] = {T(a.ghostBary[g]), T(a.ghostBary[a.G + g])};
-  bc_ghost<T>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
+  bc_ghost<T, false>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
               a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) a.ghostP[v * a.G + g] = out[v];
@@ -279,10 +279,37 @@ __global__ void bc_side_kernel(OpArgs<T> a) {
   }
   T x[2] = {a.gp[k * Np + i] + T(a.bary[i]),
             a.gp[(S + k) * Np + i] + T(a.bary[Np + i])};
-  bc_ghost<T>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
+  bc_ghost<T, false>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
               a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
+}
+
+// exact-function ghost states (boundary type 6, src/boundary.c:272-274):
+// the common boundary kernels leave them out (they would all carry the
+// exact Riemann solver's stack); this kernel runs after them where a mesh
+// has such boundaries. side = 0: at the mirrored barycentre (ghostP, as
+// bc_bary_kernel), 1: at the side midpoint (ghostS, as bc_side_kernel)
+template <typename T>
+__global__ void bc_exact_kernel(OpArgs<T> a, int side) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G || a.ghostBC[g] < 0) return;
+  const BCDev b = op_bc(a, a.ghostBC[g]);
+  if (b.type != 6) return;
+  const int i = a.ghostCell[g], k = a.ghostK[g];
+  const int Np = a.Np, S = a.S;
+  T x[2], out[4];
+  if (side) {
+    x[0] = a.gp[k * Np + i] + T(a.bary[i]);
+    x[1] = a.gp[(S + k) * Np + i] + T(a.bary[Np + i]);
+  } else {
+    x[0] = T(a.ghostBary[g]);
+    x[1] = T(a.ghostBary[a.G + g]);
+  }
+  bc_ghost_exact<T>(a.eq, a.ex, b.exactFunc, x, op_time(a, i), out);
+  T* dst = side ? a.ghostS : a.ghostP;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) dst[v * a.G + g] = out[v];
 }
 
 // ------------------------------------------------------------- flux kernel

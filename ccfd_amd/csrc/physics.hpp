@@ -751,17 +751,10 @@ CCFD_HD void exact_func(const Eq<T>& q, const ExactCfg& ex, int f,
   }
 }
 
-// characteristic (type 5) and exact-function (type 6) ghost states: kept out
-// of line so their transcendental / iterative code does not set the register
-// budget of the operator kernels (src/boundary.c:217-274)
+// characteristic ghost state (type 5; src/boundary.c:217-271)
 template <typename T>
-CCFD_HD_NOINLINE void bc_ghost_rare(const Eq<T>& q, const BCDev& b,
-                                    const ExactCfg& ex, T nx, T ny, T time,
-                                    const T* in, T* out, const T* x) {
-  if (b.type == 6) {
-    exact_func(q, ex, b.exactFunc, x, time, out);
-    return;
-  }
+CCFD_HD void bc_ghost_char(const Eq<T>& q, const BCDev& b, T nx, T ny,
+                           const T* in, T* out) {
   T bp[4] = {T(b.pVar[0]), T(b.pVar[1]), T(b.pVar[2]), T(b.pVar[3])};
   T c = tsqrt(q.gam * bp[3] / bp[0]);
   T v = nx * bp[1] + ny * bp[2];
@@ -783,8 +776,19 @@ CCFD_HD_NOINLINE void bc_ghost_rare(const Eq<T>& q, const BCDev& b,
   out[2] = ny * vx + nx * vy;
 }
 
-// boundary() (src/boundary.c:147-294). n: outward normal of the interior side.
+// exact-function ghost state (type 6; src/boundary.c:272-274), out of line:
+// a kernel that contains this call reserves the exact Riemann solver's
+// stack at every launch
 template <typename T>
+CCFD_HD_NOINLINE void bc_ghost_exact(const Eq<T>& q, const ExactCfg& ex, int f,
+                                     const T* x, T time, T* out) {
+  exact_func(q, ex, f, x, time, out);
+}
+
+// boundary() (src/boundary.c:147-294). n: outward normal of the interior side.
+// EXACT = false leaves exact-function boundaries (type 6) to a kernel of
+// their own (NaN here), so that the common boundary kernels carry no stack.
+template <typename T, bool EXACT = true>
 CCFD_HD void bc_ghost(const Eq<T>& q, const BCDev& b, const ExactCfg& ex,
                       T nx, T ny, T time, const T in[4], T out[4],
                       const T x[2]) {
@@ -810,8 +814,13 @@ CCFD_HD void bc_ghost(const Eq<T>& q, const BCDev& b, const ExactCfg& ex,
       out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
       break;
     case 5:
+      bc_ghost_char<T>(q, b, nx, ny, in, out);
+      break;
     case 6:
-      bc_ghost_rare<T>(q, b, ex, nx, ny, time, in, out, x);
+      if constexpr (EXACT)
+        bc_ghost_exact<T>(q, ex, b.exactFunc, x, time, out);
+      else
+        out[0] = out[1] = out[2] = out[3] = T(NAN);
       break;
     case 8: {
       T c = tsqrt(q.gam * in[3] / in[0]);

@@ -172,6 +172,30 @@ def test_fvtd_matches_reference_outputs(name):
     gs.close()
 
 
+@pytest.mark.parametrize("order,variant", [(1, G.KV_AUTO), (2, G.KV_SPLIT),
+                                           (2, G.KV_FUSED),
+                                           (2, G.KV_FUSED_FACE)],
+                         ids=["order1", "split", "fused", "fused_face"])
+def test_exact_function_boundaries_on_every_path(order, variant):
+    """exact-function boundaries (type 6) are set by their own kernel after
+    the common boundary kernels: every operator implementation and both
+    orders stay bit-identical to the oracle (mesh and settings of the
+    reference-generated exact-boundary Sod fixture)"""
+    g = GOLDEN["exactbc_sod_t01_o2_hll"]
+    mesh, _, prim, c = golden_io.build(g)
+    assert any(int(r[0]) // 100 == 6 for r in g["bcs"])
+    kw = dict(left=c["left"], right=c["right"], x0=c["x0"])
+    prm = G.params(flux=c["flux"], order=order, limiter=c["limiter"],
+                   venk_k=c["venk_k"], **kw)
+    cfg = B.make_cfg(flux=c["flux"], order=order, limiter=c["limiter"],
+                     venk_k=c["venk_k"], **kw)
+    gs = G.Solver(mesh, prm)
+    gs.set_variant(variant)
+    orc = B.Oracle(mesh, cfg)
+    _check_bitwise(gs, orc, prim, time=c["time"], order=order)
+    gs.close()
+
+
 def test_fused_rejected_where_it_does_not_apply():
     raw = M.cartesian(8, 4, (0.0, 0.0), (1.0, 0.5), (201, 501, 101, 401))
     bcs = [M.BC(201), M.BC(501, rho=1.0, mach=0.3, alpha=0.0, pressure=1.0),
