@@ -3,9 +3,10 @@
 // matrixVector, buildMatrix, calcDinv, LUSGS src/linearSolver.c:115-510).
 //
 //  * Newton loop on the host, all vectors device-resident ([4][Np], FP64).
-//  * GMRES_M: one cycle of at most nKdim iterations, modified Gram-Schmidt
-//    with H on the device, Givens rotations and back-solve on the host in
-//    FP64. Krylov vectors V, Z are FP64, or FP32 in FP32-flux mode.
+//  * GMRES_M: one cycle of at most nKdim iterations (optionally restarted
+//    cycles), modified Gram-Schmidt with H on the device, Givens rotations
+//    and back-solve on the host in FP64. Krylov vectors V, Z are FP64, or
+//    FP32 in FP32-flux mode.
 //  * matrixVector: Frechet difference with epsFD = sqrt(eps)/|v| in FP64
 //    (eps of the operator's arithmetic).
 //  * buildMatrix: the reference records only the diagonal and face-neighbour
@@ -15,9 +16,15 @@
 //    (2nd order) / distance-2 (1st order) colouring are perturbed together,
 //    so each recorded block sees exactly the one perturbation the reference
 //    applies for it.
-//  * LUSGS: forward / backward Gauss-Seidel sweeps in the reference's
-//    element order, executed level by level (cells of one level have no
-//    dependence on each other), so each row's arithmetic is the reference's.
+//  * LUSGS (precond = 1): forward / backward Gauss-Seidel sweeps in the
+//    reference's element order, executed level by level (cells of one level
+//    have no dependence on each other), so each row's arithmetic is the
+//    reference's. precond = 2: symmetric Gauss-Seidel sweeps by colour on
+//    the Jacobian with respect to the conservative state; precond = 3: a
+//    multigrid V-cycle of agglomerated elements around those sweeps.
+//  * Steady solves (ccfd_gpu_implicit_steady): local pseudo-time steps, rows
+//    scaled by sqrt(area) / dt, CFL number driven by the GMRES convergence,
+//    Newton updates bounded to 20% density / pressure change.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,7 +63,7 @@ struct ImplicitState {
   int* lvlF = nullptr;          // cells of forward levels (CSR items)
   int* lvlB = nullptr;          // backward
   std::vector<int> lvlFStart, lvlBStart;
-  // multicolour Gauss-Seidel sweeps (precond = 2): greedy distance-1
+  // multicolour Gauss-Seidel sweeps (precond = 2, 3): greedy distance-1
   // colouring; elements of one colour are updated together
   bool haveMC = false;
   int* mcCells = nullptr;       // cells by colour (CSR items)
