@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .gpu import FP64, KV_AUTO, GpuError, Params, lib as _gpu_lib
+from .gpu import GpuError, Params, lib as _gpu_lib
 
 
 class NoRoom(GpuError):
