@@ -370,3 +370,72 @@ def test_steady_solve_continues_across_calls():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def _local_step_reference(mesh, cfg, prim0, cfl, gam=1.4):
+    """the backward-Euler pseudo-step with local steps, X - Q - dt_i R(X) = 0,
+    solved by Newton with a dense finite-difference Jacobian; R from the
+    oracle, dt_i the CFL limit of calcTimeStep per element
+    (src/timeDiscretization.c:253-271) at the initial state"""
+    from ccfd_amd import serve as SV
+    orc = B.Oracle(mesh, cfg)
+    p = prim0
+    a = np.sqrt(gam * p[:, 3] / p[:, 0])
+    dt = cfl * mesh.area / ((np.abs(p[:, 1]) + a) * mesh.sx
+                            + (np.abs(p[:, 2]) + a) * mesh.sy)
+    orc.set_prim(prim0)
+    Q = orc.cons().copy()
+
+    def R(X):
+        orc.set_prim(SV.cons_to_prim(X.reshape(-1, 4), gam))
+        return orc.fvtd(0.0).reshape(-1)
+
+    def F(X):
+        return X - Q.reshape(-1) - np.repeat(dt, 4) * R(X)
+
+    X = Q.reshape(-1).copy()
+    f0 = np.linalg.norm(F(X))
+    for _ in range(8):
+        fx = F(X)
+        if np.linalg.norm(fx) <= 1e-13 * f0:
+            break
+        J = np.empty((X.size, X.size))
+        for j in range(X.size):
+            h = 1e-7 * max(1.0, abs(X[j]))
+            Xp = X.copy()
+            Xp[j] += h
+            J[:, j] = (F(Xp) - fx) / h
+        X = X - np.linalg.solve(J, fx)
+    assert np.linalg.norm(F(X)) <= 1e-12 * f0
+    return X.reshape(-1, 4), Q
+
+
+def test_steady_pseudo_step_solves_local_backward_euler():
+    """one pseudo-step of the steady driver, its Newton loop converged
+    tightly, lands on the solution of X - Q - dt_i R(X) = 0 with the local
+    steps dt_i of calcTimeStep's CFL limit (CFL 1 on the first step), as an
+    independent dense Newton solve on the oracle's operator finds it: the
+    local steps, the row scaling (which must not move the solution) and
+    the Newton loop of the steady path, pinned at once"""
+    raw = M.naca_ogrid(32, 12, wall_bc=101, far_bc=501)
+    bcs = [M.BC(101), M.BC(501, rho=1.0, mach=0.5, alpha=2.0, pressure=1.0)]
+    mesh = M.Mesh(raw, bcs)
+    c = np.sqrt(1.4)
+    al = np.radians(2.0)
+    prim = np.tile([1.0, 0.5 * c * np.cos(al), 0.5 * c * np.sin(al), 1.0],
+                   (mesh.n_elems, 1))
+    prim += 0.02 * M.smooth_state(mesh.bary, seed=9) - 0.02 * prim
+    kw = dict(flux=2, order=2, limiter=2, venk_k=1000.0)
+    X, Q = _local_step_reference(mesh, B.make_cfg(**kw), prim, 1.0)
+    gs = G.Solver(mesh, G.params(**kw))
+    gs.set_prim(prim)
+    assert np.allclose(gs.cons(), Q, rtol=1e-14, atol=1e-15)
+    ic = G.ImplicitCfg(60, 30, 2, 1e-11, 1e-8, 0.9, 1, 2, 3)
+    _, _, steps, nn, _ = gs.implicit_steady(ic, 1.0, 1, 1e-30)
+    err = cases.rel_err(gs.cons(), X)
+    print("Newton iterations %d, rel err vs the dense solve %.2e, step size "
+          "%.2e" % (nn, err, cases.rel_err(X, Q)))
+    assert steps == 1
+    assert err < 1e-7
+    assert cases.rel_err(X, Q) > 1e-3      # the step moved the state
+    gs.close()
