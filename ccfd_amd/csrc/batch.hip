@@ -274,11 +274,6 @@ __global__ void k_bfill_int(int* __restrict__ p, int n, int value) {
   if (i < n) p[i] = value;
 }
 
-__global__ void k_bfill_dbl(double* __restrict__ p, int n, double value) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = value;
-}
-
 // ------------------------------------------------------------ host helpers
 
 static Eq<double> beq(const ccfd_batch* b) {
