@@ -83,6 +83,11 @@ def oracle_lib():
         L.oracle_implicit_step.argtypes = [
             C.c_void_p, C.POINTER(ImplicitCfg), C.c_double, C.c_double, dp,
             ip, ip]
+        L.oracle_build_matrix.restype = C.c_int
+        L.oracle_build_matrix.argtypes = [C.c_void_p, C.c_double, C.c_double,
+                                          dp, dp, dp]
+        L.oracle_lusgs.restype = C.c_int
+        L.oracle_lusgs.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.oracle_rk_coeffs.restype = C.c_int
         L.oracle_rk_coeffs.argtypes = [C.c_int, C.c_int, dp]
         L.oracle_exact_func.argtypes = [C.POINTER(OracleCfg), C.c_void_p,
@@ -151,6 +156,30 @@ class Oracle:
         rc = self.L.oracle_implicit_step(self.p, C.byref(ic), t, dt, _d(r),
                                          C.byref(nn), C.byref(ng))
         return rc, r, nn.value, ng.value
+
+    def build_matrix(self, time, dt):
+        """blocks of I - dt dR/dpVar at the current state: diagonal Dg
+        (N, 4, 4), per slot Blk (S, 4, 4) (row: the slot's element, column:
+        its neighbour) and the inverses Dinv (N, 4, 4) of the diagonal ones"""
+        Dg = np.zeros((self.N, 4, 4))
+        Blk = np.zeros((self.S, 4, 4))
+        Dinv = np.zeros((self.N, 4, 4))
+        rc = self.L.oracle_build_matrix(self.p, time, dt, _d(Dg), _d(Blk),
+                                        _d(Dinv))
+        if rc:
+            raise RuntimeError("oracle_build_matrix failed (%d)" % rc)
+        return Dg, Blk, Dinv
+
+    def lusgs(self, Dinv, Blk, B):
+        """one LU-SGS application to B (N, 4) with the blocks above"""
+        Dinv = np.ascontiguousarray(Dinv, dtype=np.float64)
+        Blk = np.ascontiguousarray(Blk, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        X = np.zeros((self.N, 4))
+        rc = self.L.oracle_lusgs(self.p, _d(Dinv), _d(Blk), _d(B), _d(X))
+        if rc:
+            raise RuntimeError("oracle_lusgs failed (%d)" % rc)
+        return X
 
     def residual(self, ut):
         ut = np.ascontiguousarray(ut, dtype=np.float64)

@@ -1582,3 +1582,39 @@ int oracle_implicit_step(oracle_ctx *o, const oracle_implicit_cfg *ic,
 		oracle_global_residual(o, o->ut, res);
 	return 0;
 }
+
+/* buildMatrix (src/linearSolver.c:186-252) on its own, at the current state:
+ * R(X) from one operator call, then the blocks of I - dt dR/dpVar and the
+ * inverses of the diagonal ones, as GMRES_M's preconditioner set-up makes
+ * them. */
+int oracle_build_matrix(oracle_ctx *o, double time, double dt, double *Dg,
+		double *Blk, double *Dinv)
+{
+	long N = o->N;
+	/* R(X) in a buffer of its own: the implicit step's arrays stay as
+	 * they are */
+	double *saved = o->RXK, *r = malloc(sizeof(double) * 4 * N);
+	if (!r)
+		return -3;
+	oracle_fvtd(o, time, NULL, NULL, NULL, NULL, NULL);
+	memcpy(r, o->ut, sizeof(double) * 4 * N);
+	o->RXK = r;
+	lusgs_t L = {Dinv, Blk, Dg};
+	int rc = build_matrix(o, &L, time, dt);
+	o->RXK = saved;
+	free(r);
+	return rc;
+}
+
+/* LUSGS (src/linearSolver.c:261-333) with the blocks of oracle_build_matrix */
+int oracle_lusgs(const oracle_ctx *o, const double *Dinv, const double *Blk,
+		const double *B, double *X)
+{
+	double *Xs = malloc(sizeof(double) * 4 * o->N);
+	if (!Xs)
+		return -3;
+	lusgs_t L = {(double *)Dinv, (double *)Blk, NULL};
+	lusgs_apply(o, &L, B, X, Xs);
+	free(Xs);
+	return 0;
+}

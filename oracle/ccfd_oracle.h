@@ -83,6 +83,22 @@ typedef struct oracle_implicit_cfg {
 int oracle_implicit_step(oracle_ctx *o, const oracle_implicit_cfg *ic,
 		double t, double dt, double res[4], int *nNewton, int *nGMRES);
 
+/*
+ * The Jacobian blocks and the preconditioner on their own (buildMatrix +
+ * calcDinv, src/linearSolver.c:136-252; LUSGS, :261-333). Dg and Dinv hold
+ * 16 doubles per element, Blk 16 per slot: the block of I - dt dR/dpVar in
+ * the row of the slot's element and the column of its neighbour, row-major
+ * (row: component of dU/dt, column: primitive variable). oracle_build_matrix
+ * perturbs and restores the primitive state as the reference does (so it
+ * may move by an ulp) and returns -2 for a singular diagonal block;
+ * oracle_lusgs returns X = (D + U)^-1 D (D + L)^-1 B, lower and upper by
+ * element index. Both return -3 when out of memory.
+ */
+int oracle_build_matrix(oracle_ctx *o, double time, double dt, double *Dg,
+		double *Blk, double *Dinv);
+int oracle_lusgs(const oracle_ctx *o, const double *Dinv, const double *Blk,
+		const double *B, double *X);
+
 /* RK coefficient table (src/timeDiscretization.c:94-178); returns -1 for an
  * unsupported (stages, order) pair */
 int oracle_rk_coeffs(int nRKstages, int timeOrder, double coeff[6]);

@@ -8,13 +8,24 @@ linearSolver.c cannot be compiled here (they include the CGNS headers):
 * as dt -> 0 the step approaches explicit Euler, U0 + dt R(U0), to O(dt^2);
 * LU-SGS preconditioning changes the Krylov path, not the converged answer,
   and cuts the GMRES iterations;
-* a uniform state on a periodic mesh is a fixed point (R = 0).
+* a uniform state on a periodic mesh is a fixed point (R = 0);
+* the Jacobian blocks (buildMatrix) are I - dt dR/dpVar: the column of one
+  primitive variable of one element is what a forward difference of R,
+  from separate operator calls with a step of its own, gives in that
+  element and its face neighbours, and the stored inverses invert the
+  diagonal blocks;
+* LU-SGS returns x with (D + L) D^-1 (D + U) x = b for the blocks above,
+  the product assembled here with scipy from the blocks by element index.
 
 A wrong sign or factor in the Newton function, the Frechet difference, the
-Jacobian assembly or the preconditioner sweeps fails one of these.
+Jacobian assembly or the preconditioner sweeps fails one of these; a
+transposed block or swapped lower and upper halves fail the last two, which
+the converged steps alone would not show (the preconditioner changes the
+Krylov path, not the answer).
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from ccfd_amd import mesh as M
 from oracle import bindings as B
@@ -118,3 +129,99 @@ def test_uniform_state_is_fixed_point():
                                   0.0, dt)
     assert rc == 0
     np.testing.assert_allclose(o.cons(), u0, rtol=1e-14, atol=1e-14)
+
+
+def _block_case(kind, order):
+    if kind == "periodic":
+        mesh = cases.delaunay_mesh(12, seed=5, periodic=True)
+    else:
+        mesh, _ = cases.mixed_bc_mesh(12, 6)
+    prim = M.smooth_state(mesh.bary, seed=3)
+    cfg = B.make_cfg(flux=5 if order == 2 else 2, order=order, limiter=2,
+                     venk_k=5.0, cfl=4.0)
+    own, nbr = mesh.slot_elem(), mesh.nbr
+    inner = nbr >= 0
+    # one slot per (element, neighbour) pair and no self-neighbours, so
+    # every stored block is one entry of the dense matrix
+    assert not np.any(nbr[inner] == own[inner])
+    pairs = own[inner] * mesh.n_elems + nbr[inner]
+    assert len(np.unique(pairs)) == len(pairs)
+    return mesh, prim, cfg, own, nbr
+
+
+def _bsr(blocks, rows, cols, n):
+    """sparse 4n x 4n matrix from 4x4 blocks at (rows, cols)"""
+    a = np.arange(4)
+    r = np.broadcast_to(4 * rows[:, None, None] + a[None, :, None], blocks.shape)
+    c = np.broadcast_to(4 * cols[:, None, None] + a[None, None, :], blocks.shape)
+    return sp.csr_matrix((blocks.ravel(), (r.ravel(), c.ravel())),
+                         shape=(4 * n, 4 * n))
+
+
+@pytest.mark.parametrize("kind", ["periodic", "walls"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_jacobian_blocks_are_the_derivative_of_R(kind, order):
+    mesh, prim, cfg, own, nbr = _block_case(kind, order)
+    o = B.Oracle(mesh, cfg)
+    o.set_prim(prim)
+    dt, _, _ = o.calc_dt(0.0, 1e9, 1e9)
+    Dg, Blk, Dinv = o.build_matrix(0.0, dt)
+    np.testing.assert_allclose(Dinv, np.linalg.inv(Dg), rtol=1e-10,
+                               atol=1e-12 * np.abs(Dinv).max())
+    # forward differences, one primitive component at a time, with a step
+    # of their own: R is not differentiable at boundary states whose ghost
+    # copies or mirrors the interior (a central difference there misses by
+    # 6e-2), and the reference's finite differences record the forward slope
+    rng = np.random.default_rng(order)
+    h = 1e-7
+    o.set_prim(prim)
+    r0 = o.fvtd(0.0)
+    worst = 0.0
+    for k in rng.choice(mesh.n_elems, 8, replace=False):
+        nb = np.nonzero(nbr == k)[0]            # slots of rows j, column k
+        assert len(nb) >= 2
+        for iv in range(4):
+            p = prim.copy()
+            p[k, iv] += h
+            o.set_prim(p)
+            jd = (o.fvtd(0.0) - r0) / h         # dR/dpVar[k, iv], every row
+            e = np.eye(4)[iv]
+            want = {k: e - dt * jd[k]}
+            got = {k: Dg[k, :, iv]}
+            for s in nb:
+                want[own[s]] = -dt * jd[own[s]]
+                got[own[s]] = Blk[s, :, iv]
+            if order == 1:
+                # first order: R reaches the face neighbours only
+                far = np.setdiff1d(np.arange(mesh.n_elems), list(want))
+                assert np.abs(jd[far]).max() == 0.0
+            scale = max(np.abs(w).max() for w in want.values())
+            err = max(np.abs(got[j] - want[j]).max() for j in want) / scale
+            worst = max(worst, err)
+    print("%s order %d: worst block error %.2e" % (kind, order, worst))
+    assert worst < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["periodic", "walls"])
+def test_lusgs_inverts_the_symmetric_gauss_seidel_product(kind):
+    mesh, prim, cfg, own, nbr = _block_case(kind, 2)
+    o = B.Oracle(mesh, cfg)
+    o.set_prim(prim)
+    dt, _, _ = o.calc_dt(0.0, 1e9, 1e9)
+    Dg, Blk, Dinv = o.build_matrix(0.0, dt)
+    n = mesh.n_elems
+    b = np.random.default_rng(9).normal(size=(n, 4))
+    x = o.lusgs(Dinv, Blk, b)
+    diag = np.arange(n)
+    lo = (nbr >= 0) & (nbr < own)
+    up = nbr > own
+    D = _bsr(Dg, diag, diag, n)
+    Di = _bsr(np.linalg.inv(Dg), diag, diag, n)
+    L = _bsr(Blk[lo], own[lo], nbr[lo], n)
+    U = _bsr(Blk[up], own[up], nbr[up], n)
+    r = (D + L) @ (Di @ ((D + U) @ x.ravel())) - b.ravel()
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print("%s: |(D+L) D^-1 (D+U) x - b| / |b| = %.2e" % (kind, rel))
+    assert rel < 1e-12
+    # and the sweeps are not block Jacobi: the off-diagonal blocks matter
+    assert np.linalg.norm(D @ x.ravel() - b.ravel()) > 1e-3 * np.linalg.norm(b)
