@@ -1,5 +1,7 @@
-"""The C ABI libraries load and export every function include/*.h declares
-(no device calls: runs without a GPU)."""
+"""The C ABI libraries load and export every function include/*.h declares,
+and every entry point of the HIP library rejects a null handle or bad
+arguments with CCFD_EINVAL and a message before it touches a device (no
+device calls: runs without a GPU)."""
 import ctypes as C
 import os
 import re
@@ -39,3 +41,64 @@ def test_every_header_is_checked():
                   if f.endswith(".h"))
     assert hdrs == ["ccfd_batch.h", "ccfd_dropin.h", "ccfd_gpu.h",
                     "ccfd_mesh.h"]
+
+
+def _entry_points(header):
+    """(name, [parameter declarations]) of every int-returning function"""
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = []
+    for name, args in re.findall(r"^int\s+(\w+)\s*\(([^)]*)\)", src, re.M):
+        args = [a.strip() for a in args.split(",")]
+        out.append((name, [a for a in args if a and a != "void"]))
+    return out
+
+
+def _null_args(params):
+    """NULL for every pointer, 1 for every integer, 0.0 for every double"""
+    vals = []
+    for a in params:
+        if "*" in a or "[" in a:
+            vals.append(C.c_void_p(None))
+        elif "double" in a:
+            vals.append(C.c_double(0.0))
+        elif "size_t" in a or "long" in a:
+            vals.append(C.c_long(1))
+        else:
+            vals.append(C.c_int(1))
+    return vals
+
+
+@pytest.mark.parametrize("header", ["ccfd_gpu.h", "ccfd_batch.h"])
+def test_null_handles_are_rejected(header):
+    L = C.CDLL(os.path.join(LIB, "libccfd_gpu.so"))
+    L.ccfd_gpu_last_error.restype = C.c_char_p
+    checked = 0
+    for name, params in _entry_points(header):
+        first = params[0] if params else ""
+        if not ("ccfd_gpu_ctx" in first or "ccfd_batch" in first):
+            continue
+        f = getattr(L, name)
+        f.restype = C.c_int
+        rc = f(*_null_args(params))
+        if name.endswith("_destroy"):
+            assert rc == 0, name      # like free(NULL)
+        else:
+            assert rc == -1, (name, rc)  # CCFD_EINVAL
+            assert L.ccfd_gpu_last_error(), name
+        checked += 1
+    assert checked >= 15
+
+
+def test_context_free_entry_points_check_arguments_first():
+    L = C.CDLL(os.path.join(LIB, "libccfd_gpu.so"))
+    L.ccfd_gpu_last_error.restype = C.c_char_p
+    L.ccfd_gpu_riemann_batch.argtypes = [C.c_int, C.c_int, C.c_double, C.c_long,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]
+    assert L.ccfd_gpu_riemann_batch(12, 0, 1.4, 4, None, None, None) == -1
+    assert b"unknown flux" in L.ccfd_gpu_last_error()
+    assert L.ccfd_gpu_riemann_batch(5, 0, 1.4, 4, None, None, None) == -1
+    assert L.ccfd_gpu_riemann_batch(5, 0, 1.4, -1, None, None, None) == -1
+    assert L.ccfd_gpu_riemann_batch(5, 0, 1.4, 0, None, None, None) == 0
+    L.ccfd_gpu_copy_probe.argtypes = [C.c_int, C.c_size_t, C.c_int, C.c_void_p]
+    assert L.ccfd_gpu_copy_probe(0, 1024, 1, None) == -1
