@@ -137,12 +137,41 @@ def pmc_traffic(kernel_key):
         return None
 
 
-def _density_err(a, b):
-    """relative L2 and Linf of the density of state a against state b"""
+def _rel_err(a, b):
+    """relative L2 and Linf of a against b"""
     import numpy as np
-    d = a[:, 0] - b[:, 0]
-    return (float(np.linalg.norm(d) / np.linalg.norm(b[:, 0])),
-            float(np.abs(d).max() / np.abs(b[:, 0]).max()))
+    d = a - b
+    return (float(np.linalg.norm(d) / np.linalg.norm(b)),
+            float(np.abs(d).max() / np.abs(b).max()))
+
+
+def _density_err(a, b):
+    """relative L2 and Linf of the density of primitive state a against b"""
+    return _rel_err(a[:, 0], b[:, 0])
+
+
+def _energy_err(a, b, gamma=1.4):
+    """relative L2 and Linf of the total energy per volume, p / (gamma - 1)
+    + rho |v|^2 / 2, of primitive state a against b"""
+    def energy(q):
+        return q[:, 3] / (gamma - 1.0) + 0.5 * q[:, 0] * (q[:, 1] ** 2 +
+                                                         q[:, 2] ** 2)
+    return _rel_err(energy(a), energy(b))
+
+
+def _err_summary(errs, both=None):
+    """max and median relative L2, max relative Linf over solves; with a
+    mask, the same maxima over the solves it marks"""
+    import numpy as np
+    out = {"l2_max": max(e[0] for e in errs),
+           "l2_median": float(np.median([e[0] for e in errs])),
+           "linf_max": max(e[1] for e in errs)}
+    if both is not None:
+        sel = [e for e, m in zip(errs, both) if m]
+        out.update(both_converged=len(sel),
+                   both_converged_l2_max=max([e[0] for e in sel] or [0.0]),
+                   both_converged_linf_max=max([e[1] for e in sel] or [0.0]))
+    return out
 
 
 def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
@@ -194,16 +223,13 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
         res64 = srv64.run(requests(np.zeros(n_req)))
         b64.close()
         out["one_at_a_time_fp64"] = SV.summary(res64)
-        errs = [_density_err(r["prim"], q["prim"]) for r, q in zip(res, res64)]
-        both = [e for e, r, q in zip(errs, res, res64)
-                if r["converged"] and q["converged"]]
-        out["density_err_vs_fp64"] = {
-            "l2_max": max(e[0] for e in errs),
-            "l2_median": float(np.median([e[0] for e in errs])),
-            "linf_max": max(e[1] for e in errs),
-            "both_converged": len(both),
-            "both_converged_l2_max": max([e[0] for e in both] or [0.0]),
-            "both_converged_linf_max": max([e[1] for e in both] or [0.0])}
+        both = [r["converged"] and q["converged"] for r, q in zip(res, res64)]
+        out["density_err_vs_fp64"] = _err_summary(
+            [_density_err(r["prim"], q["prim"]) for r, q in zip(res, res64)],
+            both)
+        out["energy_err_vs_fp64"] = _err_summary(
+            [_energy_err(r["prim"], q["prim"]) for r, q in zip(res, res64)],
+            both)
         out["speedup_solves_per_s"] = (out["served"]["solves_per_s"] /
                                        out["one_at_a_time_fp64"]["solves_per_s"])
     return out
@@ -217,7 +243,6 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
     autotuner's operator; then every instance one at a time on the plain
     FP64 single-mesh path."""
     import concurrent.futures as cf
-    import numpy as np
     from ccfd_amd import batch as BT
     from ccfd_amd import gpu as G
     from ccfd_amd import workloads as W
@@ -256,7 +281,7 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
            "cell_stage_updates_per_s": ncell * 3 * steps / el,
            "solves_per_s": len(ids) / el}
     if baseline:
-        tot, errs = 0.0, []
+        tot, errs, eerrs = 0.0, [], []
         for k, (m, p) in zip(ids, cases):
             s = G.Solver(m, sch64)
             s.set_prim(p)
@@ -265,19 +290,20 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
             s.advance(steps, 0.0, 1e300, 1e300, residual=True)
             s.sync()
             tot += time.perf_counter() - w0
-            errs.append(_density_err(b.prim(k), s.prim()))
+            q32, q64 = b.prim(k), s.prim()
+            errs.append(_density_err(q32, q64))
+            eerrs.append(_energy_err(q32, q64))
             s.close()
         out["one_at_a_time_fp64"] = {"seconds": tot,
                                      "solves_per_s": len(ids) / tot,
                                      "cell_stage_updates_per_s":
                                          ncell * 3 * steps / tot}
         out["speedup_solves_per_s"] = out["solves_per_s"] / (len(ids) / tot)
-        out["density_err_vs_fp64"] = {
-            "l2_max": max(e[0] for e in errs),
-            "l2_median": float(np.median([e[0] for e in errs])),
-            "linf_max": max(e[1] for e in errs),
+        out["density_err_vs_fp64"] = _err_summary(errs)
+        out["density_err_vs_fp64"].update({
             "instances_l2_within_1e-6": int(sum(e[0] <= 1e-6 for e in errs)),
-            "instances": len(errs)}
+            "instances": len(errs)})
+        out["energy_err_vs_fp64"] = _err_summary(eerrs)
     b.close()
     return out
 
@@ -355,9 +381,11 @@ def implicit_run(steps, cfl, ni=512, nj=256, venk_k=1000.0, chunk=500,
         log("implicit %s: %s" % (name, json.dumps(out[name])))
         s.close()
     if len(states) == 2:
-        d = states["fp32_flux"][:, 0] - states["fp64"][:, 0]
-        out["fp32_vs_fp64_density_rel_l2"] = float(
-            np.linalg.norm(d) / np.linalg.norm(states["fp64"][:, 0]))
+        a, b = states["fp32_flux"], states["fp64"]
+        for name, f in (("density", _density_err), ("energy", _energy_err)):
+            l2, linf = f(a, b)
+            out["fp32_vs_fp64_%s_rel_l2" % name] = l2
+            out["fp32_vs_fp64_%s_rel_linf" % name] = linf
     return out
 
 

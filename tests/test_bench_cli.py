@@ -47,3 +47,26 @@ def test_compulsory_bytes_of_the_fused_stage():
     assert bench.fused_min_bytes(True, 1) == 356.0
     assert bench.fused_min_bytes(False, 4) == 180.0
     assert bench.fused_min_bytes(False, 3) == 184.0
+
+
+def test_error_summaries_against_fp64():
+    sys.path.insert(0, ROOT)
+    import bench
+    import numpy as np
+    b = np.array([[1.0, 0.0, 0.0, 1.0], [2.0, 1.0, 0.0, 0.4]])
+    # E = p / 0.4 + rho |v|^2 / 2: 2.5 and 2.0
+    a = np.array([[1.0, 0.0, 0.0, 1.1], [2.0, 1.0, 1.0, 0.4]])
+    # E of a: 2.75 and 3.0; density unchanged
+    l2, linf = bench._energy_err(a, b)
+    assert np.isclose(l2, np.hypot(0.25, 1.0) / np.hypot(2.5, 2.0))
+    assert np.isclose(linf, 1.0 / 2.5)
+    assert bench._density_err(a, b) == (0.0, 0.0)
+    assert bench._energy_err(b, b) == (0.0, 0.0)
+    errs = [(1e-7, 1e-5), (3e-7, 2e-6), (2e-6, 4e-4)]
+    s = bench._err_summary(errs, [True, True, False])
+    assert s["l2_max"] == 2e-6 and s["l2_median"] == 3e-7
+    assert s["linf_max"] == 4e-4
+    assert s["both_converged"] == 2
+    assert s["both_converged_l2_max"] == 3e-7
+    assert s["both_converged_linf_max"] == 1e-5
+    assert "both_converged" not in bench._err_summary(errs)
