@@ -129,3 +129,62 @@ def test_undefined_bc_is_rejected():
     raw = M.cartesian(4, 4, side_bc=(101, 401, 101, 999))
     with pytest.raises(ValueError):
         M.Mesh(raw, [M.BC(101), M.BC(401)])
+
+
+REF_CALC = "/root/reference/calc"
+REF_MESHES = sorted(
+    os.path.relpath(os.path.join(dp, f), REF_CALC)
+    for dp, _, fs in os.walk(REF_CALC) for f in fs if f.endswith(".msh"))
+
+
+def _bc_for(code):
+    t = code // 100
+    if t in (M.INFLOW, M.CHARACTERISTIC):
+        return M.BC(code, rho=1.0, mach=0.5, alpha=0.0, pressure=1.0)
+    if t == M.PRESSURE_OUT:
+        return M.BC(code, pressure=1.0)
+    return M.BC(code)
+
+
+def _shoelace(xy, cells):
+    x, y = xy[cells, 0], xy[cells, 1]
+    return 0.5 * np.abs((x * np.roll(y, -1, axis=1) -
+                         np.roll(x, -1, axis=1) * y).sum(axis=1))
+
+
+@pytest.mark.parametrize("name", REF_MESHES or ["none"])
+def test_reference_calc_meshes(name):
+    """the gmsh meshes the reference ships with its cases (calc/), read and
+    preprocessed: closed cells with outward normals, the least-squares
+    identity, partner sides facing back, every boundary edge one ghost side,
+    and the total area of the cells computed independently"""
+    if not REF_MESHES:
+        pytest.skip("reference cases not present")
+    path = os.path.join(REF_CALC, name)
+    if open(path).read(64).split()[1].startswith("4"):
+        with pytest.raises(OSError, match="format 2"):
+            M.Raw.read_gmsh(path)
+        return
+    raw = M.Raw.read_gmsh(path)
+    codes = sorted(set(raw.bcedge[:, 2].tolist()))
+    m = M.Mesh(raw, [_bc_for(c) for c in codes])
+    assert m.n_elems == len(raw.tri) + len(raw.quad)
+    assert m.n_ghosts == len(raw.bcedge) and (m.nbr < 0).sum() == m.n_ghosts
+    se = _per_slot_elem(m)
+    acc = np.zeros((m.n_elems, 2))
+    np.add.at(acc, se, m.n * m.len[:, None])
+    per = np.zeros(m.n_elems)
+    np.add.at(per, se, m.len)
+    assert (np.abs(acc).max(axis=1) / per).max() < 1e-12
+    assert ((m.n * m.GP).sum(axis=1) > 0).all()
+    assert (m.area > 0).all()
+    total = _shoelace(raw.xy, raw.tri[:, :3]).sum() + \
+        _shoelace(raw.xy, raw.quad[:, :4]).sum()
+    assert abs(m.area.sum() / total - 1.0) < 1e-12
+    W = np.zeros((m.n_elems, 2, 2))
+    np.add.at(W, se, m.w[:, :, None] * m.bbv[:, None, :])
+    assert np.abs(W - np.eye(2)).max() < 1e-8
+    inner = m.nbr >= 0
+    t = m.partner[inner]
+    assert (m.partner[t] == np.nonzero(inner)[0]).all()
+    assert np.allclose(m.n[inner], -m.n[t], rtol=0, atol=1e-14)
