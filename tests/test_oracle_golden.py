@@ -2,7 +2,8 @@
 by tests/golden/make_golden.py from the reference's compiled sources).
 Bit-for-bit: every flux function on 1010 state pairs, and every
 fvTimeDerivative output (dU/dt, limited gradients, side states, side
-fluxes) on 14 meshes / settings."""
+fluxes) on 14 meshes / settings; where the reference is present, also live
+against its compiled fvTimeDerivative on the gmsh meshes of its own cases."""
 import os
 
 import numpy as np
@@ -86,3 +87,72 @@ def test_reference_harness_keeps_its_own_fvtimederivative():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0 and "REF-OK" in r.stdout, r.stdout[-500:]
+
+
+REF_CALC = "/root/reference/calc"
+
+
+def _calc_meshes():
+    out = []
+    for dp, _, fs in os.walk(REF_CALC):
+        for f in fs:
+            p = os.path.join(dp, f)
+            if f.endswith(".msh") and open(p).read(64).split()[1][0] == "2":
+                out.append(os.path.relpath(p, REF_CALC))
+    return sorted(out)
+
+
+CALC_MESHES = _calc_meshes() if os.path.isdir(REF_CALC) else []
+
+
+@pytest.mark.parametrize("name", CALC_MESHES or ["none"])
+def test_oracle_matches_reference_on_its_case_meshes(name):
+    """live, on the gmsh meshes of the reference's own cases (curved walls,
+    stretched quads, unstructured triangles) with their boundary codes:
+    every fvTimeDerivative output bit for bit, HLLC, 2nd order VK, Euler or
+    (where a mesh has no-slip walls) Navier-Stokes"""
+    from ccfd_amd import mesh as M
+    from tests import cases
+    if not CALC_MESHES or not B.ref_available(True):
+        pytest.skip("reference cases or harness not present")
+    raw = M.Raw.read_gmsh(os.path.join(REF_CALC, name))
+    codes = sorted(set(raw.bcedge[:, 2].tolist()))
+    ns = any(c // 100 == M.WALL for c in codes)
+    bcs, items = [], [("nBC", len(codes))]
+    for c in codes:
+        t = c // 100
+        items.append(("BCtype", c))
+        if t in (M.INFLOW, M.CHARACTERISTIC):
+            kw = dict(rho=1.05, mach=0.45, alpha=3.0, pressure=0.95)
+            items += [(k, v) for k, v in kw.items()]
+            bcs.append(M.BC(c, **kw))
+        elif t == M.PRESSURE_OUT:
+            items.append(("pressure", 0.9))
+            bcs.append(M.BC(c, pressure=0.9))
+        else:
+            if t == M.WALL:
+                items.append(("adiabaticWall", "T"))
+            bcs.append(M.BC(c))
+    mesh = M.Mesh(raw, bcs)
+    prim = cases.random_state(mesh.n_elems, seed=len(name), mach=0.5)
+    mu = 2e-3 if ns else 0.0
+    ini = "/tmp/ccfd_calc_%d.ini" % os.getpid()
+    head = [("mu", mu), ("Pr", 0.72)] if ns else []
+    B.write_ini(ini, head + [("fluxFunction", 5)] + items +
+                [("spatialOrder", 2), ("limiter", 2), ("venk_k", 5.0)])
+    try:
+        ref = B.Ref(ns=ns)
+        ref.setup(ini, mesh)
+        ref.set_prim(prim)
+        r = ref.fvtd(0.0)
+    finally:
+        os.unlink(ini)
+        import ctypes
+        ctypes.CDLL(None).fflush(None)   # the reference's echo, captured
+    cfg = B.make_cfg(flux=5, order=2, limiter=2, venk_k=5.0,
+                     navierstokes=int(ns), mu=mu)
+    orc = B.Oracle(mesh, cfg)
+    orc.set_prim(prim)
+    o = orc.fvtd(0.0, full=True)
+    for k in ("u_t", "u_x", "u_y", "side", "flux"):
+        assert np.array_equal(o[k], r[k]), (name, k, np.abs(o[k] - r[k]).max())
