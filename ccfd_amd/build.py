@@ -119,6 +119,10 @@ def build_test_native():
         _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + CSRC,
               "-o", os.path.join(d, "libagglomerate_host.so"), src,
               os.path.join(CSRC, "agglomerate.cpp")])
+    src = os.path.join(d, "range_alloc_host.cpp")
+    if os.path.exists(src):
+        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + CSRC,
+              "-o", os.path.join(d, "librange_alloc_host.so"), src])
 
 
 def build_oracle():
