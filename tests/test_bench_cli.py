@@ -70,3 +70,70 @@ def test_error_summaries_against_fp64():
     assert s["both_converged_l2_max"] == 3e-7
     assert s["both_converged_linf_max"] == 1e-5
     assert "both_converged" not in bench._err_summary(errs)
+
+
+def test_served_sample_with_a_stand_in_batch(monkeypatch):
+    """the served workload's bookkeeping on the host: requests through the
+    real serving loop and workload generator, with a stand-in batch whose
+    FP32-flux instances return states 1e-7 off; the summary carries both
+    throughputs, the latency percentiles and the density and energy errors
+    of the FP32 results against FP64"""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    import bench
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+
+    class StandIn:
+        def __init__(self, scheme, max_sides, local_dt, hbm_bytes):
+            self.fp32 = scheme["precision"] == G.FP32_FLUX
+            self.cap = hbm_bytes // 2000
+            self.inst, self.next = {}, 0
+
+        def fits(self, mesh):
+            return sum(len(p) for p, _ in self.inst.values()) + \
+                mesh.n_elems <= self.cap
+
+        def add(self, mesh, prim, cfl=0.9):
+            k, self.next = self.next, self.next + 1
+            self.inst[k] = [np.array(prim, dtype=float), 0]
+            return k
+
+        def remove(self, k):
+            del self.inst[k]
+
+        def step(self, n):
+            for i in self.inst.values():
+                i[1] += n
+
+        def poll(self):
+            return {k: {"steps": s, "res0": np.ones(4), "nonfinite": False,
+                        "res": np.full(4, np.exp(-s / 60.0))}
+                    for k, (p, s) in self.inst.items()}
+
+        def prim(self, k):
+            p = self.inst[k][0]
+            return p * (1.0 + 1e-7) if self.fp32 else p.copy()
+
+        def cons(self, k):
+            p = self.prim(k)
+            c = p.copy()
+            c[:, 1:3] *= p[:, :1]
+            c[:, 3] = p[:, 3] / 0.4 + 0.5 * p[:, 0] * (p[:, 1] ** 2 +
+                                                       p[:, 2] ** 2)
+            return c
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(BT, "Batch", StandIn)
+    monkeypatch.setattr(G, "params", lambda **kw: dict(kw))
+    out = bench.served_run(6, (1500, 2500), 200.0, 3000, 1)
+    assert out["served"]["solves"] == 6
+    assert out["one_at_a_time_fp64"]["solves"] == 6
+    assert out["served"]["latency_p99_s"] >= out["served"]["latency_p50_s"]
+    assert out["speedup_solves_per_s"] > 0
+    for key in ("density_err_vs_fp64", "energy_err_vs_fp64"):
+        e = out[key]
+        assert 0 < e["l2_max"] < 1e-6 and 0 < e["linf_max"] < 1e-6, (key, e)
+        assert e["both_converged"] == 6
