@@ -188,3 +188,60 @@ def test_reference_calc_meshes(name):
     t = m.partner[inner]
     assert (m.partner[t] == np.nonzero(inner)[0]).all()
     assert np.allclose(m.n[inner], -m.n[t], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("bad,match", [
+    ("node", "references node"), ("negative", "references node"),
+    ("degenerate", "unmatched side"), ("bc_node", "missing node"),
+    ("bc_twice", "open mesh")])
+def test_malformed_input_is_rejected(bad, match):
+    """bad connectivity fails with a message instead of reading out of
+    bounds"""
+    raw = M.cartesian(4, 4)
+    raw.quad, raw.bcedge = raw.quad.copy(), raw.bcedge.copy()
+    if bad == "node":
+        raw.quad[0, 0] = 10 ** 6
+    elif bad == "negative":
+        raw.quad[0, 1] = -5
+    elif bad == "degenerate":
+        raw.quad[0, 1] = raw.quad[0, 0]
+    elif bad == "bc_node":
+        raw.bcedge[0, 0] = 10 ** 6
+    else:
+        raw.bcedge = np.vstack([raw.bcedge, raw.bcedge[:1]])
+    with pytest.raises(ValueError, match=match):
+        M.Mesh(raw, [M.BC(101), M.BC(401)])
+
+
+def test_element_orientation_does_not_matter():
+    """cells listed clockwise: the reference takes |cross products| for the
+    area and turns every normal outward (src/mesh.c:250-341), so the same
+    areas and outward normals come out; and where the reference is built,
+    its fvTimeDerivative and the oracle's agree bit for bit on such a mesh"""
+    raw, bcs = M.jittered_delaunay(10, 0.3, 4, periodic=False)
+    m0 = M.Mesh(raw, bcs)
+    raw.tri = raw.tri.copy()
+    raw.tri[::2, :3] = raw.tri[::2, 2::-1]
+    m1 = M.Mesh(raw, bcs)
+    assert np.allclose(m1.area, m0.area, rtol=1e-14, atol=0)
+    assert ((m1.n * m1.GP).sum(axis=1) > 0).all()
+    se = _per_slot_elem(m1)
+    acc = np.zeros((m1.n_elems, 2))
+    np.add.at(acc, se, m1.n * m1.len[:, None])
+    assert np.abs(acc).max() < 1e-13
+    from oracle import bindings as B
+    if not B.ref_available():
+        return
+    prim = cases.random_state(m1.n_elems, seed=3, mach=0.5)
+    ini = os.path.join(tempfile.mkdtemp(), "flip.ini")
+    B.write_ini(ini, [("fluxFunction", 2), ("nBC", 1), ("BCtype", 401),
+                      ("spatialOrder", 2), ("limiter", 1), ("venk_k", 1.0)])
+    ref = B.Ref()
+    ref.setup(ini, m1)
+    ref.set_prim(prim)
+    r = ref.fvtd(0.0)
+    import ctypes
+    ctypes.CDLL(None).fflush(None)
+    o = B.Oracle(m1, B.make_cfg(flux=2, order=2, limiter=1))
+    o.set_prim(prim)
+    assert np.array_equal(o.fvtd(0.0), r["u_t"])
