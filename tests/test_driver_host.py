@@ -1,0 +1,89 @@
+"""The drop-in driver's set-up from a parameter file (ccfd_init,
+ccfd_amd/csrc/driver.c; the reference's init chain, src/main.c:53-121), up
+to the point where it needs a device: every committed case file reads, and
+its mesh, boundary conditions and initial state build; removing any key
+fails with "missing ini key <key>" exactly when the reference reads that key
+without a default (a NULL proposal to readInTools' getters, found in the
+reference's sources), and otherwise falls back to the default. Host only."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "ccfd_amd", "lib", "libccfd_host.so")
+FILES = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "**", "*.ini"),
+                         recursive=True))
+REF_SRC = "/root/reference/src"
+
+
+def _lib():
+    if not os.path.exists(HOST):
+        pytest.skip("libccfd_host.so not built")
+    L = C.CDLL(HOST)
+    L.ccfd_init.argtypes = [C.c_char_p, C.c_int, C.c_char_p]
+    L.ccfd_init.restype = C.c_int
+    return L
+
+
+def _init(L, path):
+    """ccfd_init's message; an initialised case (a GPU is present) is
+    finalised again"""
+    err = C.create_string_buffer(256)
+    rc = L.ccfd_init(path.encode(), 0, err)
+    if rc == 0:
+        L.ccfd_finalize()
+        return ""
+    return err.value.decode()
+
+
+def _device_only(msg):
+    """set-up got as far as the device"""
+    return msg == "" or "device" in msg.lower() or "hip" in msg.lower()
+
+
+def _key(line):
+    head = re.split(r"[!#=]", line, maxsplit=1)[0]
+    return "".join(ch for ch in head if ch.isalnum() or ch == "_").lower()
+
+
+def _reference_mandatory():
+    src = "".join(open(f).read() for f in glob.glob(os.path.join(REF_SRC, "*.c")))
+    pat = (r'get(?:Int|Dbl|Str|Bool|IntArray|DblArray)\(\s*"(\w+)"\s*,'
+           r'\s*(?:[^,()"]+,\s*)?NULL\s*\)')
+    return {k.lower() for k in re.findall(pat, src)}
+
+
+def test_every_case_file_sets_up(tmp_path, monkeypatch):
+    L = _lib()
+    monkeypatch.chdir(tmp_path)
+    assert len(FILES) >= 15
+    for path in FILES:
+        msg = _init(L, path)
+        assert _device_only(msg), (os.path.basename(path), msg)
+
+
+def test_missing_keys_fail_as_in_the_reference(tmp_path, monkeypatch):
+    L = _lib()
+    if not os.path.isdir(REF_SRC):
+        pytest.skip("reference sources not present")
+    mand = _reference_mandatory()
+    assert {"fluxfunction", "nbc", "tend", "nbcsegments"} <= mand
+    monkeypatch.chdir(tmp_path)
+    checked = 0
+    for path in FILES:
+        lines = open(path).read().splitlines()
+        keys = list(dict.fromkeys(k for k in map(_key, lines) if k))
+        for k in keys:
+            cut = tmp_path / "cut.ini"
+            cut.write_text("\n".join(x for x in lines if _key(x) != k) + "\n")
+            msg = _init(L, str(cut))
+            name = os.path.basename(path)
+            if k in mand:
+                assert msg.lower() == "missing ini key " + k, (name, k, msg)
+            else:
+                assert _device_only(msg), (name, k, msg)
+            checked += 1
+    assert checked > 200
