@@ -87,3 +87,27 @@ def test_missing_keys_fail_as_in_the_reference(tmp_path, monkeypatch):
                 assert _device_only(msg), (name, k, msg)
             checked += 1
     assert checked > 200
+
+
+def test_command_line_errors(tmp_path):
+    """`ccfd case.ini` (src/main.c:35-130): wrong usage, an unreadable file
+    and a missing mandatory key end with status 1 and a message, before any
+    device is touched"""
+    import subprocess
+    exe = os.path.join(ROOT, "ccfd_amd", "lib", "ccfd")
+    if not os.path.exists(exe):
+        pytest.skip("ccfd not built")
+
+    def run(*args):
+        return subprocess.run([exe, *args], capture_output=True, text=True,
+                              timeout=60, cwd=tmp_path)
+    r = run()
+    assert r.returncode == 1 and "usage" in r.stdout
+    r = run(str(tmp_path / "absent.ini"))
+    assert r.returncode == 1 and "could not open parameter file" in r.stdout
+    src = [x for x in open(FILES[0]).read().splitlines()
+           if _key(x) != "fluxfunction"]
+    (tmp_path / "cut.ini").write_text("\n".join(src) + "\n")
+    r = run("cut.ini")
+    assert r.returncode == 1
+    assert "missing ini key fluxFunction" in r.stdout + r.stderr
