@@ -107,7 +107,9 @@ def build_gpu(jobs=None):
 
 
 def build_test_native():
-    """host build of the device physics for the CPU tests (test-only)"""
+    """host builds for the CPU tests (test-only). -Bsymbolic: each harness
+    calls the copy of the code compiled into it, not the same-named symbol
+    of libccfd_gpu.so when that is loaded in the same process."""
     d = os.path.join(ROOT, "tests", "native")
     src = os.path.join(d, "physics_host.cpp")
     if os.path.exists(src):
@@ -117,8 +119,15 @@ def build_test_native():
     src = os.path.join(d, "agglomerate_host.cpp")
     if os.path.exists(src):
         _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + CSRC,
-              "-o", os.path.join(d, "libagglomerate_host.so"), src,
+              "-Wl,-Bsymbolic", "-o",
+              os.path.join(d, "libagglomerate_host.so"), src,
               os.path.join(CSRC, "agglomerate.cpp")])
+    src = os.path.join(d, "soa_host.cpp")
+    if os.path.exists(src):
+        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + CSRC,
+              "-I" + INC, "-Wl,-Bsymbolic", "-o",
+              os.path.join(d, "libsoa_host.so"), src,
+              os.path.join(CSRC, "soa.cpp")])
     src = os.path.join(d, "range_alloc_host.cpp")
     if os.path.exists(src):
         _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + CSRC,
