@@ -3,8 +3,11 @@ AddressSanitizer, UndefinedBehaviorSanitizer and leak detection (gcc, host
 code only), driven by tests/native/host_sanitize.c: every committed case
 file through ccfd_init (which stops where it would create the device
 context, freeing what it built), a missing file, and the gmsh meshes of the
-reference's own cases read, built, refused when opened up, and freed. Any
-sanitizer report fails the test. Host only."""
+reference's own cases read, built, refused when opened up, and freed; and
+the host C++ of the device library (tests/native/host_sanitize_cxx.cpp):
+device layout, halo lists and multigrid levels of those meshes, and the
+batched arena's allocator through random sequences. Any sanitizer report
+fails the test. Host only."""
 import glob
 import os
 import shutil
@@ -17,40 +20,79 @@ CSRC = os.path.join(ROOT, "ccfd_amd", "csrc")
 LIB = os.path.join(ROOT, "ccfd_amd", "lib")
 
 
-def test_host_library_under_sanitizers(tmp_path):
+SAN = ["-g", "-O1", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+       "-fno-sanitize-recover=undefined"]
+LINK = ["-L" + LIB, "-lccfd_gpu", "-Wl,-rpath," + LIB,
+        "-Wl,-rpath-link,/opt/rocm/lib", "-lm"]
+ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1",
+           UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+ENV.pop("LD_PRELOAD", None)
+MESHES = sorted(glob.glob("/root/reference/calc/**/*.msh", recursive=True))
+
+
+def _host_library(tmp_path):
+    """libccfd_host.so with the sanitizers, in tmp_path"""
     cc = shutil.which("gcc")
     if not cc or not os.path.exists(os.path.join(LIB, "libccfd_gpu.so")):
         pytest.skip("gcc or libccfd_gpu.so missing")
-    san = ["-g", "-O1", "-fno-omit-frame-pointer",
-           "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
-    host = tmp_path / "libccfd_host.so"
     srcs = [os.path.join(CSRC, s) for s in ("mesh.c", "ini.c", "driver.c")]
-    link = ["-L" + LIB, "-lccfd_gpu", "-Wl,-rpath," + LIB,
-            "-Wl,-rpath-link,/opt/rocm/lib", "-lm"]
     r = subprocess.run([cc, "-std=c99", "-D_POSIX_C_SOURCE=200809L",
                         "-ffp-contract=off", "-fPIC", "-shared",
-                        "-I" + os.path.join(ROOT, "include"), "-o", str(host)]
-                       + san + srcs + link, capture_output=True, text=True,
+                        "-I" + os.path.join(ROOT, "include"), "-o",
+                        str(tmp_path / "libccfd_host.so")]
+                       + SAN + srcs + LINK, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr
+    return cc
+
+
+def _clean(r):
+    out = r.stdout + r.stderr
+    assert r.returncode == 0, out[-3000:]
+    assert "Sanitizer" not in out and "runtime error" not in out, out[-3000:]
+    return out
+
+
+def test_host_library_under_sanitizers(tmp_path):
+    cc = _host_library(tmp_path)
     exe = tmp_path / "host_sanitize"
     r = subprocess.run([cc, "-std=c99", "-I" + os.path.join(ROOT, "include"),
                         os.path.join(ROOT, "tests", "native", "host_sanitize.c"),
                         "-L" + str(tmp_path), "-lccfd_host",
                         "-Wl,-rpath," + str(tmp_path), "-o", str(exe)]
-                       + san + link, capture_output=True, text=True,
+                       + SAN + LINK, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr
     inis = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "*", "*.ini")))
-    meshes = sorted(glob.glob("/root/reference/calc/**/*.msh", recursive=True))
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1",
-               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
-    env.pop("LD_PRELOAD", None)
-    r = subprocess.run([str(exe)] + inis + ["--"] + meshes,
+    r = subprocess.run([str(exe)] + inis + ["--"] + MESHES,
                        capture_output=True, text=True, timeout=600,
-                       env=env, cwd=tmp_path)
-    out = r.stdout + r.stderr
-    assert r.returncode == 0, out[-3000:]
-    assert "Sanitizer" not in out and "runtime error" not in out, out[-3000:]
+                       env=ENV, cwd=tmp_path)
+    out = _clean(r)
     assert out.count("init ") == len(inis) and "ACCEPTED" not in out
     assert "missing file: -1" in out
+
+
+def test_device_library_host_code_under_sanitizers(tmp_path):
+    _host_library(tmp_path)
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("g++ missing")
+    exe = tmp_path / "host_sanitize_cxx"
+    r = subprocess.run([cxx, "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + CSRC,
+                        os.path.join(ROOT, "tests", "native",
+                                     "host_sanitize_cxx.cpp"),
+                        os.path.join(CSRC, "soa.cpp"),
+                        os.path.join(CSRC, "agglomerate.cpp"),
+                        "-L" + str(tmp_path), "-lccfd_host",
+                        "-Wl,-rpath," + str(tmp_path), "-o", str(exe)]
+                       + SAN + LINK, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)] + MESHES, capture_output=True, text=True,
+                       timeout=600, env=ENV, cwd=tmp_path)
+    out = _clean(r)
+    assert out.count("allocator ") == 3
+    assert "used 0 high 0 ranges 1" in out
+    assert out.count(" levels") == sum(
+        open(m).read(64).split()[1].startswith("2") for m in MESHES)
