@@ -88,6 +88,9 @@ def oracle_lib():
                                           dp, dp, dp]
         L.oracle_lusgs.restype = C.c_int
         L.oracle_lusgs.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.oracle_gmres.restype = C.c_int
+        L.oracle_gmres.argtypes = [C.c_void_p, C.POINTER(ImplicitCfg),
+                                   C.c_double, C.c_double, dp, dp, ip]
         L.oracle_rk_coeffs.restype = C.c_int
         L.oracle_rk_coeffs.argtypes = [C.c_int, C.c_int, dp]
         L.oracle_exact_func.argtypes = [C.POINTER(OracleCfg), C.c_void_p,
@@ -180,6 +183,18 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle_lusgs failed (%d)" % rc)
         return X
+
+    def gmres(self, ic, t, dt, B):
+        """one GMRES_M solve of (I - dt dR/dU) dX = -B at the current
+        state; returns (dX (N, 4), Krylov iterations)"""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        dX = np.zeros((self.N, 4))
+        n = C.c_int(0)
+        rc = self.L.oracle_gmres(self.p, C.byref(ic), t, dt, _d(B), _d(dX),
+                                 C.byref(n))
+        if rc:
+            raise RuntimeError("oracle_gmres failed (%d)" % rc)
+        return dX, n.value
 
     def residual(self, ut):
         ut = np.ascontiguousarray(ut, dtype=np.float64)

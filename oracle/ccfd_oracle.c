@@ -1618,3 +1618,37 @@ int oracle_lusgs(const oracle_ctx *o, const double *Dinv, const double *Blk,
 	free(Xs);
 	return 0;
 }
+
+/* one GMRES_M solve (src/linearSolver.c:385-510) on its own: the Krylov
+ * solve of (I - dt dR/dU) dX = -B at the current state, as a Newton
+ * iteration makes it (X = the current state, R(X) from one operator call,
+ * the operator at time t + dt, buildMatrix at t). The state is restored
+ * afterwards. */
+int oracle_gmres(oracle_ctx *o, const oracle_implicit_cfg *ic, double t,
+		double dt, const double *B, double *dX, int *nIter)
+{
+	long N = o->N;
+	double *saved[2] = {o->XK, o->RXK};
+	double *xk = malloc(sizeof(double) * 4 * N);
+	double *rxk = malloc(sizeof(double) * 4 * N);
+	double *prim = malloc(sizeof(double) * 4 * N);
+	if (!xk || !rxk || !prim) {
+		free(xk); free(rxk); free(prim);
+		return -3;
+	}
+	memcpy(xk, o->cons, sizeof(double) * 4 * N);
+	memcpy(prim, o->prim, sizeof(double) * 4 * N);
+	oracle_fvtd(o, t + dt, NULL, NULL, NULL, NULL, NULL);
+	memcpy(rxk, o->ut, sizeof(double) * 4 * N);
+	o->XK = xk;
+	o->RXK = rxk;
+	*nIter = 0;
+	int rc = gmres(o, ic, t, t + dt, dt, 1.0, B, sqrt(oracle_dot(N, B, B)),
+			dX, nIter);
+	memcpy(o->cons, xk, sizeof(double) * 4 * N);
+	memcpy(o->prim, prim, sizeof(double) * 4 * N);
+	o->XK = saved[0];
+	o->RXK = saved[1];
+	free(xk); free(rxk); free(prim);
+	return rc;
+}

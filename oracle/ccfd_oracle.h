@@ -99,6 +99,13 @@ int oracle_build_matrix(oracle_ctx *o, double time, double dt, double *Dg,
 int oracle_lusgs(const oracle_ctx *o, const double *Dinv, const double *Blk,
 		const double *B, double *X);
 
+/* one GMRES_M solve (src/linearSolver.c:385-510) of (I - dt dR/dU) dX = -B
+ * at the current state (the operator at t + dt, buildMatrix at t, as a
+ * Newton iteration of implicitTimeStep); the state is left as it was.
+ * nIter gets the Krylov iterations; returns 0, -2 (singular block) or -3. */
+int oracle_gmres(oracle_ctx *o, const oracle_implicit_cfg *ic, double t,
+		double dt, const double *B, double *dX, int *nIter);
+
 /* RK coefficient table (src/timeDiscretization.c:94-178); returns -1 for an
  * unsupported (stages, order) pair */
 int oracle_rk_coeffs(int nRKstages, int timeOrder, double coeff[6]);

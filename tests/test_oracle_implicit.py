@@ -15,7 +15,10 @@ linearSolver.c cannot be compiled here (they include the CGNS headers):
   element and its face neighbours, and the stored inverses invert the
   diagonal blocks;
 * LU-SGS returns x with (D + L) D^-1 (D + U) x = b for the blocks above,
-  the product assembled here with scipy from the blocks by element index.
+  the product assembled here with scipy from the blocks by element index;
+* GMRES_M with a full Krylov space returns the dense solution of
+  (I - dt dR/dU) x = -b, the matrix built here from forward differences of
+  R, with and without LU-SGS.
 
 A wrong sign or factor in the Newton function, the Frechet difference, the
 Jacobian assembly or the preconditioner sweeps fails one of these; a
@@ -225,3 +228,58 @@ def test_lusgs_inverts_the_symmetric_gauss_seidel_product(kind):
     assert rel < 1e-12
     # and the sweeps are not block Jacobi: the off-diagonal blocks matter
     assert np.linalg.norm(D @ x.ravel() - b.ravel()) > 1e-3 * np.linalg.norm(b)
+
+
+def _prim_of(cons):
+    p = np.empty_like(cons)
+    p[:, 0] = cons[:, 0]
+    p[:, 1] = cons[:, 1] / cons[:, 0]
+    p[:, 2] = cons[:, 2] / cons[:, 0]
+    p[:, 3] = (GAM - 1.0) * (cons[:, 3] - 0.5 * (cons[:, 1] ** 2 +
+                                                 cons[:, 2] ** 2) / cons[:, 0])
+    return p
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("precond", [0, 1])
+def test_gmres_solves_the_linear_system(order, precond):
+    """GMRES_M with a Krylov space as large as the system and a tight
+    tolerance returns the solution of (I - dt dR/dU) x = -b, the matrix
+    assembled here column by column from forward differences of R in the
+    conservative variables (separate operator calls, a step of its own):
+    a wrong sign, Givens rotation, back-substitution or right
+    preconditioning (x from Z, not V) fails it"""
+    mesh = cases.delaunay_mesh(8, seed=3, periodic=True)
+    prim = M.smooth_state(mesh.bary, seed=6)
+    cfg = B.make_cfg(flux=5, order=order, limiter=2, venk_k=5.0, cfl=2.0)
+    o = B.Oracle(mesh, cfg)
+    o.set_prim(prim)
+    dt, _, _ = o.calc_dt(0.0, 1e9, 1e9)
+    u0 = o.cons()
+    n = 4 * mesh.n_elems
+    b = np.random.default_rng(order + 2 * precond).normal(size=(mesh.n_elems, 4))
+    ic = B.ImplicitCfg(n, 1, precond, 1e-3, 1e-11, 0.9)
+    x, iters = o.gmres(ic, 0.0, dt, b)
+    assert np.array_equal(o.prim(), prim)            # state left as it was
+    # dense I - dt dR/dU by forward differences at the operator time dt
+    r0 = _rhs_at(mesh, cfg, prim, dt)
+    A = np.eye(n)
+    h = 1e-7
+    for j in range(n):
+        u = u0.copy()
+        u.flat[j] += h * max(1.0, abs(u0.flat[j]))
+        step = u.flat[j] - u0.flat[j]
+        A[:, j] -= dt * ((_rhs_at(mesh, cfg, _prim_of(u), dt) - r0) / step).ravel()
+    want = np.linalg.solve(A, -b.ravel())
+    err = np.linalg.norm(x.ravel() - want) / np.linalg.norm(want)
+    res = np.linalg.norm(A @ x.ravel() + b.ravel()) / np.linalg.norm(b)
+    print("order %d precond %d: %d iterations, |x - x_dense| / |x| %.1e, "
+          "residual %.1e" % (order, precond, iters, err, res))
+    assert err < 1e-5 and res < 1e-5
+    assert iters < n
+
+
+def _rhs_at(mesh, cfg, prim, time):
+    o = B.Oracle(mesh, cfg)
+    o.set_prim(prim)
+    return o.fvtd(time)
