@@ -137,3 +137,33 @@ def test_served_sample_with_a_stand_in_batch(monkeypatch):
         e = out[key]
         assert 0 < e["l2_max"] < 1e-6 and 0 < e["linf_max"] < 1e-6, (key, e)
         assert e["both_converged"] == 6
+
+
+def test_traffic_figures_come_from_the_committed_profiles():
+    """bench.py's roofline.traffic reads profiles/pmc_traffic.json; each
+    entry must be the scaled FETCH_SIZE + WRITE_SIZE bytes per dispatch that
+    the committed rocprofv3 summary lists for that kernel"""
+    import json
+    import re
+    prof = os.path.join(ROOT, "profiles")
+    with open(os.path.join(prof, "pmc_traffic.json")) as f:
+        traffic = json.load(f)
+    rows = {
+        "fused_face_fp64": ("fused_face_kernel_fp64_4M_delaunay.md",
+                            "fused_face_kernel<5, double, 2, 3>"),
+        "fused_face_fp32": ("fused_face_kernel_fp32_4M_delaunay.md",
+                            "fused_face_kernel<5, float, 2, 3>"),
+        "batched_fused_fp32": ("fused_kernel_fp32_batched_64_meshes.md",
+                               "ccfd::fused_kernel<5, float, 1, 3>"),
+    }
+    assert set(traffic) == set(rows)
+    for key, (md, kernel) in rows.items():
+        text = open(os.path.join(prof, md)).read()
+        head = text.index("## HBM traffic")
+        line = [x for x in text[head:].splitlines() if kernel in x][0]
+        cells = [c.strip() for c in line.strip("|").split("|")]
+        fetch, write, scaled = float(cells[1]), float(cells[2]), float(cells[3])
+        scale = float(re.search(r"FETCH is scaled by ([0-9.]+)",
+                                text).group(1))
+        assert abs(traffic[key] / scaled - 1) < 1e-3, key
+        assert abs((fetch * scale + write) * 1024 / traffic[key] - 1) < 1e-3, key
