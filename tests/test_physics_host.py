@@ -1,7 +1,8 @@
 """Host build of the device physics header (ccfd_amd/csrc/physics.hpp,
 compiled by build() into tests/native/libphysics_host.so) against the
-reference's flux vectors, bit for bit. Checks the transcription of the HIP
-flux code without a GPU."""
+reference's flux vectors, bit for bit, and the boundary ghost states against
+what each boundary type fixes. Checks the transcription of the HIP physics
+code without a GPU."""
 import ctypes as C
 import os
 
@@ -47,3 +48,66 @@ def test_fp32_fluxes_close_to_fp64(code):
     # precision loses a few digits only in cancellation-heavy corners
     assert np.median(err) < 1e-6
     assert err.max() < 5e-4
+
+
+def _bc(L, btype, bcp, interior, normals, mu=0.0):
+    out = np.empty_like(interior)
+    bcp = np.ascontiguousarray(bcp, dtype=np.float64)
+    L.phys_bc(btype, bcp.ctypes.data_as(dp), C.c_double(1.4), C.c_double(mu),
+              len(interior), normals.ctypes.data_as(dp),
+              interior.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    return out
+
+
+def test_boundary_states_closed_forms():
+    """the device ghost states (bc_ghost, src/boundary.c:147-294) against
+    what each boundary type fixes: slip walls mirror the normal velocity,
+    no-slip walls reverse it, inflow takes the boundary state, outflow the
+    interior, the pressure outlet sets the boundary pressure below sonic
+    outflow (density scaled by the pressure ratio), and the characteristic
+    boundary reduces to the interior (supersonic outflow), to the boundary
+    state (supersonic inflow) and to a fixed point where both agree"""
+    L = _lib()
+    rng = np.random.default_rng(4)
+    n = 200
+    q = np.column_stack([rng.uniform(0.2, 2, n), rng.uniform(-1, 1, n),
+                         rng.uniform(-1, 1, n), rng.uniform(0.2, 2, n)])
+    th = rng.uniform(0, 2 * np.pi, n)
+    nrm = np.ascontiguousarray(np.column_stack([np.cos(th), np.sin(th)]))
+    vn = (q[:, 1:3] * nrm).sum(axis=1)
+    mirror = q[:, 1:3] - 2 * vn[:, None] * nrm
+
+    g = _bc(L, 1, np.zeros(4), q, nrm)
+    assert np.array_equal(g[:, [0, 3]], q[:, [0, 3]])
+    np.testing.assert_allclose(g[:, 1:3], mirror, rtol=0, atol=1e-15)
+    g = _bc(L, 2, np.zeros(4), q, nrm, mu=1e-3)
+    np.testing.assert_allclose(g[:, 1:3], -q[:, 1:3], rtol=0, atol=1e-15)
+    g = _bc(L, 2, np.zeros(4), q, nrm, mu=0.0)     # inviscid: a slip wall
+    np.testing.assert_allclose(g[:, 1:3], mirror, rtol=0, atol=1e-15)
+
+    state = np.array([1.1, 0.3, -0.2, 0.9])
+    assert np.array_equal(_bc(L, 3, state, q, nrm), np.tile(state, (n, 1)))
+    assert np.array_equal(_bc(L, 4, state, q, nrm), q)
+
+    g = _bc(L, 8, np.array([0.0, 0.0, 0.0, 0.7]), q, nrm)
+    c = np.sqrt(1.4 * q[:, 3] / q[:, 0])
+    sub = vn / c < 1.0
+    assert sub.any() and (~sub).any()       # both branches occur
+    np.testing.assert_allclose(g[~sub], q[~sub], rtol=1e-15, atol=0)
+    np.testing.assert_allclose(g[sub, 3], 0.7, rtol=0, atol=0)
+    np.testing.assert_allclose(g[sub, 0], q[sub, 0] * 0.7 / q[sub, 3],
+                               rtol=1e-15)
+    assert np.array_equal(g[:, 1:3], q[:, 1:3])
+
+    # characteristic boundary: limits of the eigenvalue test
+    for k in range(20):
+        nk = nrm[k:k + 1]
+        cb = np.sqrt(1.4 * state[3] / state[0])
+        for speed, want in ((2.5 * cb, "interior"), (-2.5 * cb, "boundary")):
+            bc = state.copy()
+            bc[1:3] = speed * nk[0] + 0.1 * np.array([-nk[0, 1], nk[0, 0]])
+            g = _bc(L, 5, bc, q[k:k + 1].copy(), nk)
+            ref = q[k] if want == "interior" else bc
+            np.testing.assert_allclose(g[0], ref, rtol=1e-13, atol=1e-13)
+        g = _bc(L, 5, q[k], q[k:k + 1].copy(), nk)
+        np.testing.assert_allclose(g[0], q[k], rtol=1e-13, atol=1e-13)
