@@ -111,3 +111,21 @@ def test_command_line_errors(tmp_path):
     r = run("cut.ini")
     assert r.returncode == 1
     assert "missing ini key fluxFunction" in r.stdout + r.stderr
+
+
+def test_unsupported_settings_are_refused(tmp_path, monkeypatch):
+    """settings outside this path end set-up with a message, not a wrong
+    run: source terms, and a gmsh file of another format"""
+    L = _lib()
+    monkeypatch.chdir(tmp_path)
+    base = open(FILES[0]).read()
+    (tmp_path / "src.ini").write_text(base + "\ncalcSource = T\n")
+    assert _init(L, str(tmp_path / "src.ini")) == \
+        "source terms are not supported"
+    (tmp_path / "m4.msh").write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n")
+    lines = [x for x in base.splitlines()
+             if _key(x) not in ("meshtype", "meshformat", "meshfile")]
+    (tmp_path / "m4.ini").write_text("\n".join(
+        lines + ["meshType = 2", "meshFormat = .msh", "meshFile = m4"]) + "\n")
+    msg = _init(L, str(tmp_path / "m4.ini"))
+    assert "format 2" in msg, msg
