@@ -7,6 +7,19 @@
  *
  * Arrays are in the reference's element / slot order (ccfd_mesh), state as
  * 4 doubles per element (rho, vx, vy, p) or (rho, mx, my, E).
+ *
+ * How each part is pinned (tests/):
+ *   fluxes, fvTimeDerivative   bit for bit against the reference's compiled
+ *                              sources: fixtures and, where the reference is
+ *                              present, live (test_oracle_golden.py)
+ *   calcTimeStep, RK, timeDisc the reference's make check outputs
+ *                              (test_make_check_oracle.py)
+ *   implicit step              backward-Euler solution, explicit-Euler limit,
+ *                              fixed point (test_oracle_implicit.py, as below)
+ *   buildMatrix + calcDinv     forward differences of R; inverses
+ *   LUSGS                      the (D + L) D^-1 (D + U) product
+ *   GMRES_M                    dense solve of the finite-difference system
+ *   vectorDotProduct           correctly rounded sums
  */
 #ifndef CCFD_ORACLE_H
 #define CCFD_ORACLE_H

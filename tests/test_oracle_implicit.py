@@ -283,3 +283,18 @@ def _rhs_at(mesh, cfg, prim, time):
     o = B.Oracle(mesh, cfg)
     o.set_prim(prim)
     return o.fvtd(time)
+
+
+def test_dot_is_the_sum_of_products():
+    """vectorDotProduct (src/linearSolver.c:115-128): within a few ulps of
+    the correctly rounded sum (math.fsum) of the elementwise products, on
+    vectors whose terms cancel"""
+    import math
+    rng = np.random.default_rng(12)
+    for n in (1, 7, 1000, 40000):
+        a = rng.normal(size=(n, 4)) * np.exp(rng.uniform(-5, 5, (n, 1)))
+        b = rng.normal(size=(n, 4))
+        exact = math.fsum((a * b).ravel().tolist())
+        scale = math.fsum(np.abs(a * b).ravel().tolist())
+        assert abs(B.dot(a, b) - exact) <= 1e-15 * n * scale
+    assert B.dot(np.zeros((3, 4)), np.ones((3, 4))) == 0.0
