@@ -200,3 +200,22 @@ def test_awkward_lines(tmp_path):
         assert pair.array("absent", 2, b"1.5,2.5") == ([1.5, 2.5], [1.5, 2.5])
     finally:
         pair.close()
+
+
+def test_oracle_ini_reads_alike(tmp_path):
+    """the oracle's Python restatement of the reader (oracle/timedisc.py,
+    used by its make check runs) takes the same keys and values, in the
+    same positional order, as the reference's readInTools"""
+    from oracle import timedisc as TD
+    ref, host = _libs()
+    for path in FILES + [_write_awkward(tmp_path)]:
+        pair = _Pair(ref, host, path)
+        ini = TD.Ini(path)
+        try:
+            for key in _keys(path):
+                want, _ = pair.string(key)
+                assert ini.str(key).encode() == want, (path, key)
+            assert ini.cmds == []
+            assert ini.str("absent", "dflt") == "dflt"
+        finally:
+            pair.close()
