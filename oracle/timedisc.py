@@ -11,6 +11,10 @@ TEST INFRASTRUCTURE ONLY (tests/ and the CPU baseline use it).
                  gmsh meshes, icType 1 and 2
   run_case       timeDisc (src/timeDiscretization.c:559-777) with CSV output
                  (csvOutput, src/output.c:56-113)
+
+Pinned by the reference's own make check outputs (tests/test_make_check_oracle.py)
+and, for read_ini, by the reference's compiled readInTools on every case file
+(tests/test_ini_host.py).
 """
 import math
 import os
