@@ -3,7 +3,7 @@ timeDisc loop restated in Python over the C oracle.
 
 TEST INFRASTRUCTURE ONLY (tests/ and the CPU baseline use it).
 
-  read_ini       readInTools semantics (src/readInTools.c:38-405): keys
+  Ini            readInTools semantics (src/readInTools.c:38-405): keys
                  lower-cased, values keep [A-Za-z0-9_.,-], repeated keys are
                  consumed in order
   setup_case     initEquation / initBoundary / initMesh / initInitialCondition
@@ -13,7 +13,7 @@ TEST INFRASTRUCTURE ONLY (tests/ and the CPU baseline use it).
                  (csvOutput, src/output.c:56-113)
 
 Pinned by the reference's own make check outputs (tests/test_make_check_oracle.py)
-and, for read_ini, by the reference's compiled readInTools on every case file
+and, for Ini, by the reference's compiled readInTools on every case file
 (tests/test_ini_host.py).
 """
 import math
