@@ -1,7 +1,7 @@
 """Full runs of the oracle: the reference's ini reading, case setup and
 timeDisc loop restated in Python over the C oracle.
 
-TEST INFRASTRUCTURE ONLY (tests/ and the CPU baseline use it).
+TEST INFRASTRUCTURE ONLY (only tests/ use it).
 
   Ini            readInTools semantics (src/readInTools.c:38-405): keys
                  lower-cased, values keep [A-Za-z0-9_.,-], repeated keys are
