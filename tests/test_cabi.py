@@ -1,7 +1,9 @@
-"""The C ABI libraries load and export every function include/*.h declares,
-and every entry point of the HIP library rejects a null handle or bad
-arguments with CCFD_EINVAL and a message before it touches a device (no
-device calls: runs without a GPU)."""
+"""The C ABI libraries load and export every function include/*.h declares;
+every entry point of the HIP library rejects a null handle or bad arguments
+with CCFD_EINVAL and a message before it touches a device; and without a
+device the product path raises instead of computing on the CPU. Runs
+without a GPU (no compute calls; the last test skips where a device is
+present)."""
 import ctypes as C
 import os
 import re
@@ -102,3 +104,26 @@ def test_context_free_entry_points_check_arguments_first():
     assert L.ccfd_gpu_riemann_batch(5, 0, 1.4, 0, None, None, None) == 0
     L.ccfd_gpu_copy_probe.argtypes = [C.c_int, C.c_size_t, C.c_int, C.c_void_p]
     assert L.ccfd_gpu_copy_probe(0, 1024, 1, None) == -1
+
+
+def test_no_cpu_fallback_without_a_device():
+    """with no HIP device the product path raises the library's message: a
+    context, a batch and a flux batch all fail, none computes on the CPU
+    (skipped where a device is present)"""
+    import numpy as np
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+    from tests import cases
+    L = C.CDLL(os.path.join(LIB, "libccfd_gpu.so"))
+    h = C.c_void_p()
+    if L.ccfd_gpu_create(C.byref(h), 0) == 0:
+        L.ccfd_gpu_destroy(h)
+        pytest.skip("a HIP device is present")
+    mesh = cases.sod_mesh(20)
+    with pytest.raises(G.GpuError, match="device"):
+        G.Solver(mesh, G.params())
+    with pytest.raises(Exception, match="device"):
+        BT.Batch(G.params(), max_sides=4)
+    state = np.tile([1.0, 0.1, 0.0, 1.0], (4, 1))
+    with pytest.raises(G.GpuError):
+        G.riemann_batch(5, state, state)
