@@ -34,3 +34,49 @@ def test_oracle_reproduces_make_check(ini):
 
 def test_fourteen_active_cases():
     assert len(INIS) == 14
+
+
+def test_time_step_closed_forms():
+    """calcTimeStep (src/timeDiscretization.c:249-329) on a uniform
+    Cartesian grid, where it reduces to the textbook limits: convective
+    CFL / ((|u| + c) / dx + (|v| + c) / dy), viscous
+    DFL rho / (4 max(4/3, gamma/Pr) (mu/rho) (1/dx^2 + 1/dy^2)) with the
+    flag set when it governs, the 1-D form CFL dx / (|u| + c), and the
+    clamp to the next output time"""
+    import numpy as np
+    from ccfd_amd import mesh as M
+    from oracle import bindings as B
+    ni, nj = 40, 5
+    dx, dy = 1.0 / ni, 0.01 / nj
+    raw = M.cartesian(ni, nj, (0.0, 0.0), (1.0, 0.01), (101, 401, 101, 401))
+    mesh = M.Mesh(raw, [M.BC(101), M.BC(401)])
+    rho, u, v, p, gam = 1.3, 0.4, -0.25, 0.9, 1.4
+    prim = np.tile([rho, u, v, p], (mesh.n_elems, 1))
+    c = (gam * p / rho) ** 0.5
+    conv = 0.9 / ((abs(u) + c) / dx + (abs(v) + c) / dy)
+
+    def dt_of(**kw):
+        o = B.Oracle(mesh, B.make_cfg(**kw))
+        o.set_prim(prim)
+        return o, o.calc_dt(0.0, 1e9, 1e9)
+
+    _, (dt, visc, nf) = dt_of(cfl=0.9)
+    assert abs(dt / conv - 1) < 1e-13 and not visc and not nf
+    flags = set()
+    for mu, Pr in ((1e-7, 0.72), (1e-4, 0.72), (1e-3, 0.72), (1e-3, 2.0)):
+        vdt = 0.7 * rho / (4 * max(4 / 3, gam / Pr) * (mu / rho) *
+                           (1 / dx ** 2 + 1 / dy ** 2))
+        _, (dt, visc, _) = dt_of(cfl=0.9, dfl=0.7, mu=mu, Pr=Pr,
+                                 navierstokes=1)
+        assert abs(dt / min(conv, vdt) - 1) < 1e-13, (mu, Pr)
+        assert visc == (vdt < conv)
+        flags.add(visc)
+    assert flags == {True, False}      # both limits govern somewhere
+    _, (dt, _, _) = dt_of(cfl=0.9, ts1d=1)
+    assert abs(dt / (0.9 * dx / (abs(u) + c)) - 1) < 1e-13
+    # output clamp: a step that would pass the output time ends on it, one
+    # that would come within half a step of it takes half the remainder
+    o, _ = dt_of(cfl=0.9)
+    assert o.calc_dt(0.0, 0.5 * conv, 1e9)[0] == 0.5 * conv
+    assert abs(o.calc_dt(0.0, 1.2 * conv, 1e9)[0] - 0.6 * conv) < 1e-18
+    assert o.calc_dt(0.0, 1e9, 0.25 * conv)[0] == 0.25 * conv
