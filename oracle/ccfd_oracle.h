@@ -13,8 +13,9 @@
  *                              sources: fixtures and, where the reference is
  *                              present, live (test_oracle_golden.py)
  *   calcTimeStep, RK, timeDisc the reference's make check outputs; the
- *                              time step's closed forms on a uniform grid
- *                              (test_make_check_oracle.py)
+ *                              time step's closed forms on a uniform grid;
+ *                              every RK table against the reference's source
+ *                              text (test_make_check_oracle.py)
  *   implicit step              backward-Euler solution, explicit-Euler limit,
  *                              fixed point (test_oracle_implicit.py, as below)
  *   buildMatrix + calcDinv     forward differences of R; inverses

@@ -80,3 +80,41 @@ def test_time_step_closed_forms():
     assert o.calc_dt(0.0, 0.5 * conv, 1e9)[0] == 0.5 * conv
     assert abs(o.calc_dt(0.0, 1.2 * conv, 1e9)[0] - 0.6 * conv) < 1e-18
     assert o.calc_dt(0.0, 1e9, 0.25 * conv)[0] == 0.25 * conv
+
+
+def test_rk_tables_are_the_reference_tables():
+    """every (stages, order) coefficient table of the oracle equals the one
+    in the reference's source (src/timeDiscretization.c:94-178), read as
+    text: make check runs only the order-2 tables and forward Euler"""
+    import re
+    from oracle import bindings as B
+    src = "/root/reference/src/timeDiscretization.c"
+    if not os.path.exists(src):
+        pytest.skip("reference sources not present")
+    lines = open(src).read().splitlines()
+    start = next(i for i, x in enumerate(lines) if "nRKstages == 1" in x)
+    tables, stages, order = {(1, 1): {1: 1.0}}, None, None
+    for i in range(start, len(lines)):
+        x = lines[i]
+        if "default:" in x:
+            break
+        m = re.search(r"case (\d+):", x)
+        if m:
+            if "switch (timeOrder)" in lines[i + 1]:
+                stages = int(m.group(1))
+            else:
+                order = int(m.group(1))
+            continue
+        m = re.search(r"RKcoeff\[(\d+)\] = ([0-9.]+);", x)
+        if m and stages:
+            tables.setdefault((stages, order), {})[int(m.group(1))] = \
+                float(m.group(2))
+    assert len(tables) == 10
+    for (s, o), coeff in tables.items():
+        got = B.rk_coeffs(s, o)
+        assert got is not None, (s, o)
+        assert sorted(coeff) == list(range(1, s + 1))
+        for k, v in coeff.items():
+            assert got[k] == v, (s, o, k)
+        assert got[0] == 0.0
+    assert B.rk_coeffs(2, 2) is None
