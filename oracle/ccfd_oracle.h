@@ -22,6 +22,8 @@
  *   LUSGS                      the (D + L) D^-1 (D + U) product
  *   GMRES_M                    dense solve of the finite-difference system
  *   vectorDotProduct           correctly rounded sums
+ *   exact Riemann solution     Toro's published Sod solution, isentropic
+ *                              fan (test_oracle_golden.py)
  */
 #ifndef CCFD_ORACLE_H
 #define CCFD_ORACLE_H

@@ -156,3 +156,46 @@ def test_oracle_matches_reference_on_its_case_meshes(name):
     o = orc.fvtd(0.0, full=True)
     for k in ("u_t", "u_x", "u_y", "side", "flux"):
         assert np.array_equal(o[k], r[k]), (name, k, np.abs(o[k] - r[k]).max())
+
+
+def test_exact_riemann_solution_matches_the_textbook():
+    """the oracle's exact Riemann solver (src/exactRiemann.c:59-214, used by
+    Godunov's flux and exact-function states) on Sod's problem against
+    the published solution (Toro, Riemann Solvers and Numerical Methods for
+    Fluid Dynamics, test 1: p* = 0.30313, u* = 0.92745, rho*L = 0.42632,
+    rho*R = 0.26557), sampled at t = 0.2 in every region"""
+    import ctypes as C
+    from tests import cases
+    mesh = cases.sod_mesh(10)
+    cfg = B.make_cfg(left=(1.0, 0.0, 0.0, 1.0), right=(0.125, 0.0, 0.0, 0.1),
+                     x0=0.5)
+    L = B.oracle_lib()
+
+    def at(x, t=0.2):
+        pv = np.zeros(4)
+        L.oracle_exact_func(C.byref(cfg), C.cast(mesh.ptr, C.c_void_p), 5,
+                            np.array([x, 0.005]).ctypes.data_as(B.dp), t,
+                            pv.ctypes.data_as(B.dp))
+        return pv
+
+    ps, us, rl, rr = 0.30313, 0.92745, 0.42632, 0.26557
+    cl = (1.4 * ps / rl) ** 0.5
+    t = 0.2
+    # left state, fan, left star, right star, right state
+    np.testing.assert_allclose(at(0.2), [1.0, 0.0, 0.0, 1.0], rtol=1e-14)
+    np.testing.assert_allclose(at(0.5 + 0.5 * (us - cl) * t + 0.5 * us * t),
+                               [rl, us, 0.0, ps], rtol=5e-5)
+    np.testing.assert_allclose(at(0.5 + 0.5 * (us + 1.75216) * t),
+                               [rr, us, 0.0, ps], rtol=5e-5)
+    np.testing.assert_allclose(at(0.9), [0.125, 0.0, 0.0, 0.1], rtol=1e-14)
+    # inside the rarefaction the flow is isentropic: p / rho^gamma of the
+    # left state, and u + 2c / (gamma - 1) is the left Riemann invariant
+    f = at(0.5 - 0.6 * t)
+    c = (1.4 * f[3] / f[0]) ** 0.5
+    assert 0.0 < f[1] < us and rl < f[0] < 1.0
+    assert abs(f[3] / f[0] ** 1.4 - 1.0) < 1e-12
+    assert abs(f[1] + 2 * c / 0.4 - 2 * 1.4 ** 0.5 / 0.4) < 1e-12
+    # the shock moves at the published speed 1.75216: just behind and
+    # just ahead of it
+    assert at(0.5 + 1.7510 * t)[0] == pytest.approx(rr, rel=5e-5)
+    assert at(0.5 + 1.7530 * t)[0] == 0.125
