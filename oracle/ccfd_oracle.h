@@ -22,6 +22,7 @@
  *   LUSGS                      the (D + L) D^-1 (D + U) product
  *   GMRES_M                    dense solve of the finite-difference system
  *   vectorDotProduct           correctly rounded sums
+ *   globalResidual             area-weighted RMS (test_make_check_oracle.py)
  *   exact Riemann solution     Toro's published Sod solution, isentropic
  *                              fan (test_oracle_golden.py)
  */

@@ -118,3 +118,21 @@ def test_rk_tables_are_the_reference_tables():
             assert got[k] == v, (s, o, k)
         assert got[0] == 0.0
     assert B.rk_coeffs(2, 2) is None
+
+
+def test_global_residual_is_the_area_weighted_rms():
+    """globalResidual (src/analyze.c:521-539): per variable the
+    area-weighted root mean square of dU/dt over the domain; a constant
+    field gives its magnitude"""
+    import numpy as np
+    from oracle import bindings as B
+    from tests import cases
+    mesh = cases.delaunay_mesh(14, seed=7, periodic=False)
+    o = B.Oracle(mesh, B.make_cfg())
+    rng = np.random.default_rng(3)
+    ut = rng.normal(size=(mesh.n_elems, 4)) * [1.0, 10.0, 0.1, 3.0]
+    want = np.sqrt(np.average(ut ** 2, axis=0, weights=mesh.area))
+    np.testing.assert_allclose(o.residual(ut), want, rtol=1e-13)
+    c = np.array([-0.5, 2.0, 0.0, 7.0])
+    np.testing.assert_allclose(o.residual(np.tile(c, (mesh.n_elems, 1))),
+                               np.abs(c), rtol=1e-13, atol=0)
