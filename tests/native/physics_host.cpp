@@ -36,3 +36,19 @@ extern "C" void phys_bc(int type, const double* bcp, double gam, double mu,
     bc_ghost<double>(q, b, ex, nrm[2 * i], nrm[2 * i + 1], 0.0, in + 4 * i,
                      out + 4 * i, x);
 }
+
+// state conversions: mode 0 prim_cons, 1 cons_prim, 2 cons_char (4 -> 3,
+// reference state ref), 3 char_cons (3 -> 4; slot 2 left 0)
+extern "C" void phys_conv(int mode, double gam, long n, const double* in,
+                          double* out, const double* ref) {
+  Eq<double> q = {gam, gam - 1.0, 1.0 / (gam - 1.0), 0.0, 0.72};
+  for (long i = 0; i < n; ++i) {
+    if (mode == 0) prim_cons<double>(q, in + 4 * i, out + 4 * i);
+    else if (mode == 1) cons_prim<double>(q, in + 4 * i, out + 4 * i);
+    else if (mode == 2) cons_char<double>(q, in + 4 * i, out + 3 * i, ref + 4 * i);
+    else {
+      out[4 * i + 2] = 0.0;
+      char_cons<double>(q, in + 3 * i, out + 4 * i, ref + 4 * i);
+    }
+  }
+}

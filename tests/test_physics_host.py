@@ -111,3 +111,54 @@ def test_boundary_states_closed_forms():
             np.testing.assert_allclose(g[0], ref, rtol=1e-13, atol=1e-13)
         g = _bc(L, 5, q[k], q[k:k + 1].copy(), nk)
         np.testing.assert_allclose(g[0], q[k], rtol=1e-13, atol=1e-13)
+
+
+def _conv(L, mode, x, ref=None, width=4):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((len(x), width))
+    r = np.ascontiguousarray(ref if ref is not None else np.zeros((len(x), 4)))
+    L.phys_conv(mode, C.c_double(1.4), len(x), x.ctypes.data_as(dp),
+                out.ctypes.data_as(dp), r.ctypes.data_as(dp))
+    return out
+
+
+def test_state_conversions():
+    """equation of state and characteristic transform of the device code
+    (src/equationOfState.c:19-110): primitive <-> conservative round trip;
+    the positivity clamp of consPrim; char_cons inverts cons_char; and the
+    columns of char_cons are right eigenvectors of the 1-D Euler flux
+    Jacobian at the reference state with eigenvalues u - c, u, u + c"""
+    L = _lib()
+    rng = np.random.default_rng(8)
+    n = 100
+    p = np.column_stack([rng.uniform(0.1, 3, n), rng.uniform(-2, 2, n),
+                         rng.uniform(-2, 2, n), rng.uniform(0.1, 3, n)])
+    c = _conv(L, 0, p)
+    assert np.allclose(c[:, 3], p[:, 3] / 0.4 + 0.5 * p[:, 0] *
+                       (p[:, 1] ** 2 + p[:, 2] ** 2), rtol=1e-14)
+    np.testing.assert_allclose(_conv(L, 1, c), p, rtol=1e-13, atol=1e-14)
+    # clamp: vanishing density, negative internal energy
+    bad = np.array([[1e-20, 0.0, 0.0, 1.0], [1.0, 2.0, 0.0, 0.5]])
+    q = _conv(L, 1, bad)
+    assert q[0, 0] == 1e-10 and q[1, 3] == 1e-10
+    # characteristic transform at reference states (1-D: rho, m, E)
+    ref = p.copy()
+    ch = _conv(L, 2, c, ref, width=3)
+    back = _conv(L, 3, ch, ref)
+    np.testing.assert_allclose(back[:, [0, 1, 3]], c[:, [0, 1, 3]],
+                               rtol=1e-11, atol=1e-11)
+    g = 1.4
+    for k in range(10):
+        rho, u, _, pr = ref[k]
+        a = np.sqrt(g * pr / rho)
+        H = a * a / (g - 1) + 0.5 * u * u
+        A = np.array([[0.0, 1.0, 0.0],
+                      [0.5 * (g - 3) * u * u, (3 - g) * u, g - 1],
+                      [u * (0.5 * (g - 1) * u * u - H), H - (g - 1) * u * u,
+                       g * u]])
+        for j, lam in enumerate((u - a, u, u + a)):
+            e = np.zeros((1, 3))
+            e[0, j] = 1.0
+            r = _conv(L, 3, e, ref[k:k + 1])[0, [0, 1, 3]]
+            np.testing.assert_allclose(A @ r, lam * r, rtol=1e-12,
+                                       atol=1e-12 * np.abs(r).max())
