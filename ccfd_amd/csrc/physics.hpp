@@ -840,13 +840,13 @@ CCFD_HD void bc_ghost(const Eq<T>& q, const BCDev& b, const ExactCfg& ex,
 // ------------------------------------------------------------ limiters
 // per-side limiter value for one variable (src/reconstruction.c:59-69,
 // 143-160); d = grad . GP
-template <int LIM, typename T>
 //
 // The two signs of d use the same expression on different bounds, so the
 // bound is selected first and the expression evaluated once: the same
 // operations on the same operands as the branches of the reference, without
 // divergent paths in a wavefront. d == 0 (or NaN) gives 1, as in the
 // reference; the expression's value there is discarded.
+template <int LIM, typename T>
 CCFD_HD T limiter_phi(T d, T dMax, T dMin, T dMax2, T dMin2, T eps) {
   const bool pos = d > T(0.0);
   const T dm = pos ? dMax : dMin;

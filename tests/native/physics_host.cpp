@@ -52,3 +52,17 @@ extern "C" void phys_conv(int mode, double gam, long n, const double* in,
     }
   }
 }
+
+// per-side limiter value: lim 1 Barth-Jespersen, 2 Venkatakrishnan
+extern "C" void phys_limiter(int lim, long n, const double* d,
+                             const double* dMax, const double* dMin,
+                             double eps, double* out) {
+  for (long i = 0; i < n; ++i)
+    out[i] = lim == 1
+                 ? limiter_phi<1, double>(d[i], dMax[i], dMin[i],
+                                          dMax[i] * dMax[i],
+                                          dMin[i] * dMin[i], eps)
+                 : limiter_phi<2, double>(d[i], dMax[i], dMin[i],
+                                          dMax[i] * dMax[i],
+                                          dMin[i] * dMin[i], eps);
+}

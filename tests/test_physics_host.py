@@ -162,3 +162,41 @@ def test_state_conversions():
             r = _conv(L, 3, e, ref[k:k + 1])[0, [0, 1, 3]]
             np.testing.assert_allclose(A @ r, lam * r, rtol=1e-12,
                                        atol=1e-12 * np.abs(r).max())
+
+
+def test_limiter_functions():
+    """the per-side limiter value of the device code (src/reconstruction.c:
+    25-192) against the published functions: Barth-Jespersen
+    min(1, bound / d); Venkatakrishnan, with epsilon = 0,
+    (y^2 + 2y) / (y^2 + y + 2) for y = bound / d, tending to 1 as epsilon
+    grows; the bound of the side d points to, and 1 where d = 0"""
+    L = _lib()
+    rng = np.random.default_rng(5)
+    n = 400
+    d = rng.normal(size=n)
+    dmax = np.abs(rng.normal(size=n))
+    dmin = -np.abs(rng.normal(size=n))
+    d[:5] = 0.0
+
+    def phi(lim, eps=0.0, d=d, dmax=dmax, dmin=dmin):
+        out = np.empty(len(d))
+        args = [np.ascontiguousarray(a) for a in (d, dmax, dmin)]
+        L.phys_limiter(lim, len(d), *[a.ctypes.data_as(dp) for a in args],
+                       C.c_double(eps), out.ctypes.data_as(dp))
+        return out
+
+    nz = d != 0
+    y = np.where(d > 0, dmax, dmin)[nz] / d[nz]
+    bj, vk = phi(1), phi(2)
+    np.testing.assert_allclose(bj[nz], np.minimum(1.0, y), rtol=1e-15)
+    np.testing.assert_allclose(vk[nz], (y * y + 2 * y) / (y * y + y + 2),
+                               rtol=1e-13)
+    assert np.all(bj[~nz] == 1.0) and np.all(vk[~nz] == 1.0)
+    assert np.all((vk[nz] >= 0) & (vk[nz] < 1.0 + 1e-12) | (y > 1))
+    assert abs(phi(2, d=np.array([1.0]), dmax=np.array([1.0]),
+                   dmin=np.array([-1.0]))[0] - 0.75) < 1e-15
+    # sign symmetry: the mirrored side gives the same value
+    np.testing.assert_allclose(phi(2, d=-d, dmax=-dmin, dmin=-dmax), vk,
+                               rtol=1e-15)
+    # a large epsilon switches the limiter off
+    assert np.all(np.abs(phi(2, eps=1e12)[nz] - 1.0) < 1e-9)
