@@ -66,3 +66,11 @@ extern "C" void phys_limiter(int lim, long n, const double* d,
                                           dMax[i] * dMax[i],
                                           dMin[i] * dMin[i], eps);
 }
+
+// viscous flux (f in x, g in y) at a face state with its gradients
+extern "C" void phys_visc(double gam, double mu, double Pr, const double* st,
+                          const double* gx, const double* gy, double* f,
+                          double* g) {
+  Eq<double> q = {gam, gam - 1.0, 1.0 / (gam - 1.0), mu, Pr};
+  diffusion_flux<double>(q, st, gx, gy, f, g);
+}

@@ -200,3 +200,46 @@ def test_limiter_functions():
                                rtol=1e-15)
     # a large epsilon switches the limiter off
     assert np.all(np.abs(phi(2, eps=1e12)[nz] - 1.0) < 1e-9)
+
+
+def test_viscous_flux_is_the_navier_stokes_flux():
+    """diffusionFlux (src/fluxCalculation.c:855-873) of the device code
+    against the textbook Navier-Stokes flux: stress tensor
+    mu (grad u + grad u^T) - 2/3 mu (div u) I, its work u . tau, and
+    Fourier conduction mu gamma / ((gamma - 1) Pr) grad(p / rho), the
+    temperature gradient taken by central differences; a rigid rotation
+    carries no stress"""
+    L = _lib()
+    rng = np.random.default_rng(2)
+    gam, mu, Pr = 1.4, 3e-3, 0.72
+
+    def visc(st, gx, gy):
+        f, g = np.zeros(4), np.zeros(4)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (st, gx, gy)]
+        L.phys_visc(C.c_double(gam), C.c_double(mu), C.c_double(Pr),
+                    *[x.ctypes.data_as(dp) for x in a], f.ctypes.data_as(dp),
+                    g.ctypes.data_as(dp))
+        return f, g
+
+    k = mu * gam / ((gam - 1) * Pr)
+    for _ in range(20):
+        st = np.array([rng.uniform(0.5, 2), rng.normal(), rng.normal(),
+                       rng.uniform(0.5, 2)])
+        gx, gy = rng.normal(size=4), rng.normal(size=4)
+        J = np.array([[gx[1], gy[1]], [gx[2], gy[2]]])   # d(u, v)/d(x, y)
+        tau = mu * (J + J.T) - 2.0 / 3.0 * mu * np.trace(J) * np.eye(2)
+        h = 1e-6
+        for axis, (grad, flux) in enumerate(((gx, 0), (gy, 1))):
+            T = [(st[3] + s * h * grad[3]) / (st[0] + s * h * grad[0])
+                 for s in (1.0, -1.0)]
+            dT = (T[0] - T[1]) / (2 * h)
+            want = np.array([0.0, tau[0, axis], tau[1, axis],
+                             st[1] * tau[0, axis] + st[2] * tau[1, axis] +
+                             k * dT])
+            got = visc(st, gx, gy)[flux]
+            np.testing.assert_allclose(got, want, rtol=1e-8, atol=1e-12)
+    # rigid rotation u = -w y, v = w x: no stress, no work
+    w = 0.7
+    st = np.array([1.0, 0.3, -0.2, 1.0])
+    f, g = visc(st, np.array([0, 0, w, 0.0]), np.array([0, -w, 0, 0.0]))
+    assert np.all(np.abs(f) < 1e-18) and np.all(np.abs(g) < 1e-18)
