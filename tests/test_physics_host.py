@@ -243,3 +243,24 @@ def test_viscous_flux_is_the_navier_stokes_flux():
     st = np.array([1.0, 0.3, -0.2, 1.0])
     f, g = visc(st, np.array([0, 0, w, 0.0]), np.array([0, -w, 0, 0.0]))
     assert np.all(np.abs(f) < 1e-18) and np.all(np.abs(g) < 1e-18)
+
+
+@pytest.mark.parametrize("code", list(range(1, 12)))
+def test_fluxes_are_consistent(code):
+    """every numerical flux of the device code is consistent: with equal
+    states on both sides it returns the Euler flux of that state in the
+    face-normal frame, (rho u, rho u^2 + p, rho u v, u (E + p))"""
+    L = _lib()
+    rng = np.random.default_rng(code)
+    n = 200
+    s = np.column_stack([rng.uniform(0.2, 2, n), rng.uniform(-2, 2, n),
+                         rng.uniform(-2, 2, n), rng.uniform(0.2, 2, n)])
+    s = np.ascontiguousarray(s)
+    F = np.empty_like(s)
+    L.phys_riemann(code, 0, C.c_double(1.4), n, s.ctypes.data_as(dp),
+                   s.ctypes.data_as(dp), F.ctypes.data_as(dp))
+    rho, u, v, p = s.T
+    E = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    want = np.column_stack([rho * u, rho * u * u + p, rho * u * v,
+                            u * (E + p)])
+    np.testing.assert_allclose(F, want, rtol=1e-10, atol=1e-12)
