@@ -1,0 +1,214 @@
+"""The reference-named entry points of libccfd_host.so on the GPU, called the
+way ccfd's own C code calls them, against the oracle:
+
+* fvTimeDerivative (src/finiteVolume.h:14): dU/dt bit for bit;
+* convectiveFlux (src/fluxCalculation.c:803-844) under every iFlux a case
+  file selects: bit for bit (Godunov's exact solver calls pow: 1e-10);
+* vectorDotProduct (src/linearSolver.c:115-128) on [var][elem] double**
+  vectors: the oracle's serial sum to 1e-13;
+* GMRES_M (src/linearSolver.c:385-510) on the implicit Sod case: the
+  oracle's single GMRES cycle to 1e-7;
+* the complete C program of INTEGRATION.md, compiled and run.
+"""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from ccfd_amd import _native as N
+from ccfd_amd import gpu as G
+from oracle import bindings as B
+from oracle import timedisc as TD
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CHECK = os.path.join(HERE, "golden", "check")
+SOD_IMPLICIT = os.path.join(HERE, "golden", "calc", "sod_implicit.ini")
+LIB = os.path.join(ROOT, "ccfd_amd", "lib")
+INC = os.path.join(ROOT, "include")
+
+dp = C.POINTER(C.c_double)
+dpp = C.POINTER(dp)
+
+
+def _host():
+    G.lib()
+    L = N._load("libccfd_host.so")
+    L.ccfd_init.restype = C.c_int
+    L.ccfd_init.argtypes = [C.c_char_p, C.c_int, C.c_char_p]
+    L.ccfd_finalize.argtypes = []
+    L.ccfd_case_context.restype = C.c_void_p
+    L.fvTimeDerivative.argtypes = [C.c_double]
+    L.convectiveFlux.argtypes = [C.c_double] * 8 + [dp]
+    L.vectorDotProduct.restype = C.c_double
+    L.vectorDotProduct.argtypes = [dpp, dpp]
+    L.GMRES_M.argtypes = [C.c_double, C.c_double, C.c_double, dpp,
+                          C.c_double, dp, dpp]
+    return L
+
+
+def _init(L, ini):
+    err = C.create_string_buffer(256)
+    rc = L.ccfd_init(ini.encode(), 0, err)
+    assert rc == 0, err.value.decode()
+
+
+def _rows(a):
+    """[var][elem] double** view of an (N, 4) array: four contiguous rows,
+    as the reference allocates its vectors (src/linearSolver.c:60-99)"""
+    rows = np.ascontiguousarray(a.T)
+    arr = (dp * 4)(*[rows[v].ctypes.data_as(dp) for v in range(4)])
+    return arr, rows
+
+
+def _ut(L, n):
+    ut = np.empty((n, 4))
+    G.check(G.lib().ccfd_gpu_get_ut(C.c_void_p(L.ccfd_case_context()),
+                                    N.dptr(ut)))
+    return ut
+
+
+@pytest.mark.parametrize("name", ["sod_SO2_VK", "sod_SO2_BJ", "sod_FF02",
+                                  "sod_FF09"])
+def test_fvtimederivative_matches_the_oracle(name):
+    ini = os.path.join(CHECK, name + ".ini")
+    L = _host()
+    _init(L, ini)
+    try:
+        mesh, cfg, prim, _ = TD.setup_case(ini)
+        L.fvTimeDerivative(0.0)
+        ut = _ut(L, mesh.n_elems)
+    finally:
+        L.ccfd_finalize()
+    orc = B.Oracle(mesh, cfg)
+    orc.set_prim(prim)
+    uo = orc.fvtd(0.0)
+    assert np.abs(uo).max() > 0.0
+    assert np.array_equal(ut, uo)
+
+
+FF = ["sod_FF%02d" % k for k in (1, 2, 3, 4, 5, 6, 7, 9, 11)]
+
+
+@pytest.mark.parametrize("name", FF)
+def test_convectiveflux_matches_the_oracle(name):
+    """each check case selects its own iFlux; convectiveFlux then evaluates
+    that flux, as fluxCalculation calls it face by face"""
+    ini = os.path.join(CHECK, name + ".ini")
+    L = _host()
+    _init(L, ini)
+    _, cfg, _, c = TD.setup_case(ini)
+    rng = np.random.default_rng(c["flux"])
+    n = 64
+    Ls = np.column_stack([rng.uniform(0.2, 2, n), rng.uniform(-1.5, 1.5, n),
+                          rng.uniform(-1, 1, n), rng.uniform(0.2, 2, n)])
+    Rs = np.column_stack([rng.uniform(0.2, 2, n), rng.uniform(-1.5, 1.5, n),
+                          rng.uniform(-1, 1, n), rng.uniform(0.2, 2, n)])
+    got = np.empty((n, 4))
+    try:
+        for i in range(n):
+            f = (C.c_double * 4)()
+            l, r = Ls[i], Rs[i]
+            L.convectiveFlux(l[0], r[0], l[1], r[1], l[2], r[2], l[3], r[3],
+                             C.cast(f, dp))
+            got[i] = list(f)
+    finally:
+        L.ccfd_finalize()
+    want = B.riemann(cfg, c["flux"], Ls, Rs)
+    if c["flux"] == 1:
+        assert np.allclose(got, want, rtol=1e-10, atol=1e-12)
+    else:
+        assert np.array_equal(got, want)
+
+
+def test_vectordotproduct_matches_the_oracle():
+    ini = os.path.join(CHECK, "sod_SO2_VK.ini")
+    L = _host()
+    _init(L, ini)
+    mesh, _, _, _ = TD.setup_case(ini)
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(mesh.n_elems, 4))
+    b = rng.normal(size=(mesh.n_elems, 4))
+    pa, ra = _rows(a)
+    pb, rb = _rows(b)
+    try:
+        d = L.vectorDotProduct(pa, pb)
+        d2 = L.vectorDotProduct(pa, pb)
+    finally:
+        L.ccfd_finalize()
+    o = B.dot(a, b)
+    print("vectorDotProduct %.17g oracle %.17g" % (d, o))
+    assert d == d2
+    assert abs(d - o) <= 1e-13 * np.abs(a * b).sum()
+
+
+def test_gmres_m_matches_the_oracle():
+    """GMRES_M(time, dt, alpha, B, normB, abortCrit, deltaX) about the
+    initial state of calc/riemann/sod_implicit.ini (nKdim 5, LU-SGS): the
+    abort criterion is epsGMRES |B| (src/linearSolver.c:388) and deltaX is
+    the oracle's single Krylov cycle"""
+    L = _host()
+    _init(L, SOD_IMPLICIT)
+    mesh, cfg, prim, c = TD.setup_case(SOD_IMPLICIT)
+    gs_dt = None
+    try:
+        ctx = C.c_void_p(L.ccfd_case_context())
+        dt, vd = C.c_double(), C.c_int()
+        G.check(G.lib().ccfd_gpu_calc_dt(ctx, 0.0, c["IOtimeInterval"],
+                                         c["tEnd"], C.byref(dt), C.byref(vd)))
+        gs_dt = dt.value
+        rng = np.random.default_rng(8)
+        Bv = rng.normal(size=(mesh.n_elems, 4)) * 1e-2
+        normB = float(np.sqrt(B.dot(Bv, Bv)))
+        pB, rB = _rows(Bv)
+        dX = np.zeros((4, mesh.n_elems))
+        pX = (dp * 4)(*[dX[v].ctypes.data_as(dp) for v in range(4)])
+        ac = C.c_double(0.0)
+        L.GMRES_M(gs_dt, gs_dt, 1.0, pB, normB, C.byref(ac), pX)
+    finally:
+        L.ccfd_finalize()
+    assert ac.value == c["epsGMRES"] * normB
+    orc = B.Oracle(mesh, cfg)
+    orc.set_prim(prim)
+    dto, _, _ = orc.calc_dt(0.0, c["IOtimeInterval"], c["tEnd"])
+    assert dto == gs_dt
+    io = B.ImplicitCfg(c["nKdim"], c["nNewtonIter"], int(c["precond"]),
+                       c["epsNewton"], c["epsGMRES"], c["gammaEW"])
+    dXo, it = orc.gmres(io, 0.0, dto, Bv)
+    err = np.abs(dX.T - dXo).max() / np.abs(dXo).max()
+    print("GMRES_M: %d oracle iterations, rel err %.2e" % (it, err))
+    assert err < 1e-7
+
+
+def test_the_integration_program_runs(tmp_path):
+    """INTEGRATION.md's C program: ccfd_init, fvTimeDerivative, timeDisc,
+    ccfd_write_csv on sod_TO2_4RK.ini, then the CSV is the oracle's run"""
+    import re
+    cc = shutil.which("gcc")
+    if not cc:
+        pytest.skip("gcc not available")
+    with open(os.path.join(ROOT, "INTEGRATION.md")) as f:
+        progs = [b for b in re.findall(r"```c\n(.*?)```", f.read(), re.S)
+                 if "int main" in b]
+    assert len(progs) == 1
+    (tmp_path / "app.c").write_text(progs[0])
+    r = subprocess.run([cc, "-std=c99", "-I" + INC, "app.c", "-L" + LIB,
+                        "-lccfd_host", "-lccfd_gpu", "-Wl,-rpath," + LIB,
+                        "-Wl,-rpath-link,/opt/rocm/lib", "-lm", "-o", "app"],
+                       cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    ini = os.path.join(CHECK, "sod_TO2_4RK.ini")
+    shutil.copy(ini, tmp_path / "sod_TO2_4RK.ini")
+    r = subprocess.run(["./app"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    text = (tmp_path / "final.csv").read_text()
+    _, _, prim, mesh, _ = TD.run_case(ini)
+    _, otext = TD.csv_rows(mesh, prim)
+    assert text == otext
