@@ -6,16 +6,23 @@ Workload (default): Delaunay triangulation of a jittered 1415 x 1415 point
 grid on the periodic unit square (3,998,792 triangles), smooth seeded state
 (density / pressure = 1 + 0.1 x 4 sine modes, Mach 0.5 flow at a random
 angle), HLLC flux, 2nd order with Venkatakrishnan limiting (K = 5),
-nRKstages = 4 / timeOrder = 2, FP64. A step is one calcTimeStep plus four
-fvTimeDerivative + RK stage updates; the metric counts cell-stage updates.
+nRKstages = 4 / timeOrder = 2, FP64. A step is the reference's explicit
+step: calcTimeStep, four fvTimeDerivative + RK stage updates and
+globalResidual (src/timeDiscretization.c:362-401). The metric counts
+cell-stage updates.
 
-Prints one JSON line (rank 0) with the metric, a roofline entry for the
-dominant kernel (the fused operator + stage update, or the flux + gather +
-stage update kernel of the split pair) from HIP-event kernel timings over the
-timed region, and a CPU baseline: the reference's own OpenMP
-fvTimeDerivative on the same mesh and state, timed on this host.
+Prints one JSON line with the metric, a roofline entry for the dominant
+kernel (the fused operator + stage update, or the flux + gather + stage
+update kernel of the split pair) from HIP-event kernel timings over the
+timed region, the same measurement for the Roe, HLL and AUSMD fluxes and
+for the FP32-flux mode, the dU/dt of the initial state compared with the
+reference's own fvTimeDerivative (compiled from its sources) on the same
+mesh and state for every flux, and a CPU baseline: that reference
+fvTimeDerivative timed with OpenMP on this host. Beside it, short batched
+and served samples of the many-mesh workloads.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -38,6 +45,12 @@ B_FLUX_FP32 = 186.0   # same with FP32 operator data (FP64 U0 / U)
 # whole stage in one launch
 B_STAGE_FP64 = 572.0
 B_STAGE_FP32 = 330.0
+FLUX_NAMES = {2: "roe", 3: "hll", 5: "hllc", 9: "ausmd"}
+# the device code the committed traffic figures were measured on
+KERNEL_SOURCES = ("fused.hpp", "operator.hpp", "physics.hpp", "layout.hpp",
+                  "op_flux.hip", "soa.cpp")
+
+
 # what the fused kernel itself must move per element and stage (each array
 # once, side states in LDS), FP64: reads state 32 + links 12 + LSQ weights
 # 48 + GP 48 + eps^2 8 + side normals 48 + signed lengths 24 + 1/area 8 +
@@ -66,16 +79,32 @@ def build_case(n, seed):
     return raw, bcs, mesh, prim
 
 
-def cpu_baseline(mesh, prim, flux, venk_k, seconds):
-    """The reference's fvTimeDerivative (compiled from its own sources with
-    OpenMP, oracle/_ref/libccfdref_omp.so) on the same mesh and state."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _has_x86_64_v3():
+    """AVX2, FMA, BMI2 and MOVBE: what -march=x86-64-v3 code may use"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    fl = set(line.split(":", 1)[1].split())
+                    return {"avx2", "fma", "bmi2", "movbe", "f16c"} <= fl
+    except OSError:
+        pass
+    return False
+
+
+def _ref_ini(flux, venk_k):
     from oracle import bindings as B
-    if not B.ref_available(False) or not os.path.exists(
-            os.path.join(ROOT, "oracle", "_ref", "libccfdref_omp.so")):
-        return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(
-        16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     ini = "/tmp/ccfd_bench_ref_%d.ini" % os.getpid()
     B.write_ini(ini, [("fluxFunction", flux), ("nBC", 4),
                       ("BCtype", 701), ("connection", (0.0, 1.0)),
@@ -84,13 +113,44 @@ def cpu_baseline(mesh, prim, flux, venk_k, seconds):
                       ("BCtype", 712), ("connection", (0.0, 0.0)),
                       ("spatialOrder", 2), ("limiter", 2),
                       ("venk_k", venk_k)])
+    return ini
+
+
+def reference_run(mesh, prim, flux, venk_k, seconds, parity_fluxes):
+    """The reference's fvTimeDerivative, compiled from its own sources with
+    OpenMP (oracle/_ref), on the same mesh and state: dU/dt of the state
+    for every flux in parity_fluxes (for the parity check), then the main
+    flux timed for about `seconds` (the CPU baseline). Returns
+    (baseline dict or None, {flux: dU/dt})."""
+    from oracle import bindings as B
+    if not B.ref_available(False):
+        return None, {}
+    v3 = _has_x86_64_v3() and os.path.exists(
+        os.path.join(ROOT, "oracle", "_ref", "libccfdref_omp_v3.so"))
+    if not v3 and not os.path.exists(
+            os.path.join(ROOT, "oracle", "_ref", "libccfdref_omp.so")):
+        return None, {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(
+        16, os.cpu_count() or 1)
+    # read by libgomp when the library below loads it
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    ini = _ref_ini(flux, venk_k)
     devnull = os.open(os.devnull, os.O_WRONLY)
     saved = os.dup(1)
     os.dup2(devnull, 1)   # the reference prints its parameter echo
+    uts = {}
     try:
-        ref = B.Ref(omp=True)
+        ref = B.Ref(omp=True, v3=v3)
+        t0 = time.time()
         ref.setup(ini, mesh)
+        t_setup = time.time() - t0
         ref.set_prim(prim)
+        for code in parity_fluxes:
+            ref.set_flux(code)
+            uts[code] = ref.fvtd_ut(0.0)
+        ref.set_flux(flux)
         ref.fvtd_only(0.0)   # warm-up
         calls, t0 = 0, time.time()
         while calls < 3 or time.time() - t0 < seconds:
@@ -105,36 +165,81 @@ def cpu_baseline(mesh, prim, flux, venk_k, seconds):
         os.unlink(ini)
     except OSError:
         pass
-    cpu = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {
+    flags = ("-std=c99 -O3 -march=x86-64-v3 -flto -fopenmp" if v3 else
+             "-std=c99 -O3 -fopenmp")
+    base = {
         "value": mesh.n_elems * calls / el,
         "unit": "cell-evaluations/s (reference fvTimeDerivative, OpenMP)",
         "cores": threads,
         "kind": "reference",
-        "sample": "%d fvTimeDerivative calls on the same %d-triangle mesh "
-                  "and state (%.1f s; RK update ~2%% of CPU time not "
-                  "included), CPU: %s" % (calls, mesh.n_elems, el, cpu),
+        "sample": "%d fvTimeDerivative calls on the same %d-triangle mesh and "
+                  "state (%.1f s; RK update, dt and residual, about 8%% of "
+                  "the reference's CPU time per step, not included). Build: "
+                  "the reference's own finiteVolume.c, reconstruction.c, "
+                  "boundary.c, fluxCalculation.c, equationOfState.c and "
+                  "helpers, gcc %s (the reference's Makefile flags with "
+                  "-march=x86-64-v3 for -march=native). Data: the "
+                  "reference's elem_t / side_t records and pointer links, "
+                  "allocated as contiguous arrays by the harness "
+                  "(oracle/ref_harness.c) instead of one malloc per record "
+                  "by mesh.c (set-up %.1f s). OpenMP: %d threads, "
+                  "OMP_PROC_BIND=%s, OMP_PLACES=%s. CPU: %s"
+                  % (calls, mesh.n_elems, el, flags, t_setup, threads,
+                     os.environ.get("OMP_PROC_BIND"),
+                     os.environ.get("OMP_PLACES"), _cpu_model()),
     }
+    return base, uts
+
+
+def kernel_source_sha():
+    """hash of the device sources the committed traffic figures depend on"""
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        try:
+            with open(os.path.join(ROOT, "ccfd_amd", "csrc", f), "rb") as fh:
+                h.update(fh.read())
+        except OSError:
+            h.update(b"missing:" + f.encode())
+    return h.hexdigest()[:16]
 
 
 def pmc_traffic(kernel_key):
     """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE, copy-kernel scaled)."""
+    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE, copy-kernel scaled),
+    with its source: None, and a note, when the device sources have changed
+    since those counters were read."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get(kernel_key)
     except (OSError, ValueError):
-        return None
+        return None, "no committed counter summary"
+    ent = d.get("kernels", {}).get(kernel_key)
+    if ent is None:
+        return None, "no counter figure for %s" % kernel_key
+    sha = kernel_source_sha()
+    if d.get("kernel_source_sha") != sha:
+        return None, ("stale: profiles/pmc_traffic.json was measured on "
+                      "device sources %s, these are %s"
+                      % (d.get("kernel_source_sha"), sha))
+    return ent["bytes_per_launch"], "profiles/%s (%s)" % (ent["summary"],
+                                                        ent["command"])
+
+
+def _ut_parity(got, ref, gamma=1.4):
+    """dU/dt of the device against the reference's: bitwise, the largest
+    difference relative to the largest value of its variable, and the
+    relative L2 / Linf of the density and total-energy rates"""
+    import numpy as np
+    scale = np.maximum(np.abs(ref).max(axis=0), 1e-300)
+    d = np.abs(got - ref)
+    out = {"bitwise": bool(np.array_equal(got, ref)),
+           "max_rel_err": float((d / scale).max())}
+    for name, v in (("density", 0), ("energy", 3)):
+        l2, linf = _rel_err(got[:, v], ref[:, v])
+        out["%s_ut_rel_l2" % name] = l2
+        out["%s_ut_rel_linf" % name] = linf
+    return out
 
 
 def _rel_err(a, b):
@@ -159,13 +264,24 @@ def _energy_err(a, b, gamma=1.4):
     return _rel_err(energy(a), energy(b))
 
 
-def _err_summary(errs, both=None):
-    """max and median relative L2, max relative Linf over solves; with a
-    mask, the same maxima over the solves it marks"""
+def _max_mach(prim, gamma=1.4):
     import numpy as np
-    out = {"l2_max": max(e[0] for e in errs),
-           "l2_median": float(np.median([e[0] for e in errs])),
-           "linf_max": max(e[1] for e in errs)}
+    c = np.sqrt(gamma * prim[:, 3] / prim[:, 0])
+    return float((np.hypot(prim[:, 1], prim[:, 2]) / c).max())
+
+
+def _err_summary(errs, both=None):
+    """max and median relative L2, max relative Linf over the solves with
+    finite errors, and the count of the others; with a mask, the same
+    maxima over the solves it marks"""
+    import numpy as np
+
+    ok = [e for e in errs if np.isfinite(e[0]) and np.isfinite(e[1])]
+    out = {"l2_max": max([e[0] for e in ok] or [0.0]),
+           "l2_median": float(np.median([e[0] for e in ok])) if ok else None,
+           "linf_max": max([e[1] for e in ok] or [0.0]),
+           # solves whose error is not finite (not in the maxima above)
+           "nonfinite": len(errs) - len(ok)}
     if both is not None:
         sel = [e for e, m in zip(errs, both) if m]
         out.update(both_converged=len(sel),
@@ -230,8 +346,17 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
         out["energy_err_vs_fp64"] = _err_summary(
             [_energy_err(r["prim"], q["prim"]) for r, q in zip(res, res64)],
             both)
-        out["speedup_solves_per_s"] = (out["served"]["solves_per_s"] /
-                                       out["one_at_a_time_fp64"]["solves_per_s"])
+        one = out["one_at_a_time_fp64"]
+        # solves: requests that reached the target drop; capped and
+        # non-finite retirements are failures in both columns
+        out["speedup_solves_per_s"] = (
+            out["served"]["solves_per_s"] / one["solves_per_s"]
+            if one.get("solves_per_s") else None)
+        out["converged_in_both"] = int(sum(both))
+        # the requests both modes solved, each mode timed over its own run
+        out["speedup_on_requests_both_solved"] = (
+            (sum(both) / out["served"]["makespan_s"]) /
+            (sum(both) / one["makespan_s"]) if sum(both) else None)
     return out
 
 
@@ -243,12 +368,16 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
     autotuner's operator; then every instance one at a time on the plain
     FP64 single-mesh path."""
     import concurrent.futures as cf
+    import multiprocessing as mp
     from ccfd_amd import batch as BT
     from ccfd_amd import gpu as G
     from ccfd_amd import workloads as W
     specs = W.batched_specs(n_inst, seed=4, cells=cells)
     t0 = time.time()
-    with cf.ProcessPoolExecutor(min(16, os.cpu_count() or 4)) as ex:
+    # spawned workers, not forked ones: this process may already hold a HIP
+    # runtime (pinned staging pages that a fork turns copy-on-write)
+    with cf.ProcessPoolExecutor(min(16, os.cpu_count() or 4),
+                                mp_context=mp.get_context("spawn")) as ex:
         raws = list(ex.map(W.batched_raw, specs, chunksize=2))
     with cf.ThreadPoolExecutor(8) as ex:
         cases = list(ex.map(lambda sr: W.batched_case(*sr), zip(specs, raws)))
@@ -300,9 +429,16 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
                                          ncell * 3 * steps / tot}
         out["speedup_solves_per_s"] = out["solves_per_s"] / (len(ids) / tot)
         out["density_err_vs_fp64"] = _err_summary(errs)
+        bad = [k for k, e in enumerate(errs) if not e[0] <= 1e-6]
         out["density_err_vs_fp64"].update({
-            "instances_l2_within_1e-6": int(sum(e[0] <= 1e-6 for e in errs)),
-            "instances": len(errs)})
+            "instances_l2_within_1e-6": len(errs) - len(bad),
+            "instances": len(errs),
+            # instances over the 1e-6 target: failures of the FP32-flux
+            # mode at that tolerance (shocks moved by a fraction of a cell)
+            "instances_over_1e-6": [
+                {"instance": k, "cells": cases[k][0].n_elems,
+                 "density_rel_l2": errs[k][0], "density_rel_linf": errs[k][1],
+                 "max_mach": _max_mach(cases[k][1])} for k in bad]})
         out["energy_err_vs_fp64"] = _err_summary(eerrs)
     b.close()
     return out
@@ -415,6 +551,13 @@ def main():
                          "short served sample unless --no-served); batched: "
                          "many meshes in one arena; served: the request loop")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-flux-variants", action="store_true",
+                    help="operator workload: only the main flux and precision")
+    ap.add_argument("--variant-steps", type=int, default=10,
+                    help="timed steps of the other fluxes / FP32 mode")
+    ap.add_argument("--served-requests", type=int, default=12)
+    ap.add_argument("--served-rate", type=float, default=4.0)
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--min-cells", type=float, default=None)
@@ -499,11 +642,14 @@ def main():
         return
 
     raw, bcs, mesh, prim = build_case(args.n, args.seed)
+    del raw, bcs
     prec = G.FP64 if args.precision == "fp64" else G.FP32_FLUX
-    prm = G.params(flux=args.flux, order=2, limiter=2, venk_k=args.venk_k,
-                   nrk=4, torder=2, cfl=0.9, precision=prec)
+
+    def prm_of(flux, precision):
+        return G.params(flux=flux, order=2, limiter=2, venk_k=args.venk_k,
+                        nrk=4, torder=2, cfl=0.9, precision=precision)
     t0 = time.time()
-    solver = G.Solver(mesh, prm)
+    solver = G.Solver(mesh, prm_of(args.flux, prec))
     solver.set_variant({"auto": G.KV_AUTO, "split": G.KV_SPLIT,
                         "fused": G.KV_FUSED,
                         "fused_face": G.KV_FUSED_FACE}[args.variant])
@@ -511,72 +657,124 @@ def main():
     fused = kv in (G.KV_FUSED, G.KV_FUSED_FACE)
     solver.set_prim(prim)
     log("upload %.1f s" % (time.time() - t0))
-    big = 1e300
-    t = 0.0
-    if args.warmup > 0:
-        t, _ = solver.advance(args.warmup, t, big, big, residual=False)
-    solver.sync()
-
-    # timed region: K steps, HIP events around every kernel on the stream
-    solver.sync()
-    w0 = time.perf_counter()
-    ms, launches = solver.profile_steps(args.steps, big, big)
-    solver.sync()
-    w1 = time.perf_counter()
-    el = w1 - w0
-    stages = 4 * args.steps
     N = mesh.n_elems
-    value = N * stages / el
-    ms_step = 1e3 * el / args.steps
+    big = 1e300
+    uts = {}   # (flux, precision) -> dU/dt of the initial state
 
-    bstage = B_STAGE_FP64 if prec == G.FP64 else B_STAGE_FP32
+    def measure(flux, precision, steps, warmup):
+        """dU/dt of the initial state, then `warmup` untimed and `steps`
+        timed steps (calcTimeStep + 4 stages + globalResidual each)"""
+        solver.set_params(prm_of(flux, precision))
+        solver.set_prim(prim)
+        solver.fvtd(0.0)
+        uts[(flux, precision)] = solver.ut()
+        solver.set_prim(prim)
+        if warmup > 0:
+            solver.advance(warmup, 0.0, big, big, residual=True)
+        solver.sync()
+        w0 = time.perf_counter()
+        ms, launches, res = solver.profile_steps(steps, big, big,
+                                                 residual=True)
+        solver.sync()
+        el = time.perf_counter() - w0
+        bstage = B_STAGE_FP64 if precision == G.FP64 else B_STAGE_FP32
+        bker = bstage if fused else (B_FLUX_FP64 if precision == G.FP64
+                                     else B_FLUX_FP32)
+        achieved = bker * N / (ms["flux"] * 1e-3) / 1e9
+        return {"value": N * 4 * steps / el, "ms_per_step": 1e3 * el / steps,
+                "seconds": el, "kernel_ms": {k: round(v, 4)
+                                             for k, v in ms.items()},
+                "launches": launches, "residual": res.tolist(),
+                "achieved_gbs": achieved, "bytes_per_cell": bker,
+                "frac": achieved / PEAK_HBM_GBS}
+
+    # timed region of the headline: K steps, HIP events around every kernel
+    main = measure(args.flux, prec, args.steps, args.warmup)
+    ms = main["kernel_ms"]
+    log("main: %.3f ms / step, kernel %.4f ms" % (main["ms_per_step"],
+                                                   ms["flux"]))
+    pname = args.precision
     if fused:
-        bker = bstage
         kname = ("%s (reconstruction + limiter + Riemann flux + gather + "
                  "RK stage update, side states in LDS)"
                  % ("fused_face_kernel" if kv == G.KV_FUSED_FACE
                     else "fused_kernel"))
         kkey = "%s_%s" % ("fused_face" if kv == G.KV_FUSED_FACE else "fused",
-                          args.precision)
+                          pname)
     else:
-        bker = B_FLUX_FP64 if prec == G.FP64 else B_FLUX_FP32
         kname = "flux_kernel (Riemann flux + gather + RK stage update)"
-        kkey = "flux_%s" % args.precision
-    achieved = bker * N / (ms["flux"] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1),
+        kkey = "flux_%s" % pname
+    traffic, tsrc = pmc_traffic(kkey)
+    roof = {"bound": "hbm", "achieved": round(main["achieved_gbs"], 1),
             "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": pmc_traffic(kkey),
+            "frac": round(main["frac"], 4),
+            "traffic": traffic,
+            "traffic_source": tsrc,
             "kernel": kname,
-            "kernel_ms": round(ms["flux"], 4),
-            "bytes_per_cell": bker}
+            "kernel_ms": ms["flux"],
+            "kernel_launches": main["launches"][2],
+            "bytes_per_cell": main["bytes_per_cell"],
+            "bytes_note": "achieved = algorithmic bytes per cell and stage "
+                          "(the two-phase dataflow of fvTimeDerivative + RK "
+                          "update: %g B) x cells / mean kernel time over the "
+                          "timed launches" % main["bytes_per_cell"]}
+    if traffic:
+        roof["measured_bytes_per_cell"] = round(traffic / N, 1)
+        roof["measured_hbm_gbs"] = round(traffic / (ms["flux"] * 1e-3) / 1e9, 1)
     if fused:
         bmin = fused_min_bytes(prec == G.FP64, 4)
         roof["compulsory_bytes_per_cell"] = bmin
         roof["compulsory_gbs"] = round(bmin * N / (ms["flux"] * 1e-3) / 1e9, 1)
 
+    # the other fluxes of the workload, and the FP32-flux mode
+    others = {}
+    flux_list = [] if args.no_flux_variants else [
+        f for f in (2, 3, 5, 9) if f != args.flux]
+    for f in flux_list:
+        r = measure(f, prec, args.variant_steps, 2)
+        log("%s: %.3f ms / step" % (FLUX_NAMES.get(f, f), r["ms_per_step"]))
+        others[FLUX_NAMES.get(f, str(f))] = r
+    fp32 = None
+    if prec == G.FP64 and not args.no_flux_variants:
+        fp32 = measure(args.flux, G.FP32_FLUX, args.variant_steps, 2)
+        log("fp32 flux: %.3f ms / step" % fp32["ms_per_step"])
+
     copy_gbs = None
     if args.copy_probe:
         copy_gbs = G.copy_probe(1 << 30, 20)
         log("copy probe: %.0f GB/s" % copy_gbs)
+    solver.close()
+    del solver
 
-    cpu = None
+    # the reference's fvTimeDerivative on the same mesh and state: parity of
+    # every flux, and the CPU baseline
+    cpu, refut = None, {}
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(mesh, prim, args.flux, args.venk_k,
-                               args.cpu_seconds)
-        except Exception as e:   # the baseline must not hide the GPU result
-            log("cpu baseline failed:", e)
-            cpu = None
+            cpu, refut = reference_run(
+                mesh, prim, args.flux, args.venk_k, args.cpu_seconds,
+                [args.flux] + flux_list)
+        except Exception as e:   # must not hide the GPU result
+            log("reference run failed:", e)
+    parity = {}
+    for (f, p), ut in uts.items():
+        if f not in refut:
+            continue
+        key = "%s_%s" % (FLUX_NAMES.get(f, str(f)),
+                         "fp64" if p == G.FP64 else "fp32_flux")
+        parity[key] = _ut_parity(ut, refut[f])
+    del uts, refut
+    for name, r in others.items():
+        r["parity_vs_reference"] = parity.get("%s_%s" % (name, pname))
 
     out = {
         "metric": "cell-stage updates/s (4M tri HLLC, 2nd order VK, RK4)",
-        "value": value,
+        "value": main["value"],
         "unit": "cell-stage updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": main["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -585,27 +783,47 @@ def main():
         "config": {
             "workload": "fvTimeDerivative + RK stage, periodic jittered "
                         "Delaunay %d triangles, HLLC, VK K=%g, "
-                        "nRKstages=4 timeOrder=2, %s"
+                        "nRKstages=4 timeOrder=2, %s; a step is "
+                        "calcTimeStep + 4 stages + globalResidual"
                         % (N, args.venk_k, args.precision),
             "cells": N, "stages_per_step": 4, "flux": args.flux,
             "precision": args.precision,
             "operator": {G.KV_SPLIT: "split", G.KV_FUSED: "fused",
                          G.KV_FUSED_FACE: "fused_face"}[kv],
         },
-        "hbm_gbs_stage_effective": round(value * bstage / 1e9, 1),
-        "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "kernel_ms": ms,
+        "residual_last_step": main["residual"],
+        "parity_vs_reference": parity.get("%s_%s" % (
+            FLUX_NAMES.get(args.flux, str(args.flux)), pname)),
+        "fluxes": others,
+        "fp32_flux": fp32,
         "copy_probe_gbs": copy_gbs,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    solver.close()
-    del solver
+    if fp32 is not None:
+        fp32["roofline_frac_330B"] = fp32.pop("frac")
+        fp32["error_vs_reference_fp64"] = parity.get(
+            "%s_fp32_flux" % FLUX_NAMES.get(args.flux, str(args.flux)))
+    if not args.no_batched:
+        try:
+            out["batched_sample"] = batched_run(
+                args.instances, (args.min_cells or 2e3,
+                                 args.max_cells or 5e5), args.batch_steps,
+                args.hbm_gb)
+        except Exception as e:   # must not hide the main result
+            log("batched sample failed:", e)
+            out["batched_sample"] = {"error": str(e)}
     if not args.no_served:
-        # a short served sample beside the stage benchmark
+        # a short served sample over the full request size range
         try:
             out["served_sample"] = served_run(
-                24, (args.min_cells or 1e4, args.max_cells or 6e4), 40.0,
-                min(args.max_steps, 8000), 16)
+                args.served_requests, (args.min_cells or 1e4,
+                                       args.max_cells or 2e6),
+                args.served_rate, min(args.max_steps, 8000), 32)
+            out["served_sample"]["note"] = (
+                "%d requests (the full stream has 2000) over the full 1e4 - "
+                "2e6 cell range" % args.served_requests)
         except Exception as e:   # must not hide the main result
             log("served sample failed:", e)
             out["served_sample"] = {"error": str(e)}

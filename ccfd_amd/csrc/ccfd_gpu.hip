@@ -100,6 +100,48 @@ void ccfd::launch_flux_code<float>(int code, const OpArgs<float>& a, int nb,
   kFluxF[code](a, nb, st);
 }
 
+#define DECL_JAC1(k)                                                         \
+  void ccfd_launch_jac1_##k##_d(const OpArgs<double>&, const double*, double,  \
+                                double, const Eq<double>&, double*, double*,   \
+                                hipStream_t);                                  \
+  void ccfd_launch_jac1_##k##_f(const OpArgs<float>&, const double*, double,   \
+                                double, const Eq<double>&, float*, float*,     \
+                                hipStream_t);
+DECL_JAC1(1) DECL_JAC1(2) DECL_JAC1(3) DECL_JAC1(4) DECL_JAC1(5) DECL_JAC1(6)
+DECL_JAC1(7) DECL_JAC1(8) DECL_JAC1(9) DECL_JAC1(10) DECL_JAC1(11)
+
+typedef void (*jac1_fn_d)(const OpArgs<double>&, const double*, double, double,
+                          const Eq<double>&, double*, double*, hipStream_t);
+typedef void (*jac1_fn_f)(const OpArgs<float>&, const double*, double, double,
+                          const Eq<double>&, float*, float*, hipStream_t);
+static const jac1_fn_d kJac1D[12] = {
+    nullptr, ccfd_launch_jac1_1_d, ccfd_launch_jac1_2_d, ccfd_launch_jac1_3_d,
+    ccfd_launch_jac1_4_d, ccfd_launch_jac1_5_d, ccfd_launch_jac1_6_d,
+    ccfd_launch_jac1_7_d, ccfd_launch_jac1_8_d, ccfd_launch_jac1_9_d,
+    ccfd_launch_jac1_10_d, ccfd_launch_jac1_11_d};
+static const jac1_fn_f kJac1F[12] = {
+    nullptr, ccfd_launch_jac1_1_f, ccfd_launch_jac1_2_f, ccfd_launch_jac1_3_f,
+    ccfd_launch_jac1_4_f, ccfd_launch_jac1_5_f, ccfd_launch_jac1_6_f,
+    ccfd_launch_jac1_7_f, ccfd_launch_jac1_8_f, ccfd_launch_jac1_9_f,
+    ccfd_launch_jac1_10_f, ccfd_launch_jac1_11_f};
+
+template <>
+void ccfd::launch_jac1_code<double, double>(int code, const OpArgs<double>& a,
+                                            const double* X, double reps,
+                                            double srEps, const Eq<double>& qd,
+                                            double* Dg, double* Blk,
+                                            hipStream_t st) {
+  kJac1D[code](a, X, reps, srEps, qd, Dg, Blk, st);
+}
+template <>
+void ccfd::launch_jac1_code<float, float>(int code, const OpArgs<float>& a,
+                                          const double* X, double reps,
+                                          double srEps, const Eq<double>& qd,
+                                          float* Dg, float* Blk,
+                                          hipStream_t st) {
+  kJac1F[code](a, X, reps, srEps, qd, Dg, Blk, st);
+}
+
 // --------------------------------------------------------------- kernels
 
 // prim (double, staging) -> prim_T and U = primCons(prim)
@@ -218,6 +260,31 @@ __global__ void k_dt_final(int nb, const double* __restrict__ part,
     scal[2] = vd;
     scal[3] = fmax(scal[3], sf[0]);
   }
+}
+
+// first stage of the residual reduction for many blocks: block b of
+// kResMid sums a contiguous share of each variable's nb partials into
+// mid[v * kResMid + b] (fixed order)
+constexpr int kResMid = 64;
+__global__ __launch_bounds__(256) void k_res_mid(int nb,
+                                                 const double* __restrict__ part,
+                                                 double* __restrict__ mid) {
+  __shared__ double s4[4][256];
+  const int per = (nb + kResMid - 1) / kResMid;
+  const int k0 = blockIdx.x * per, k1 = min(nb, k0 + per);
+  for (int v = 0; v < 4; ++v) {
+    double acc = 0.0;
+    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x)
+      acc += part[v * nb + k];
+    s4[v][threadIdx.x] = acc;
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int v = 0; v < 4; ++v) s4[v][threadIdx.x] += s4[v][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) mid[threadIdx.x * kResMid + blockIdx.x] = s4[threadIdx.x][0];
 }
 
 // globalResidual final reduction (src/analyze.c:534-538); res -> scal[4..7]
@@ -527,7 +594,8 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   c->ghostP = dmalloc(c, 4 * (size_t)std::max(G, 1) * 8);
   c->ghostS = dmalloc(c, 4 * (size_t)std::max(G, 1) * 8);
   c->stage = (double*)dmalloc(c, std::max(e4, 4 * (size_t)S * Np) * 8);
-  c->part = (double*)dmalloc(c, 4 * (size_t)c->nb * 8 + 64);
+  // per-block partials (4 per block) and the residual's middle stage
+  c->part = (double*)dmalloc(c, (4 * (size_t)c->nb + 4 * kResMid + 8) * 8);
   c->scal = (double*)dmalloc(c, 64 * 8);
   if (!c->prim[0] || !c->prim[1] || !c->U || !c->U0 || !c->ut || !c->ux ||
       !c->uy || !c->side || !c->ghostP || !c->ghostS || !c->stage || !c->part || !c->scal)
@@ -913,6 +981,21 @@ extern "C" int ccfd_gpu_calc_dt(ccfd_gpu_ctx* c, double t, double pTime,
 }
 
 // one explicit step; dt from the host (dtdev == nullptr) or the device
+// globalResidual from the per-block partials in c->part: one workgroup
+// for few blocks, else a middle stage of kResMid blocks first
+static void launch_res_final(ccfd_gpu_ctx* c, int advanceT) {
+  if (c->nb <= 4 * kResMid) {
+    hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->st, c->nb,
+                       c->part, c->totalArea_q, c->scal, advanceT);
+    return;
+  }
+  double* mid = c->part + 4 * (size_t)c->nb;
+  hipLaunchKernelGGL(k_res_mid, dim3(kResMid), dim3(256), 0, c->st, c->nb,
+                     c->part, mid);
+  hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->st, kResMid, mid,
+                     c->totalArea_q, c->scal, advanceT);
+}
+
 template <typename T>
 static void step_t(ccfd_gpu_ctx* c, double t, double dt, bool devDt, bool res) {
   std::swap(c->U, c->U0);   // U0 <- current state
@@ -944,8 +1027,7 @@ static void step_t(ccfd_gpu_ctx* c, double t, double dt, bool devDt, bool res) {
     c->cur = 1 - c->cur;
   }
   if (res)
-    hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->st, c->nb, c->part,
-                       c->totalArea_q, c->scal, devDt ? 1 : 0);
+    launch_res_final(c, devDt ? 1 : 0);
   else if (devDt)
     hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, c->st, c->scal);
 }
@@ -977,9 +1059,11 @@ extern "C" int ccfd_gpu_advance(ccfd_gpu_ctx* c, int nSteps, double* t,
   c->hscal[0] = *t;
   hipMemcpyAsync(c->scal, c->hscal, 8, hipMemcpyHostToDevice, c->st);
   hipMemsetAsync(c->scal + 3, 0, 8, c->st);
+  // each step: calcTimeStep, the RK stages and, when res is requested,
+  // globalResidual (src/timeDiscretization.c:362-401), all on the device
   for (int s = 0; s < nSteps; ++s) {
     launch_dt(c, 1, 0.0, pTime, stopTime);
-    bool r = res && s == nSteps - 1;
+    const bool r = res != nullptr;
     if (c->prm.precision == CCFD_FP64) step_t<double>(c, 0.0, 0.0, true, r);
     else step_t<float>(c, 0.0, 0.0, true, r);
   }
@@ -1025,8 +1109,7 @@ int ccfd::residual_of(ccfd_gpu_ctx* c, bool fp64, double res[4]) {
   else
     hipLaunchKernelGGL(k_res_partial<float>, dim3(c->nb), dim3(kBlock), 0, c->st,
                        c->N, c->Np, (const float*)c->ut, c->area, c->part);
-  hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->st, c->nb, c->part,
-                     c->totalArea_q, c->scal, 0);
+  launch_res_final(c, 0);
   if (hipMemcpyAsync(c->hscal, c->scal, 8 * 8, hipMemcpyDeviceToHost, c->st) !=
           hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
@@ -1153,8 +1236,8 @@ extern "C" int ccfd_gpu_riemann_batch(int code, int precision, double gam,
 }
 
 extern "C" int ccfd_gpu_profile_steps(ccfd_gpu_ctx* c, int nSteps, double pTime,
-                                     double stopTime, double* ms,
-                                     int* launches) {
+                                     double stopTime, int residual, double* ms,
+                                     int* launches, double res[4]) {
   if (!c || !c->hasMesh || !ms || nSteps <= 0)
     return fail(CCFD_EINVAL, "bad profile arguments");
   hipSetDevice(c->device);
@@ -1164,7 +1247,9 @@ extern "C" int ccfd_gpu_profile_steps(ccfd_gpu_ctx* c, int nSteps, double pTime,
   double t = 0.0;
   hipEvent_t all0 = take_event(c), all1 = nullptr;
   hipEventRecord(all0, c->st);
-  int rc = ccfd_gpu_advance(c, nSteps, &t, pTime, stopTime, nullptr);
+  double r4[4];
+  int rc = ccfd_gpu_advance(c, nSteps, &t, pTime, stopTime,
+                            residual ? r4 : nullptr);
   all1 = take_event(c);
   hipEventRecord(all1, c->st);
   c->timing = false;
@@ -1185,6 +1270,8 @@ extern "C" int ccfd_gpu_profile_steps(ccfd_gpu_ctx* c, int nSteps, double pTime,
     if (launches) launches[k] = (int)n[k];
   }
   ms[KT_N] = tot / nSteps;
+  if (residual && res)
+    for (int v = 0; v < 4; ++v) res[v] = r4[v];
   c->spans.clear();
   c->evUsed = 0;
   return CCFD_OK;

@@ -4,9 +4,9 @@
 //
 //  * Newton loop on the host, all vectors device-resident ([4][Np], FP64).
 //  * GMRES_M: one cycle of at most nKdim iterations (optionally restarted
-//    cycles), modified Gram-Schmidt with H on the device, Givens rotations
-//    and back-solve on the host in FP64. Krylov vectors V, Z are FP64, or
-//    FP32 in FP32-flux mode.
+//    cycles); modified Gram-Schmidt, Givens rotations and back-solve in
+//    FP64 on the device, with no host round trip per iteration. Krylov
+//    vectors V, Z are FP64, or FP32 in FP32-flux mode.
 //  * matrixVector: Frechet difference with epsFD = sqrt(eps)/|v| in FP64
 //    (eps of the operator's arithmetic).
 //  * buildMatrix: the reference records only the diagonal and face-neighbour
@@ -31,6 +31,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "agglomerate.hpp"
@@ -91,7 +92,19 @@ struct ImplicitState {
   int* flag = nullptr;          // singular-block flag
   double* dtLoc = nullptr;      // [Np] local pseudo-time steps (steady solves)
   double gmresRel = 0.0;        // |residual| / |rhs| the last GMRES call reached
+  // device-resident Arnoldi (GmresDev): H, rotations, gm, alp, scalars
+  double* gdev = nullptr;
+  double* partA = nullptr;      // [kRedBlocks] partial sums, ping-pong
+  double* partB = nullptr;
+  double* cgsPart = nullptr;    // [nK + 1][kRedBlocks] CGS2 dot partials
+  double* cgsH = nullptr;       // [nK + 8] one CGS2 pass's coefficients
+  int* hflag = nullptr;         // pinned: per iteration, 1 = cycle done
+  int* hflagDev = nullptr;      // its device address
+  std::vector<hipEvent_t> ev;   // end of each queued iteration
   double steadyCfl = 0.0;       // CFL number a steady solve stopped at
+  // first-order conservative Jacobian: 1 = element by element (jac1_kernel),
+  // 0 = coloured finite differences (the general path)
+  int jacMethod = 1;
   std::vector<void*> allocs;
 };
 
@@ -108,6 +121,8 @@ static void* imalloc(ccfd_gpu_ctx* c, ImplicitState* s, size_t bytes) {
 void ccfd::implicit_free(ccfd_gpu_ctx* c) {
   if (!c->imp) return;
   for (void* p : c->imp->allocs) hipFree(p);
+  for (hipEvent_t e : c->imp->ev) hipEventDestroy(e);
+  if (c->imp->hflag) hipHostFree(c->imp->hflag);
   delete c->imp;
   c->imp = nullptr;
 }
@@ -182,58 +197,50 @@ __global__ void k_neg_div(int N, int Np, const double* __restrict__ a,
   for (int v = 0; v < 4; ++v) out[v * Np + i] = K(-a[v * Np + i] / s);
 }
 
-// dot of a K-vector with an FP64 vector (Arnoldi in FP64), block partials
-template <typename K>
-__global__ __launch_bounds__(256) void k_dotk_partial(int N, int Np,
-                                                      const K* __restrict__ a,
-                                                      const double* __restrict__ b,
-                                                      double* part) {
-  __shared__ double red[4];
-  int i = blockIdx.x * 256 + threadIdx.x;
-  double x = 0.0;
-  if (i < N) {
-    x += double(a[i]) * b[i];
-    x += double(a[Np + i]) * b[Np + i];
-    x += double(a[2 * Np + i]) * b[2 * Np + i];
-    x += double(a[3 * Np + i]) * b[3 * Np + i];
-  }
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-}
+// ---------------------------------------------- device-resident Arnoldi
+//
+// One GMRES_M iteration (src/linearSolver.c:422-468) runs without a host
+// round trip: the Hessenberg matrix, the Givens rotations, the residual
+// estimate and the back-solve live in device memory (GmresDev), every
+// reduction is finished on the device, and the host only learns at the
+// end (or, lagging behind, while it queues further iterations) whether
+// the abort criterion was met. Kernels of an iteration queued after that
+// do nothing (they read the done flag first).
+//
+// Dot products use a fixed grid of kRedBlocks blocks striding over the
+// elements, so the partial sums of every product come in the same fixed
+// order; the kernels that need a product reduce its partials themselves,
+// in the same order in every block.
 
-__global__ void k_sum_to(int nb, const double* __restrict__ part, double* out,
-                         int sqrt_out) {
-  __shared__ double s[256];
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) acc += part[k];
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = sqrt_out ? sqrt(s[0]) : s[0];
-}
+constexpr int kRedBlocks = 512;   // blocks of the fixed reduction grid
 
-// W -= h * Vn with h on the device (modified Gram-Schmidt step)
-template <typename K>
-__global__ void k_axpy_dev(int N, int Np, const double* __restrict__ h,
-                           const K* __restrict__ Vn, double* __restrict__ W) {
-  FOR_CELLS
-  const double hh = *h;
-  for (int v = 0; v < 4; ++v) W[v * Np + i] -= hh * double(Vn[v * Np + i]);
-}
+struct GmresDev {
+  // H(i, j) = H[i * ld + j], ld = nK + 1
+  double* H;
+  double* cs;     // [nK] Givens cosines
+  double* sn;     // [nK] sines
+  double* gm;     // [nK + 1] rotated right-hand side
+  double* alp;    // [nK + 1] back-solve coefficients
+  double* sc;     // scalars, see below
+  int ld;
+};
+// scalars of GmresDev::sc
+enum { GS_DONE = 0, GS_MSTAR, GS_ABORT, GS_NORMB, GS_HNEXT, GS_REL, GS_VV,
+       GS_N };
+
+__device__ inline bool gm_done(const double* sc) { return sc[GS_DONE] != 0.0; }
 
 // matrixVector perturbation: epsFD = sqrt(eps) / sqrt(vv); U = XK + epsFD v;
 // prim = consPrim(U) (src/linearSolver.c:346-360)
+// (nothing once the GMRES cycle is done: sc[GS_DONE], sc may be null)
 template <typename T, typename K>
 __global__ void k_jv_perturb(int N, int Np, const double* __restrict__ vv,
                              double sqeps, const double* __restrict__ XK,
                              const K* __restrict__ v, double* __restrict__ U,
-                             T* __restrict__ prim, double* epsOut, Eq<double> q) {
+                             T* __restrict__ prim, double* epsOut, Eq<double> q,
+                             const double* sc) {
   FOR_CELLS
+  if (sc && gm_done(sc)) return;
   const double eps = sqeps / sqrt(*vv);
   if (i == 0) *epsOut = eps;
   double c[4], p[4];
@@ -257,8 +264,9 @@ __global__ void k_jv_diff(int N, int Np, const K* __restrict__ v,
                           const double* __restrict__ epsp, double adt,
                           const double* __restrict__ dtv, double alpha,
                           const double* __restrict__ area,
-                          double* __restrict__ res) {
+                          double* __restrict__ res, const double* sc) {
   FOR_CELLS
+  if (sc && gm_done(sc)) return;
   const double eps = *epsp;
   const double a = elem_adt(adt, dtv, alpha, i);
   const double w = dtv ? sqrt(area[i]) : 1.0;
@@ -269,12 +277,242 @@ __global__ void k_jv_diff(int N, int Np, const K* __restrict__ v,
   }
 }
 
-// dX += sum_n alp[n] Z_n (src/linearSolver.c:483-489)
+
+// sum of nr partials, the same fixed-order tree in every block; the result
+// is returned to every thread (red: >= 256 doubles of shared memory)
+__device__ inline double block_sum_partials(const double* __restrict__ part,
+                                            int nr, double* red) {
+  double x = 0.0;
+  for (int k = threadIdx.x; k < nr; k += blockDim.x) x += part[k];
+  red[threadIdx.x] = x;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  const double s = red[0];
+  __syncthreads();
+  return s;
+}
+
+// block partial of sum x (256 threads) into part[blockIdx.x]
+__device__ inline void block_partial(double x, double* red, double* part) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// start of a cycle: H, gm and the flags cleared, gm[0] = beta, the abort
+// criterion epsGMRES |B| (src/linearSolver.c:388)
+__global__ void k_gm_init(GmresDev g, int nK, double beta, double abortCrit,
+                          double normB) {
+  for (int k = threadIdx.x; k < (nK + 1) * g.ld; k += blockDim.x) g.H[k] = 0.0;
+  for (int k = threadIdx.x; k <= nK; k += blockDim.x) {
+    g.gm[k] = k == 0 ? beta : 0.0;
+    g.alp[k] = 0.0;
+  }
+  if (threadIdx.x == 0) {
+    g.sc[GS_DONE] = 0.0;
+    g.sc[GS_MSTAR] = -1.0;
+    g.sc[GS_ABORT] = abortCrit;
+    g.sc[GS_NORMB] = normB;
+    g.sc[GS_REL] = 0.0;
+  }
+}
+
+// partials of |v|^2 (FP64 products of the basis values) for the Frechet
+// step of matrixVector (src/linearSolver.c:346-347)
 template <typename K>
-__global__ void k_update_dx(int N, int Np, int m, const double* __restrict__ alp,
+__global__ __launch_bounds__(256) void k_norm_partial(int N, int Np,
+                                                      const K* __restrict__ v,
+                                                      const double* sc,
+                                                      double* __restrict__ part) {
+  __shared__ double red[4];
+  if (gm_done(sc)) return;
+  double x = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+    for (int k = 0; k < 4; ++k) {
+      const double a = double(v[k * Np + i]);
+      x += a * a;
+    }
+  block_partial(x, red, part);
+}
+
+// sum of the partials into sc[GS_VV]
+__global__ __launch_bounds__(256) void k_norm_final(const double* __restrict__ part,
+                                                    int nr, double* sc) {
+  __shared__ double red[256];
+  if (gm_done(sc)) return;
+  const double s = block_sum_partials(part, nr, red);
+  if (threadIdx.x == 0) sc[GS_VV] = s;
+}
+
+// first Gram-Schmidt dot: partials of V_0 . W
+template <typename K>
+__global__ __launch_bounds__(256) void k_mgs_first(int N, int Np,
+                                                   const K* __restrict__ V0,
+                                                   const double* __restrict__ W,
+                                                   const double* sc,
+                                                   double* __restrict__ part) {
+  __shared__ double red[4];
+  if (gm_done(sc)) return;
+  double x = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+    for (int k = 0; k < 4; ++k)
+      x += double(V0[k * Np + i]) * W[k * Np + i];
+  block_partial(x, red, part);
+}
+
+// modified Gram-Schmidt step n of iteration m (src/linearSolver.c:440-450):
+// h = <V_n, W> from the partials of the previous launch, H(n, m) = h,
+// W -= h V_n, then the partials of the next product: <V_{n+1}, W>, or
+// <W, W> after the last step (n = m)
+template <typename K>
+__global__ __launch_bounds__(256) void k_mgs_step(
+    int N, int Np, int n, int m, const K* __restrict__ V, size_t vs,
+    double* __restrict__ W, const double* __restrict__ partIn,
+    double* __restrict__ partOut, GmresDev g) {
+  __shared__ double red[256];
+  if (gm_done(g.sc)) return;
+  const double h = block_sum_partials(partIn, gridDim.x, red);
+  if (blockIdx.x == 0 && threadIdx.x == 0) g.H[n * g.ld + m] = h;
+  const K* Vn = V + n * vs;
+  const K* Vx = n < m ? V + (n + 1) * vs : nullptr;
+  double x = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+    for (int k = 0; k < 4; ++k) {
+      const int o = k * Np + i;
+      const double w = W[o] - h * double(Vn[o]);
+      W[o] = w;
+      x += Vx ? double(Vx[o]) * w : w * w;
+    }
+  block_partial(x, red, partOut);
+}
+
+// H(m+1, m) = |W|, the previous rotations applied to column m, a new
+// rotation, the rotated right-hand side (src/linearSolver.c:451-468); at the
+// abort criterion or the last iteration the back-solve for alp
+// (src/linearSolver.c:470-482) and the done flag. One block of 256.
+__global__ __launch_bounds__(256) void k_givens(GmresDev g, int m, int nK,
+                                                const double* __restrict__ part,
+                                                int nr, int* hflag) {
+  __shared__ double red[256];
+  if (gm_done(g.sc)) return;
+  const double ww = block_sum_partials(part, nr, red);
+  if (threadIdx.x != 0) return;
+  double* H = g.H;
+  const int ld = g.ld;
+#define HM(i, j) H[(i) * ld + (j)]
+  const double hnext = sqrt(ww);
+  HM(m + 1, m) = hnext;
+  g.sc[GS_HNEXT] = hnext;
+  for (int n = 0; n <= m - 1; ++n) {
+    const double tmp = g.cs[n] * HM(n, m) + g.sn[n] * HM(n + 1, m);
+    HM(n + 1, m) = -g.sn[n] * HM(n, m) + g.cs[n] * HM(n + 1, m);
+    HM(n, m) = tmp;
+  }
+  const double bet = sqrt(HM(m, m) * HM(m, m) + HM(m + 1, m) * HM(m + 1, m));
+  g.sn[m] = HM(m + 1, m) / bet;
+  g.cs[m] = HM(m, m) / bet;
+  HM(m, m) = bet;
+  g.gm[m + 1] = -g.sn[m] * g.gm[m];
+  g.gm[m] = g.cs[m] * g.gm[m];
+  const bool conv = fabs(g.gm[m + 1]) <= g.sc[GS_ABORT];
+  if (conv || m == nK - 1) {
+    for (int n = m; n >= 0; --n) {
+      double a = g.gm[n];
+      for (int o = n + 1; o <= m; ++o) a -= HM(n, o) * g.alp[o];
+      g.alp[n] = a / HM(n, n);
+    }
+    const double nb = g.sc[GS_NORMB];
+    g.sc[GS_REL] = nb > 0.0 ? fabs(g.gm[m + 1]) / nb : 0.0;
+    g.sc[GS_MSTAR] = m;
+    g.sc[GS_DONE] = conv ? 2.0 : 1.0;
+  }
+  // the host polls this (pinned) word
+  if (hflag)
+    __hip_atomic_store(hflag + m, conv || m == nK - 1 ? 1 : 0,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+#undef HM
+}
+
+// classical Gram-Schmidt pass over V_0..V_m (steady solves, run twice:
+// CGS2): partials of <V_n, W> for every n, part[n * gridDim.x + block]
+template <typename K>
+__global__ __launch_bounds__(256) void k_cgs_dots(int N, int Np, int m,
+                                                  const K* __restrict__ V,
+                                                  size_t vs,
+                                                  const double* __restrict__ W,
+                                                  const double* sc,
+                                                  double* __restrict__ part) {
+  __shared__ double red[4];
+  if (gm_done(sc)) return;
+  for (int n = 0; n <= m; ++n) {
+    const K* Vn = V + n * vs;
+    double x = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+      for (int k = 0; k < 4; ++k) x += double(Vn[k * Np + i]) * W[k * Np + i];
+    block_partial(x, red, part + (size_t)n * gridDim.x);
+    __syncthreads();
+  }
+}
+
+// h_n = sum of the partials of <V_n, W> (block n of m + 1); the first pass
+// stores H(n, m) = h_n, the second adds its correction
+__global__ __launch_bounds__(256) void k_cgs_sum(int m, int nr,
+                                                 const double* __restrict__ part,
+                                                 double* __restrict__ h,
+                                                 GmresDev g, int second) {
+  __shared__ double red[256];
+  if (gm_done(g.sc)) return;
+  const int n = blockIdx.x;
+  const double x = block_sum_partials(part + (size_t)n * nr, nr, red);
+  if (threadIdx.x == 0) {
+    h[n] = x;
+    if (second) g.H[n * g.ld + m] += x; else g.H[n * g.ld + m] = x;
+  }
+}
+
+// W -= sum_n h_n V_n; with wwPart, the partials of |W|^2 afterwards
+template <typename K>
+__global__ __launch_bounds__(256) void k_cgs_update(
+    int N, int Np, int m, const K* __restrict__ V, size_t vs,
+    const double* __restrict__ h, double* __restrict__ W, const double* sc,
+    double* __restrict__ wwPart) {
+  __shared__ double red[4];
+  if (gm_done(sc)) return;
+  double x = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+    for (int k = 0; k < 4; ++k) {
+      const int o = k * Np + i;
+      double w = W[o];
+      for (int n = 0; n <= m; ++n) w -= h[n] * double(V[n * vs + o]);
+      W[o] = w;
+      x += w * w;
+    }
+  if (wwPart) block_partial(x, red, wwPart);
+}
+
+// V_{m+1} = W / H(m+1, m) (src/linearSolver.c:492-494)
+template <typename K>
+__global__ void k_scale_dev(int N, int Np, const double* __restrict__ W,
+                            K* __restrict__ out, const double* sc) {
+  FOR_CELLS
+  if (gm_done(sc)) return;
+  const double h = sc[GS_HNEXT];
+  for (int v = 0; v < 4; ++v) out[v * Np + i] = K(W[v * Np + i] / h);
+}
+
+// dX += sum_{n <= m*} alp[n] Z_n (src/linearSolver.c:483-489), m* and alp
+// from the device
+template <typename K>
+__global__ void k_update_dx(int N, int Np, const double* __restrict__ sc,
+                            const double* __restrict__ alp,
                             const K* __restrict__ Z, size_t zstride,
                             double* __restrict__ dX) {
   FOR_CELLS
+  const int m = (int)sc[GS_MSTAR];
   for (int v = 0; v < 4; ++v) {
     double acc = dX[v * Np + i];
     for (int n = 0; n <= m; ++n)
@@ -283,9 +521,12 @@ __global__ void k_update_dx(int N, int Np, int m, const double* __restrict__ alp
   }
 }
 
+// x += w d, w from the device when wdev is set
 __global__ void k_axpy(int N, int Np, double* __restrict__ x,
-                       const double* __restrict__ d, double w) {
+                       const double* __restrict__ d, double w,
+                       const double* wdev = nullptr) {
   FOR_CELLS
+  if (wdev) w = *wdev;
   if (w == 1.0)
     for (int v = 0; v < 4; ++v) x[v * Np + i] += d[v * Np + i];
   else
@@ -319,8 +560,12 @@ __global__ __launch_bounds__(256) void k_relchange_partial(
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// out = max of the partials; with wout, the update weight of a bounded
+// Newton update, w = maxChange / max where the max exceeds maxChange (else
+// 1), and wmin = min(wmin, w)
 __global__ void k_max_final(int nb, const double* __restrict__ part,
-                            double* out) {
+                            double* out, double maxChange = 0.0,
+                            double* wout = nullptr, double* wmin = nullptr) {
   __shared__ double sm[256];
   double r = 0.0;
   for (int k = threadIdx.x; k < nb; k += blockDim.x) r = fmax(r, part[k]);
@@ -330,8 +575,17 @@ __global__ void k_max_final(int nb, const double* __restrict__ part,
     if (threadIdx.x < st) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + st]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sm[0];
+  if (threadIdx.x == 0) {
+    *out = sm[0];
+    if (wout) {
+      const double w = sm[0] > maxChange ? maxChange / sm[0] : 1.0;
+      *wout = w;
+      *wmin = fmin(*wmin, w);
+    }
+  }
 }
+
+__global__ void k_set1(double* x, double v) { *x = v; }
 
 // ---- LU-SGS assembly (coloured finite differences) ----
 
@@ -751,6 +1005,29 @@ __global__ void k_local_dt(int N, int Np, const T* __restrict__ prim,
 
 static dim3 cells_grid(const ccfd_gpu_ctx* c) { return dim3((c->N + 255) / 256); }
 
+// doubles of the GmresDev buffer for Krylov dimension nK
+static size_t gmres_dev_doubles(int nK) {
+  return (size_t)(nK + 1) * (nK + 1) + 2 * (size_t)nK + 2 * (size_t)(nK + 1) +
+         GS_N;
+}
+
+static GmresDev gmres_view(double* b, int nK) {
+  GmresDev g;
+  g.ld = nK + 1;
+  g.H = b;
+  g.cs = g.H + (size_t)(nK + 1) * (nK + 1);
+  g.sn = g.cs + nK;
+  g.gm = g.sn + nK;
+  g.alp = g.gm + nK + 1;
+  g.sc = g.alp + nK + 1;
+  return g;
+}
+
+// blocks of the fixed reduction grid for N elements
+static int red_blocks(const ccfd_gpu_ctx* c) {
+  return std::max(1, std::min(kRedBlocks, (c->N + 255) / 256));
+}
+
 static int ensure(ccfd_gpu_ctx* c, int nK) {
   ImplicitState* s = c->imp;
   int prec = c->prm.precision;
@@ -773,9 +1050,23 @@ static int ensure(ccfd_gpu_ctx* c, int nK) {
   s->H = (double*)imalloc(c, s, (nK + 8) * 8);
   s->alp = (double*)imalloc(c, s, (nK + 8) * 8);
   s->part = (double*)imalloc(c, s, (c->nb + 64) * 8);
+  s->gdev = (double*)imalloc(c, s, gmres_dev_doubles(nK) * 8);
+  s->partA = (double*)imalloc(c, s, kRedBlocks * 8);
+  s->partB = (double*)imalloc(c, s, kRedBlocks * 8);
+  s->cgsPart = (double*)imalloc(c, s, (size_t)(nK + 1) * kRedBlocks * 8);
+  s->cgsH = (double*)imalloc(c, s, (nK + 8) * 8);
   if (!s->XK || !s->RXK || !s->Q || !s->FXK || !s->dX || !s->W || !s->B ||
-      !s->V || !s->Z || !s->H || !s->alp || !s->part)
+      !s->V || !s->Z || !s->H || !s->alp || !s->part || !s->gdev ||
+      !s->partA || !s->partB || !s->cgsPart || !s->cgsH)
     return fail(CCFD_ENOMEM, "implicit solver allocation failed");
+  if (hipHostMalloc((void**)&s->hflag, (nK + 8) * sizeof(int),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&s->hflagDev, s->hflag, 0) != hipSuccess)
+    return fail(CCFD_ENOMEM, "pinned GMRES flags allocation failed");
+  s->ev.resize(nK);
+  for (auto& e : s->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(CCFD_EHIP, "GMRES event creation failed");
   return CCFD_OK;
 }
 
@@ -1142,26 +1433,44 @@ static int build_matrix(ccfd_gpu_ctx* c, double tGlobal, double dt,
                          (T*)c->prim[c->cur]);
   };
   const int opOrder = c->prm.spatialOrder;
-  const double* base = s->RXK;
-  if (jacOrder != opOrder) {
-    // the first-order operator (split kernels, zero gradients) at XK
-    c->prm.spatialOrder = jacOrder;
-    perturb(-1, 0);
-    operator_at<T>(c, tGlobal);
-    hipLaunchKernelGGL((k_copy_cvt<T, double>), dim3((4 * Np + 255) / 256),
-                       dim3(256), 0, c->st, 4 * (long)Np, (const T*)c->ut, s->W);
-    base = s->W;
-  }
-  for (int col = 0; col < s->nColors; ++col) {
-    for (int iv = 0; iv < 4; ++iv) {
-      perturb(col, iv);
+  // (the operator and matrix types always agree when this runs; the other
+  // instantiations never take the branch)
+  if (std::is_same<T, M>::value && consJac && jacOrder == 1 &&
+      s->jacMethod == 1) {
+    // first-order Jacobian of the conservative state, element by element in
+    // one launch (jac1_kernel): the same blocks as the coloured differences
+    c->prm.spatialOrder = 1;
+    OpArgs<T> a = op_args<T>(c);
+    a.time0 = tGlobal;
+    c->prm.spatialOrder = opOrder;
+    if (c->hasExact && c->G > 0)
+      hipLaunchKernelGGL(bc_exact_kernel<T>, dim3((c->G + 127) / 128),
+                         dim3(128), 0, c->st, a, 1);
+    if constexpr (std::is_same<T, M>::value)
+      launch_jac1_code<T, T>(c->prm.iFlux, a, s->XK, reps, srEps, eqd(c),
+                             (T*)s->Dg, (T*)s->Blk, c->st);
+  } else {
+    const double* base = s->RXK;
+    if (jacOrder != opOrder) {
+      // the first-order operator (split kernels, zero gradients) at XK
+      c->prm.spatialOrder = jacOrder;
+      perturb(-1, 0);
       operator_at<T>(c, tGlobal);
-      hipLaunchKernelGGL((k_fd_record<T, M>), cells_grid(c), dim3(256), 0,
-                         c->st, N, Np, S, c->link, s->color, col, iv,
-                         (const T*)c->ut, base, srEps, (M*)s->Dg, (M*)s->Blk);
+      hipLaunchKernelGGL((k_copy_cvt<T, double>), dim3((4 * Np + 255) / 256),
+                         dim3(256), 0, c->st, 4 * (long)Np, (const T*)c->ut, s->W);
+      base = s->W;
     }
+    for (int col = 0; col < s->nColors; ++col) {
+      for (int iv = 0; iv < 4; ++iv) {
+        perturb(col, iv);
+        operator_at<T>(c, tGlobal);
+        hipLaunchKernelGGL((k_fd_record<T, M>), cells_grid(c), dim3(256), 0,
+                           c->st, N, Np, S, c->link, s->color, col, iv,
+                           (const T*)c->ut, base, srEps, (M*)s->Dg, (M*)s->Blk);
+      }
+    }
+    c->prm.spatialOrder = opOrder;
   }
-  c->prm.spatialOrder = opOrder;
   hipLaunchKernelGGL(k_fd_finish<M>, cells_grid(c), dim3(256), 0, c->st, N, Np,
                      S, c->link, dt, dtv, c->area, (M*)s->Dg, (M*)s->Blk,
                      (M*)s->Dinv, s->flag);
@@ -1302,10 +1611,20 @@ static void lusgs(ccfd_gpu_ctx* c, int mode, int sweeps, const K* b, K* x) {
 // GMRES_M (src/linearSolver.c:385-510) with B = s->B, result in s->dX.
 // cfg->cycles > 1 restarts it (each cycle from the true residual
 // -B - J dX) until the tolerance or the cycle count; one cycle is GMRES_M.
+//
+// The iterations are queued without waiting for each other: the
+// orthogonalisation, the Givens rotations and the back-solve run on the
+// device (k_mgs_*, k_givens), and the host waits only for the iteration
+// kLookahead places behind the one it queues next, reading the done flag
+// k_givens leaves in pinned memory. Iterations queued after the abort
+// criterion was met find the flag set and do no Arnoldi work, so the
+// result is that of stopping at the first iteration that met it.
+constexpr int kLookahead = 2;
+
 template <typename T, typename K>
 static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
                  double tGlobal, double time, double dt, const double* dtv,
-                 double alpha, double normB, int* nIter) {
+                 double alpha, double normB, int* nIter, bool cgs2) {
   ImplicitState* s = c->imp;
   const int N = c->N, Np = c->Np, nK = cfg->nKdim;
   const size_t vs = 4 * (size_t)Np;
@@ -1315,9 +1634,9 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
   const double sqeps = c->prm.precision == CCFD_FP64 ? sqrt(DBL_EPSILON)
                                                      : sqrt((double)FLT_EPSILON);
   const int cycles = std::max(cfg->cycles, 1);
+  const int R = red_blocks(c);
+  const GmresDev g = gmres_view(s->gdev, nK);
   hipMemsetAsync(s->dX, 0, vs * 8, c->st);
-  std::vector<double> H((nK + 1) * (nK + 1), 0.0), gm(nK + 1, 0.0),
-      Cg(nK + 1, 0.0), Sg(nK + 1, 0.0), alp(nK + 1, 0.0);
   if (cfg->precond) {
     const bool consJac = cfg->precond >= 2;
     const int jo = jacobian_order(c, cfg);
@@ -1328,44 +1647,53 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     if (rc) return rc;
   }
   // matrixVector (src/linearSolver.c:343-373) of v into s->W; |v|^2 in FP64
-  // (v copied to W first)
-  double* vv = s->H + nK + 4;
+  // (sc null: outside a cycle, always evaluated)
   double* epsFD = s->H + nK + 5;
-  auto jv = [&](const K* v) {
-    hipLaunchKernelGGL((k_copy_cvt<K, double>), dim3((vs + 255) / 256), dim3(256),
-                       0, c->st, (long)vs, v, s->W);
-    dot_dev(c, s->W, s->W, vv);
+  auto jv = [&](const K* v, const double* sc) {
+    hipLaunchKernelGGL(k_norm_partial<K>, dim3(R), dim3(256), 0, c->st, N, Np,
+                       v, g.sc, s->partA);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, c->st, s->partA, R,
+                       g.sc);
     hipLaunchKernelGGL((k_jv_perturb<T, K>), cells_grid(c), dim3(256), 0, c->st,
-                       N, Np, vv, sqeps, s->XK, v, c->U, (T*)c->prim[c->cur],
-                       epsFD, eqd(c));
+                       N, Np, g.sc + GS_VV, sqeps, s->XK, v, c->U,
+                       (T*)c->prim[c->cur], epsFD, eqd(c), sc);
     operator_at<T>(c, time);
     hipLaunchKernelGGL((k_jv_diff<T, K>), cells_grid(c), dim3(256), 0, c->st, N,
                        Np, v, (const T*)c->ut, s->RXK, epsFD, alpha * dt, dtv,
-                       alpha, c->area, s->W);
+                       alpha, c->area, s->W, sc);
   };
-#define HM(i, j) H[(i) * (nK + 1) + (j)]
   for (int cyc = 0; cyc < cycles; ++cyc) {
     double beta = normB;
     if (cyc == 0) {
+      hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(256), 0, c->st, g, nK, beta,
+                         abortCrit, normB);
       hipLaunchKernelGGL(k_neg_div<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
                          s->B, V, normB);
     } else {
       // r = -B - J dX; V_0 = r / |r| (dX staged in V_0 for the product)
+      hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(256), 0, c->st, g, nK, 0.0,
+                         abortCrit, normB);
       hipLaunchKernelGGL((k_copy_cvt<double, K>), dim3((vs + 255) / 256),
                          dim3(256), 0, c->st, (long)vs, s->dX, V);
-      jv(V);
+      jv(V, nullptr);
       hipLaunchKernelGGL(k_axpy, cells_grid(c), dim3(256), 0, c->st, N, Np, s->W,
                          s->B, 1.0);
       beta = sqrt(dot_host(c, s->W, s->W));
       if (!(beta > abortCrit)) break;
+      hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(256), 0, c->st, g, nK, beta,
+                         abortCrit, normB);
       hipLaunchKernelGGL(k_neg_div<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
                          s->W, V, beta);
     }
-    std::fill(H.begin(), H.end(), 0.0);
-    std::fill(gm.begin(), gm.end(), 0.0);
-    gm[0] = beta;
+    for (int m = 0; m < nK + 8; ++m) s->hflag[m] = 0;
     for (int m = 0; m < nK; ++m) {
-      (*nIter)++;
+      if (m >= kLookahead) {
+        const int w = m - kLookahead;
+        if (hipEventSynchronize(s->ev[w]) != hipSuccess)
+          return fail(CCFD_EHIP, "GMRES iteration failed: %s",
+                      hipGetErrorString(hipGetLastError()));
+        if (__atomic_load_n(s->hflag + w, __ATOMIC_ACQUIRE)) break;
+      }
       K* Vm = V + m * vs;
       K* Zm = Z + m * vs;
       if (cfg->precond) {
@@ -1376,59 +1704,53 @@ static int gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
       } else {
         hipMemcpyAsync(Zm, Vm, vs * sizeof(K), hipMemcpyDeviceToDevice, c->st);
       }
-      jv(Zm);
-      // modified Gram-Schmidt (src/linearSolver.c:440-450)
-      for (int n = 0; n <= m; ++n) {
-        K* Vn = V + n * vs;
-        hipLaunchKernelGGL(k_dotk_partial<K>, dim3(c->nb), dim3(256), 0, c->st, N,
-                           Np, Vn, s->W, s->part);
-        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, c->st, c->nb, s->part,
-                           s->H + n, 0);
-        hipLaunchKernelGGL(k_axpy_dev<K>, cells_grid(c), dim3(256), 0, c->st, N,
-                           Np, s->H + n, Vn, s->W);
-      }
-      dot_dev(c, s->W, s->W, s->H + m + 1);
-      std::vector<double> col(m + 2);
-      hipMemcpyAsync(col.data(), s->H, (m + 2) * 8, hipMemcpyDeviceToHost, c->st);
-      if (hipStreamSynchronize(c->st) != hipSuccess)
-        return fail(CCFD_EHIP, "GMRES iteration failed: %s",
-                    hipGetErrorString(hipGetLastError()));
-      for (int n = 0; n <= m; ++n) HM(n, m) = col[n];
-      HM(m + 1, m) = sqrt(col[m + 1]);
-      // Givens rotations (src/linearSolver.c:456-468)
-      for (int n = 0; n <= m - 1; ++n) {
-        double tmp = Cg[n] * HM(n, m) + Sg[n] * HM(n + 1, m);
-        HM(n + 1, m) = -Sg[n] * HM(n, m) + Cg[n] * HM(n + 1, m);
-        HM(n, m) = tmp;
-      }
-      double bet = sqrt(HM(m, m) * HM(m, m) + HM(m + 1, m) * HM(m + 1, m));
-      Sg[m] = HM(m + 1, m) / bet;
-      Cg[m] = HM(m, m) / bet;
-      HM(m, m) = bet;
-      gm[m + 1] = -Sg[m] * gm[m];
-      gm[m] = Cg[m] * gm[m];
-      const bool done = fabs(gm[m + 1]) <= abortCrit;
-      if (done || m == nK - 1) {
-        s->gmresRel = normB > 0.0 ? fabs(gm[m + 1]) / normB : 0.0;
-        for (int n = m; n >= 0; --n) {
-          alp[n] = gm[n];
-          for (int o = n + 1; o <= m; ++o) alp[n] -= HM(n, o) * alp[o];
-          alp[n] /= HM(n, n);
+      jv(Zm, g.sc);
+      double* pin = s->partA;
+      if (!cgs2) {
+        // modified Gram-Schmidt (src/linearSolver.c:440-450), one launch per
+        // basis vector, the last one also giving |W|^2
+        hipLaunchKernelGGL(k_mgs_first<K>, dim3(R), dim3(256), 0, c->st, N, Np,
+                           V, s->W, g.sc, s->partA);
+        double* pout = s->partB;
+        for (int n = 0; n <= m; ++n) {
+          hipLaunchKernelGGL(k_mgs_step<K>, dim3(R), dim3(256), 0, c->st, N, Np,
+                             n, m, V, vs, s->W, pin, pout, g);
+          std::swap(pin, pout);
         }
-        hipMemcpyAsync(s->alp, alp.data(), (m + 1) * 8, hipMemcpyHostToDevice,
-                       c->st);
-        hipLaunchKernelGGL(k_update_dx<K>, cells_grid(c), dim3(256), 0, c->st, N,
-                           Np, m, s->alp, Z, vs, s->dX);
-        if (hipStreamSynchronize(c->st) != hipSuccess)
-          return fail(CCFD_EHIP, "GMRES update failed");
-        if (done) return CCFD_OK;
-        break;
+      } else {
+        // classical Gram-Schmidt twice (CGS2): three launches per pass
+        // whatever m, orthogonal to working precision as MGS is
+        for (int pass = 0; pass < 2; ++pass) {
+          hipLaunchKernelGGL(k_cgs_dots<K>, dim3(R), dim3(256), 0, c->st, N, Np,
+                             m, V, vs, s->W, g.sc, s->cgsPart);
+          hipLaunchKernelGGL(k_cgs_sum, dim3(m + 1), dim3(256), 0, c->st, m, R,
+                             s->cgsPart, s->cgsH, g, pass);
+          hipLaunchKernelGGL(k_cgs_update<K>, dim3(R), dim3(256), 0, c->st, N,
+                             Np, m, V, vs, s->cgsH, s->W, g.sc,
+                             pass ? s->partA : nullptr);
+        }
       }
-      hipLaunchKernelGGL(k_scale<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
-                         s->W, V + (m + 1) * vs, HM(m + 1, m));
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->st, g, m, nK, pin,
+                         R, s->hflagDev);
+      if (m + 1 < nK)
+        hipLaunchKernelGGL(k_scale_dev<K>, cells_grid(c), dim3(256), 0, c->st, N,
+                           Np, s->W, V + (m + 1) * vs, g.sc);
+      if (hipEventRecord(s->ev[m], c->st) != hipSuccess)
+        return fail(CCFD_EHIP, "GMRES event record failed");
     }
+    // dX += Z alp over the iterations up to the one that stopped the cycle
+    hipLaunchKernelGGL(k_update_dx<K>, cells_grid(c), dim3(256), 0, c->st, N, Np,
+                       g.sc, g.alp, Z, vs, s->dX);
+    double gsc[GS_N];
+    if (hipMemcpyAsync(gsc, g.sc, sizeof(gsc), hipMemcpyDeviceToHost, c->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      return fail(CCFD_EHIP, "GMRES update failed: %s",
+                  hipGetErrorString(hipGetLastError()));
+    *nIter += (int)gsc[GS_MSTAR] + 1;
+    s->gmresRel = gsc[GS_REL];
+    if (gsc[GS_DONE] == 2.0) return CCFD_OK;   // abort criterion met
   }
-#undef HM
   return CCFD_OK;
 }
 
@@ -1436,9 +1758,14 @@ template <typename T>
 static int gmres_any(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
                      double tGlobal, double time, double dt, const double* dtv,
                      double alpha, double normB, int* nIter) {
+  // steady solves (local steps) orthogonalise by CGS2; the time-accurate
+  // step and GMRES_M itself keep the reference's modified Gram-Schmidt
+  const bool cgs2 = dtv != nullptr;
   if (c->prm.precision == CCFD_FP64)
-    return gmres<T, double>(c, cfg, tGlobal, time, dt, dtv, alpha, normB, nIter);
-  return gmres<T, float>(c, cfg, tGlobal, time, dt, dtv, alpha, normB, nIter);
+    return gmres<T, double>(c, cfg, tGlobal, time, dt, dtv, alpha, normB, nIter,
+                            cgs2);
+  return gmres<T, float>(c, cfg, tGlobal, time, dt, dtv, alpha, normB, nIter,
+                         cgs2);
 }
 
 // Newton residual F(X) = X - Q - alpha*dt*R(X) in FP64 (the outer residual
@@ -1504,6 +1831,9 @@ static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     if (!rc && cfg->precond == 3) rc = prepare_mg(c);
     if (rc) return rc;
   }
+  double* wmin = s->H + cfg->nKdim + 3;
+  if (maxChange > 0.0)
+    hipLaunchKernelGGL(k_set1, dim3(1), dim3(1), 0, c->st, wmin, 1.0);
   while ((n2FK > eps2loc * n2F0) && (nIn < cfg->nNewtonIter)) {
     nIn++;
     if (mixed) {
@@ -1513,34 +1843,41 @@ static int implicit_step_t(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     hipMemcpyAsync(s->B, s->FXK, vs * 8, hipMemcpyDeviceToDevice, c->st);
     int rc = gmres_any<T>(c, cfg, t, time, dt, dtv, alpha, sqrt(n2FK), nGMRES);
     if (rc) return rc;
-    double w = 1.0;
+    // bounded update (steady solves): the weight stays on the device, the
+    // smallest one of the step is read with the residual below
+    const double* wdev = nullptr;
     if (maxChange > 0.0) {
       hipLaunchKernelGGL(k_relchange_partial, dim3(c->nb), dim3(256), 0, c->st,
                          N, Np, s->XK, s->dX, c->prm.gam - 1.0, s->part);
       hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, c->st, c->nb,
-                         s->part, s->H + cfg->nKdim + 6);
-      double mx = 0.0;
-      hipMemcpyAsync(&mx, s->H + cfg->nKdim + 6, 8, hipMemcpyDeviceToHost, c->st);
-      if (hipStreamSynchronize(c->st) != hipSuccess)
-        return fail(CCFD_EHIP, "update bound failed");
-      if (mx > maxChange) w = maxChange / mx;
-      if (relax) *relax = fmin(*relax, w);
+                         s->part, s->H + cfg->nKdim + 6, maxChange,
+                         s->H + cfg->nKdim + 7, wmin);
+      wdev = s->H + cfg->nKdim + 7;
     }
-    // XK += dX; U = XK; prim = consPrim(U) (src/timeDiscretization.c:505-520)
+    // XK += w dX; U = XK; prim = consPrim(U) (src/timeDiscretization.c:505-520)
     hipLaunchKernelGGL(k_axpy, cells_grid(c), dim3(256), 0, c->st, N, Np, s->XK,
-                       s->dX, w);
+                       s->dX, 1.0, wdev);
     outer_residual(c, s->XK, time, alpha * dt, dtv, alpha, nullptr);
-    n2FK = dot_host(c, s->FXK, s->FXK);
+    // a steady step stops after nNewtonIter iterations whatever |F|
+    if (!(steady && nIn >= cfg->nNewtonIter))
+      n2FK = dot_host(c, s->FXK, s->FXK);
   }
   *nNewton += nIn;
   if (nIn == cfg->nNewtonIter && !steady)
     return fail(CCFD_ENOCONV,
                 "Newton NOT converged with %d Newton iterations (|F|^2/|F0|^2 = %g)",
                 nIn, n2FK / n2F0);
+  if (maxChange > 0.0)
+    hipMemcpyAsync(c->hscal + 48, wmin, 8, hipMemcpyDeviceToHost, c->st);
   // globalResidual of the FP64 dU/dt of the new state (ut holds it)
   if (res) {
     int rc = residual_of(c, true, res);
     if (rc) return rc;
+  }
+  if (maxChange > 0.0) {
+    if (hipStreamSynchronize(c->st) != hipSuccess)
+      return fail(CCFD_EHIP, "update bound failed");
+    if (relax) *relax = fmin(*relax, c->hscal[48]);
   }
   // leave the primitive state in the context's arithmetic
   return mixed ? refresh_prim(c) : CCFD_OK;
@@ -1685,4 +2022,78 @@ extern "C" int ccfd_gpu_gmres(ccfd_gpu_ctx* c, const ccfd_gpu_implicit_cfg* cfg,
     hipLaunchKernelGGL(k_load_state<float>, cells_grid(c), dim3(256), 0, c->st,
                        N, Np, s->XK, c->U, (float*)c->prim[c->cur], eqd(c));
   return download_vec(c, s->dX, dX);
+}
+
+// the preconditioner's blocks (buildMatrix + calcDinv, src/linearSolver.c:
+// 186-252) at the resident state, in the reference's element and side
+// order: D (N x 4 x 4, I - dt dR/dX for each element) and, per side, the
+// block towards the face neighbour (nSlots x 4 x 4, -dt dR_i/dX_j; zero for
+// boundary sides). X is the primitive state for precond 1 (the reference's
+// perturbation) and the conservative state otherwise. method 0 assembles
+// by coloured finite differences, 1 element by element (first-order
+// conservative Jacobians only; other cases fall back to method 0).
+extern "C" int ccfd_gpu_jacobian_blocks(ccfd_gpu_ctx* c,
+                                        const ccfd_gpu_implicit_cfg* cfg,
+                                        double time, double dt, int method,
+                                        double* D, double* Blk) {
+  if (!c || !c->hasMesh || !cfg || !D || !Blk || method < 0 || method > 1 ||
+      cfg->precond < 1 || cfg->precond > 3 || cfg->jacobianOrder < 0 ||
+      cfg->jacobianOrder > 2)
+    return fail(CCFD_EINVAL, "bad Jacobian-block arguments");
+  hipSetDevice(c->device);
+  int rc = ensure(c, std::max(cfg->nKdim, 1));
+  if (rc) return rc;
+  ImplicitState* s = c->imp;
+  const int N = c->N, Np = c->Np, S = c->S;
+  const int jo = jacobian_order(c, cfg);
+  if ((rc = prepare_lusgs(c, jo))) return rc;
+  if (cfg->precond >= 2 && (rc = prepare_multicolor(c))) return rc;
+  hipMemcpyAsync(s->XK, c->U, 4 * (size_t)Np * 8, hipMemcpyDeviceToDevice, c->st);
+  const int saved = s->jacMethod;
+  s->jacMethod = method;
+  const bool cons = cfg->precond >= 2;
+  // R(X) at the linearisation point, prim = consPrim(X) as a Newton
+  // iteration has it
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_load_state<T>, cells_grid(c), dim3(256), 0, c->st, N,
+                       Np, s->XK, c->U, (T*)c->prim[c->cur], eqd(c));
+    operator_at<T>(c, time);
+    hipLaunchKernelGGL((k_copy_cvt<T, double>), dim3((4 * Np + 255) / 256),
+                       dim3(256), 0, c->st, 4 * (long)Np, (const T*)c->ut,
+                       s->RXK);
+    return build_matrix<T, T>(c, time, dt, nullptr, cons, jo, false);
+  };
+  rc = c->prm.precision == CCFD_FP64 ? run(double(0)) : run(float(0));
+  s->jacMethod = saved;
+  // restore the primitive state of the resident U
+  if (c->prm.precision == CCFD_FP64)
+    hipLaunchKernelGGL(k_load_state<double>, cells_grid(c), dim3(256), 0, c->st,
+                       N, Np, s->XK, c->U, (double*)c->prim[c->cur], eqd(c));
+  else
+    hipLaunchKernelGGL(k_load_state<float>, cells_grid(c), dim3(256), 0, c->st,
+                       N, Np, s->XK, c->U, (float*)c->prim[c->cur], eqd(c));
+  if (rc) return rc;
+  const size_t kb = c->prm.precision == CCFD_FP64 ? 8 : 4;
+  std::vector<unsigned char> hd(16 * (size_t)Np * kb), hb(16 * (size_t)S * Np * kb);
+  if (hipMemcpyAsync(hd.data(), s->Dg, hd.size(), hipMemcpyDeviceToHost,
+                     c->st) != hipSuccess ||
+      hipMemcpyAsync(hb.data(), s->Blk, hb.size(), hipMemcpyDeviceToHost,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(CCFD_EHIP, "Jacobian block download failed");
+  auto at = [&](const std::vector<unsigned char>& v, size_t k) {
+    return kb == 8 ? ((const double*)v.data())[k]
+                   : (double)((const float*)v.data())[k];
+  };
+  for (int i = 0; i < N; ++i) {
+    const int e = c->perm[i];
+    const long s0 = c->slotStart[e], ns = c->slotStart[e + 1] - s0;
+    for (int k = 0; k < 16; ++k) {
+      D[16 * (size_t)e + k] = at(hd, (size_t)k * Np + i);
+      for (int sl = 0; sl < ns; ++sl)
+        Blk[16 * (size_t)(s0 + sl) + k] = at(hb, ((size_t)k * S + sl) * Np + i);
+    }
+  }
+  return CCFD_OK;
 }

@@ -2,7 +2,9 @@
 // Built once per flux code with -DFLUX_CODE=<1..11> so the 11 flux
 // specialisations compile in parallel; each object exports
 // ccfd_launch_flux_<code>_{d,f} (two-kernel path, FP64 / FP32) and
-// ccfd_launch_fused_<code>_{d,f} (fused LDS paths) used by ccfd_gpu.hip.
+// ccfd_launch_fused_<code>_{d,f} (fused LDS paths) used by ccfd_gpu.hip,
+// and ccfd_launch_jac1_<code>_{d,f} (first-order Jacobian blocks) used by
+// implicit.hip.
 #include "fused.hpp"
 
 #ifndef FLUX_CODE
@@ -52,7 +54,32 @@ static void dispatch_fused(const OpArgs<T>& a, const FusedArgs& f, int lim,
   }
 }
 
+template <typename T, typename M>
+static void dispatch_jac1(const OpArgs<T>& a, const double* X, double reps,
+                          double srEps, const Eq<double>& qd, M* Dg, M* Blk,
+                          hipStream_t st) {
+  const dim3 g((a.N + kBlock - 1) / kBlock), b(kBlock);
+  if (a.ns)
+    hipLaunchKernelGGL((jac1_kernel<FLUX_CODE, T, M, 1>), g, b, 0, st, a, X,
+                       reps, srEps, qd, Dg, Blk);
+  else
+    hipLaunchKernelGGL((jac1_kernel<FLUX_CODE, T, M, 0>), g, b, 0, st, a, X,
+                       reps, srEps, qd, Dg, Blk);
+}
+
 }  // namespace ccfd
+
+void CCFD_CAT(CCFD_CAT(ccfd_launch_jac1_, FLUX_CODE), _d)(
+    const ccfd::OpArgs<double>& a, const double* X, double reps, double srEps,
+    const ccfd::Eq<double>& qd, double* Dg, double* Blk, hipStream_t st) {
+  ccfd::dispatch_jac1<double, double>(a, X, reps, srEps, qd, Dg, Blk, st);
+}
+
+void CCFD_CAT(CCFD_CAT(ccfd_launch_jac1_, FLUX_CODE), _f)(
+    const ccfd::OpArgs<float>& a, const double* X, double reps, double srEps,
+    const ccfd::Eq<double>& qd, float* Dg, float* Blk, hipStream_t st) {
+  ccfd::dispatch_jac1<float, float>(a, X, reps, srEps, qd, Dg, Blk, st);
+}
 
 void CCFD_CAT(CCFD_CAT(ccfd_launch_flux_, FLUX_CODE), _d)(
     const ccfd::OpArgs<double>& a, int nb, hipStream_t st) {

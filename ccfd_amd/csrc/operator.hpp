@@ -519,9 +519,205 @@ __global__ __launch_bounds__(kBlock) void flux_kernel(OpArgs<T> a) {
   if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
 }
 
+// ------------------------------------- first-order Jacobian, face by face
+//
+// The blocks of dR/dU (R the first-order operator, U the conservative
+// state) that the coloured finite differences of the preconditioner record
+// - the diagonal block of each element and the block of each side towards
+// its face neighbour - evaluated element by element in one launch: R_i at
+// the element's state, then with each variable of the element (diagonal
+// block) or of one neighbour (that side's block) perturbed by reps, each
+// recorded as (R_i(perturbed) - R_i) / reps. A first-order R_i depends on
+// its own state and its face neighbours' only, so this is exactly what the
+// coloured differences record, with the same operations (flux_kernel's
+// first-order side flux, summed in side order), hence the same values.
+//
+// One side's contribution to the flux sum of element i at first order:
+// the Riemann flux of the owning orientation (src/fluxCalculation.c:884-
+// 997), minus the viscous flux with the neighbour-difference face gradient
+// (cell gradients are zero at first order), times the length, negated on
+// the side that does not own the face.
+template <int CODE, typename T, int NS>
+__device__ inline void side_flux_o1(const OpArgs<T>& a, int i, int s, int lk,
+                                    const T own[4], const T oth[4],
+                                    T f[4]) {
+  const int Np = a.Np, S = a.S;
+  const T nx = a.fn[s * Np + i], ny = a.fn[(S + s) * Np + i];
+  const T fl = a.flen[s * Np + i];
+  const bool owner = fl > T(0.0);
+  const T len = owner ? fl : -fl;
+  const T* L = owner ? own : oth;
+  const T* R = owner ? oth : own;
+  T Lr[4] = {L[0], nx * L[1] + ny * L[2], -ny * L[1] + nx * L[2], L[3]};
+  T Rr[4] = {R[0], nx * R[1] + ny * R[2], -ny * R[1] + nx * R[2], R[3]};
+  T F[4];
+  conv_flux_t<CODE, T>(a.eq, Lr, Rr, F);
+  f[0] = F[0];
+  f[1] = nx * F[1] - ny * F[2];
+  f[2] = ny * F[1] + nx * F[2];
+  f[3] = F[3];
+  if (NS) {
+    T pa_[4], pb_[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pa_[v] = owner ? own[v] : oth[v];
+    if (lk < 0) {
+      const int g = -lk - 1;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) pb_[v] = a.ghostP[v * a.G + g];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) pb_[v] = owner ? oth[v] : own[v];
+    }
+    T bvx = a.bbv[s * Np + i], bvy = a.bbv[(S + s) * Np + i];
+    if (!owner) {
+      bvx = -bvx;
+      bvy = -bvy;
+    }
+    const T bd = a.bbd[s * Np + i];
+    const T bb0 = bvx / bd, bb1 = bvy / bd;
+    T st[4], gxf[4], gyf[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      st[v] = T(0.5) * (R[v] + L[v]);
+      T gmx = T(0.5) * (T(0.0) + T(0.0));
+      T gmy = T(0.5) * (T(0.0) + T(0.0));
+      T cr = gmx * bb0 + gmy * bb1 - (pb_[v] - pa_[v]) / bd;
+      gxf[v] = gmx - cr * bb0;
+      gyf[v] = gmy - cr * bb1;
+    }
+    T fx[4], fy[4];
+    diffusion_flux<T>(a.eq, st, gxf, gyf, fx, fy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) f[v] -= (fx[v] * nx + fy[v] * ny);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    f[v] *= len;
+    if (!owner) f[v] = -f[v];
+  }
+}
+
+// state of element j from the FP64 conservative state X, perturbed by reps
+// in variable iv (iv < 0: none), in the operator's arithmetic
+template <typename T>
+__device__ inline void jac_state(const double* __restrict__ X, int Np, int j,
+                                 int iv, double reps, const Eq<double>& qd,
+                                 T p[4]) {
+  double u[4], q[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    u[v] = X[v * Np + j];
+    if (v == iv) u[v] += reps;
+  }
+  cons_prim<double>(qd, u, q);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) p[v] = T(q[v]);
+}
+
+// ghost side state of element i's boundary side at first order
+// (setBCatSides on the element state); exact-function boundaries do not
+// depend on it and are read from ghostS
+template <typename T>
+__device__ inline void jac_ghost(const OpArgs<T>& a, int i, int s, int g,
+                                 const T in[4], T out[4]) {
+  const int Np = a.Np, S = a.S;
+  const BCDev b = op_bc(a, a.ghostBC[g]);
+  if (b.type == 6) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = a.ghostS[v * a.G + g];
+    return;
+  }
+  T x[2] = {a.gp[s * Np + i] + T(a.bary[i]),
+            a.gp[(S + s) * Np + i] + T(a.bary[Np + i])};
+  T inc[4] = {in[0], in[1], in[2], in[3]};
+  bc_ghost<T, false>(a.eq, b, a.ex, a.fn[s * Np + i], a.fn[(S + s) * Np + i],
+                     op_time(a, i), inc, out, x);
+}
+
+template <int CODE, typename T, typename M, int NS>
+__global__ __launch_bounds__(kBlock) void jac1_kernel(
+    OpArgs<T> a, const double* __restrict__ X, double reps, double srEps,
+    Eq<double> qd, M* __restrict__ Dg, M* __restrict__ Blk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const int Np = a.Np, S = a.S;
+  const T aq = a.areaq[i];
+  T p0[4];
+  jac_state<T>(X, Np, i, -1, reps, qd, p0);
+  // base side fluxes and R_i
+  T fb[kMaxSlots][4], R0[4];
+  T acc[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+  for (int s = 0; s < S; ++s) {
+    const int lk = a.link[s * Np + i];
+    if (lk == kNoSide) continue;
+    T oth[4];
+    if (lk >= 0) jac_state<T>(X, Np, lk >> 2, -1, reps, qd, oth);
+    else jac_ghost<T>(a, i, s, -lk - 1, p0, oth);
+    side_flux_o1<CODE, T, NS>(a, i, s, lk, p0, oth, fb[s]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] += fb[s][v];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) R0[v] = (T(0.0) - acc[v]) * aq;
+  for (int iv = 0; iv < 4; ++iv) {
+    // diagonal block: the element's own variable iv perturbed
+    T p1[4];
+    jac_state<T>(X, Np, i, iv, reps, qd, p1);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = T(0.0);
+    for (int s = 0; s < S; ++s) {
+      const int lk = a.link[s * Np + i];
+      if (lk == kNoSide) continue;
+      T oth[4], f[4];
+      if (lk >= 0) {
+        const int j = lk >> 2;
+        jac_state<T>(X, Np, j, j == i ? iv : -1, reps, qd, oth);
+      } else {
+        jac_ghost<T>(a, i, s, -lk - 1, p1, oth);
+      }
+      side_flux_o1<CODE, T, NS>(a, i, s, lk, p1, oth, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] += f[v];
+    }
+#pragma unroll
+    for (int jv = 0; jv < 4; ++jv) {
+      const T r = (T(0.0) - acc[jv]) * aq;
+      Dg[(4 * jv + iv) * Np + i] =
+          M((double(r) - double(R0[jv])) * srEps);
+    }
+    // side blocks: the face neighbour's variable iv perturbed
+    for (int s = 0; s < S; ++s) {
+      const int lk = a.link[s * Np + i];
+      if (lk < 0 || (lk >> 2) == i) continue;
+      T q1[4], f[4];
+      jac_state<T>(X, Np, lk >> 2, iv, reps, qd, q1);
+      side_flux_o1<CODE, T, NS>(a, i, s, lk, p0, q1, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = T(0.0);
+      for (int s2 = 0; s2 < S; ++s2) {
+        const int l2 = a.link[s2 * Np + i];
+        if (l2 == kNoSide) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] += s2 == s ? f[v] : fb[s2][v];
+      }
+#pragma unroll
+      for (int jv = 0; jv < 4; ++jv) {
+        const T r = (T(0.0) - acc[jv]) * aq;
+        Blk[((size_t)(4 * jv + iv) * S + s) * Np + i] =
+            M((double(r) - double(R0[jv])) * srEps);
+      }
+    }
+  }
+}
+
 // launch entry of one flux code (defined in op_flux.hip, one object per code)
 template <typename T>
 void launch_flux_code(int code, const OpArgs<T>& a, int nblocks,
                       hipStream_t st);
+// the first-order Jacobian blocks above (M: the matrix type)
+template <typename T, typename M>
+void launch_jac1_code(int code, const OpArgs<T>& a, const double* X,
+                      double reps, double srEps, const Eq<double>& qd, M* Dg,
+                      M* Blk, hipStream_t st);
 
 }  // namespace ccfd

@@ -98,12 +98,16 @@ def lib():
                                                ip]),
         "ccfd_gpu_get_variant": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
-                                             C.c_double, dp, ip]),
+                                             C.c_double, C.c_int, dp, ip,
+                                             dp]),
         "ccfd_gpu_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
         "ccfd_gpu_gmres": (C.c_int, [vp, C.POINTER(ImplicitCfg), C.c_double,
                                      C.c_double, C.c_double, C.c_double, dp,
                                      C.c_double, dp, ip]),
         "ccfd_gpu_dot": (C.c_int, [vp, dp, dp, dp]),
+        "ccfd_gpu_jacobian_blocks": (C.c_int, [vp, C.POINTER(ImplicitCfg),
+                                               C.c_double, C.c_double,
+                                               C.c_int, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -264,6 +268,16 @@ class Solver:
         check(self.L.ccfd_gpu_dot(self.h, N.dptr(a), N.dptr(b), C.byref(out)))
         return out.value
 
+    def jacobian_blocks(self, cfg, time, dt, method=1):
+        """the preconditioner's blocks at the resident state (reference
+        order): D (N, 4, 4) and the side blocks (nSlots, 4, 4); method 0
+        coloured finite differences, 1 element by element"""
+        D = np.empty((self.N, 4, 4))
+        Bk = np.empty((self.S, 4, 4))
+        check(self.L.ccfd_gpu_jacobian_blocks(self.h, C.byref(cfg), time, dt,
+                                              method, N.dptr(D), N.dptr(Bk)))
+        return D, Bk
+
     def implicit_steady(self, cfg, cfl, max_steps, drop, res0=None):
         """implicit pseudo-time steps with local steps until the density
         residual drops by `drop`; returns (res, res0, steps, newton, gmres).
@@ -279,12 +293,15 @@ class Solver:
     def sync(self):
         check(self.L.ccfd_gpu_synchronize(self.h))
 
-    def profile_steps(self, nsteps, ptime=1e300, tend=1e300):
-        """average kernel durations (ms) over nsteps device-driven steps:
-        dict(dt, recon, flux, other, step) and launch counts"""
+    def profile_steps(self, nsteps, ptime=1e300, tend=1e300, residual=True):
+        """average kernel durations (ms) over nsteps device-driven steps
+        (each with globalResidual unless residual is False): dict(dt, recon,
+        flux, other, step), launch counts and the last step's residual"""
         ms = np.zeros(5)
         cnt = (C.c_int * 4)()
+        r = np.zeros(4)
         check(self.L.ccfd_gpu_profile_steps(self.h, nsteps, ptime, tend,
-                                            N.dptr(ms), cnt))
+                                            1 if residual else 0, N.dptr(ms),
+                                            cnt, N.dptr(r)))
         keys = ("dt", "recon", "flux", "other", "step")
-        return dict(zip(keys, ms.tolist())), list(cnt)
+        return dict(zip(keys, ms.tolist())), list(cnt), r

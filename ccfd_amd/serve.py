@@ -209,21 +209,34 @@ class Server:
 
 
 def summary(results, makespan=None):
-    """solves/s over the run and latency percentiles (seconds)"""
+    """Solves/s and latency percentiles (seconds) of a run. A solve counts
+    when it reached the target residual drop; requests retired at the step
+    cap or with a non-finite state are failures, counted apart, and the
+    retirement rate over all requests is given beside the solve rate."""
     if not results:
-        return {"solves": 0}
-    lat = np.array([r["latency"] for r in results])
+        return {"solves": 0, "requests": 0}
+    conv = [r for r in results if r["converged"]]
+    capped = [r for r in results if not r["converged"] and not r.get("nonfinite")]
+    nonfin = [r for r in results if not r["converged"] and r.get("nonfinite")]
     t_end = max(r["done"] for r in results)
     t_beg = min(r["arrival"] for r in results)
     span = makespan if makespan is not None else t_end - t_beg
+
+    def pct(rs, q):
+        return float(np.percentile([r["latency"] for r in rs], q)) if rs else None
     return {
-        "solves": len(results),
-        "converged": int(sum(r["converged"] for r in results)),
+        "requests": len(results),
+        "solves": len(conv),
+        "failed_step_cap": len(capped),
+        "failed_nonfinite": len(nonfin),
         "makespan_s": span,
-        "solves_per_s": len(results) / span if span > 0 else float("inf"),
-        "latency_p50_s": float(np.percentile(lat, 50)),
-        "latency_p99_s": float(np.percentile(lat, 99)),
-        "latency_max_s": float(lat.max()),
+        "solves_per_s": len(conv) / span if span > 0 else float("inf"),
+        "retirements_per_s": len(results) / span if span > 0 else float("inf"),
+        "latency_p50_s": pct(conv, 50),
+        "latency_p99_s": pct(conv, 99),
+        "latency_p50_all_s": pct(results, 50),
+        "latency_p99_all_s": pct(results, 99),
+        "latency_max_s": float(max(r["latency"] for r in results)),
         "cells_total": int(sum(r["n"] for r in results)),
         "steps_total": int(sum(r["steps"] for r in results)),
         "polish_steps_total": int(sum(r.get("polish_steps", 0)

@@ -214,6 +214,19 @@ int ccfd_gpu_gmres(ccfd_gpu_ctx *ctx, const ccfd_gpu_implicit_cfg *cfg,
  * order (deterministic, not the reference's OpenMP order) */
 int ccfd_gpu_dot(ccfd_gpu_ctx *ctx, const double *a, const double *b,
 		double *out);
+/*
+ * The LU-SGS / Gauss-Seidel preconditioner's blocks (buildMatrix + calcDinv,
+ * src/linearSolver.c:186-252) at the resident state for cfg's precond
+ * (1..3) and jacobianOrder, time and dt, in the reference's element and
+ * side order: D (N x 16, the diagonal block I - dt dR/dX row by row) and
+ * Blk (nSlots x 16, the block of each side towards its face neighbour,
+ * zero at boundary sides). method 0: coloured finite differences; 1:
+ * element by element in one launch (first-order conservative Jacobians;
+ * otherwise as 0). Both give the same blocks.
+ */
+int ccfd_gpu_jacobian_blocks(ccfd_gpu_ctx *ctx,
+		const ccfd_gpu_implicit_cfg *cfg, double time, double dt,
+		int method, double *D, double *Blk);
 
 /* ---- flux functions in isolation ---- */
 /*
@@ -247,14 +260,17 @@ int ccfd_gpu_get_variant(ccfd_gpu_ctx *ctx, int *variant);
 
 /* ---- timing helpers ---- */
 /*
- * Run nSteps device-driven steps (as ccfd_gpu_advance from t = 0) with HIP
- * events around every kernel on the context's stream. ms[0..3]: average
- * duration of the dt reduction, reconstruction kernel, flux / update kernel
- * and other kernels; ms[4]: wall time per step. launches[0..3] (may be NULL)
- * gets the launch counts.
+ * Run nSteps device-driven steps (as ccfd_gpu_advance from t = 0; with
+ * residual != 0 each step also computes globalResidual, as the reference's
+ * explicit step does) with HIP events around every kernel on the context's
+ * stream. ms[0..3]: average duration of the dt reduction, reconstruction
+ * kernel, flux / update kernel and other kernels; ms[4]: wall time per
+ * step. launches[0..3] (may be NULL) gets the launch counts; res (may be
+ * NULL) the residual of the last step.
  */
 int ccfd_gpu_profile_steps(ccfd_gpu_ctx *ctx, int nSteps, double pTime,
-		double stopTime, double *ms, int *launches);
+		double stopTime, int residual, double *ms, int *launches,
+		double res[4]);
 /*
  * Streaming copy of `bytes` between two device buffers, repeated `reps`
  * times in each of two launch shapes (grid-stride loop; one 16 KiB tile per

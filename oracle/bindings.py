@@ -240,9 +240,10 @@ class Ref:
 
     _cache = {}
 
-    def __init__(self, ns=False, omp=False):
+    def __init__(self, ns=False, omp=False, v3=False):
         name = "libccfdref_ns.so" if ns else (
-            "libccfdref_omp.so" if omp else "libccfdref.so")
+            ("libccfdref_omp_v3.so" if v3 else "libccfdref_omp.so") if omp
+            else "libccfdref.so")
         path = os.path.join(HERE, "_ref", name)
         if path not in Ref._cache:
             if not os.path.exists(path):
@@ -256,6 +257,7 @@ class Ref:
             L.ref_riemann.argtypes = [C.c_int, C.c_long, dp, dp, dp]
             L.ref_set_riemann.argtypes = [dp, dp, C.c_double]
             L.ref_set_constants.argtypes = [C.c_double, C.c_double, C.c_double]
+            L.ref_set_flux.argtypes = [C.c_int]
             Ref._cache[path] = L
         self.L = Ref._cache[path]
 
@@ -285,6 +287,16 @@ class Ref:
         gp = np.empty((max(self.G, 1), 4))
         self.L.ref_fvtd(time, _d(ut), _d(ux), _d(uy), _d(sp), _d(fl), _d(gp))
         return dict(u_t=ut, u_x=ux, u_y=uy, side=sp, flux=fl, ghost=gp[:self.G])
+
+    def set_flux(self, code):
+        """iFlux of later fvtd calls"""
+        self.L.ref_set_flux(code)
+
+    def fvtd_ut(self, time=0.0):
+        """dU/dt only"""
+        ut = np.empty((self.N, 4))
+        self.L.ref_fvtd(time, _d(ut), None, None, None, None, None)
+        return ut
 
     def fvtd_only(self, time=0.0):
         self.L.ref_fvtd(time, None, None, None, None, None, None)

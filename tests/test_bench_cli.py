@@ -141,29 +141,75 @@ def test_served_sample_with_a_stand_in_batch(monkeypatch):
 
 def test_traffic_figures_come_from_the_committed_profiles():
     """bench.py's roofline.traffic reads profiles/pmc_traffic.json; each
-    entry must be the scaled FETCH_SIZE + WRITE_SIZE bytes per dispatch that
-    the committed rocprofv3 summary lists for that kernel"""
+    entry must be the scaled FETCH_SIZE + WRITE_SIZE bytes per dispatch
+    that the committed rocprofv3 summary of that kernel lists"""
     import json
     import re
     prof = os.path.join(ROOT, "profiles")
     with open(os.path.join(prof, "pmc_traffic.json")) as f:
-        traffic = json.load(f)
-    rows = {
-        "fused_face_fp64": ("fused_face_kernel_fp64_4M_delaunay.md",
-                            "fused_face_kernel<5, double, 2, 3>"),
-        "fused_face_fp32": ("fused_face_kernel_fp32_4M_delaunay.md",
-                            "fused_face_kernel<5, float, 2, 3>"),
-        "batched_fused_fp32": ("fused_kernel_fp32_batched_64_meshes.md",
-                               "ccfd::fused_kernel<5, float, 1, 3>"),
-    }
-    assert set(traffic) == set(rows)
-    for key, (md, kernel) in rows.items():
-        text = open(os.path.join(prof, md)).read()
+        traffic = json.load(f)["kernels"]
+    assert set(traffic) == {"fused_face_fp64", "fused_face_fp32"}
+    for key, ent in traffic.items():
+        text = open(os.path.join(prof, ent["summary"])).read()
         head = text.index("## HBM traffic")
-        line = [x for x in text[head:].splitlines() if kernel in x][0]
-        cells = [c.strip() for c in line.strip("|").split("|")]
-        fetch, write, scaled = float(cells[1]), float(cells[2]), float(cells[3])
-        scale = float(re.search(r"FETCH is scaled by ([0-9.]+)",
-                                text).group(1))
-        assert abs(traffic[key] / scaled - 1) < 1e-3, key
-        assert abs((fetch * scale + write) * 1024 / traffic[key] - 1) < 1e-3, key
+        tab = text[head:].splitlines()
+
+        def last_cell(name):
+            line = [x for x in tab if x.startswith("| " + name)][0]
+            return float(re.findall(r"[0-9.]+e[0-9]+", line)[-1])
+        fetch, write, total = (last_cell("FETCH_SIZE"), last_cell("WRITE_SIZE"),
+                               last_cell("total"))
+        assert abs(ent["bytes_per_launch"] / total - 1) < 1e-3, key
+        assert abs((fetch + write) / total - 1) < 2e-3, key
+
+
+def test_traffic_figure_only_for_the_sources_it_was_measured_on(tmp_path,
+                                                                monkeypatch):
+    """the committed counter figure is used only while the device sources
+    hash to what it was measured on; otherwise the line says it is stale"""
+    sys.path.insert(0, ROOT)
+    import json
+
+    import bench
+    (tmp_path / "profiles").mkdir()
+    d = {"kernel_source_sha": "aaaa", "kernels": {"fused_face_fp64": {
+        "bytes_per_launch": 1.5e9, "summary": "x.md", "command": "cmd"}}}
+    (tmp_path / "profiles" / "pmc_traffic.json").write_text(json.dumps(d))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    monkeypatch.setattr(bench, "kernel_source_sha", lambda: "aaaa")
+    t, src = bench.pmc_traffic("fused_face_fp64")
+    assert t == 1.5e9 and "x.md" in src
+    assert bench.pmc_traffic("fused_face_fp32")[0] is None
+    monkeypatch.setattr(bench, "kernel_source_sha", lambda: "bbbb")
+    t, src = bench.pmc_traffic("fused_face_fp64")
+    assert t is None and "stale" in src
+
+
+def test_committed_traffic_figures_match_the_sources():
+    """profiles/pmc_traffic.json names the hash of the committed device
+    sources"""
+    sys.path.insert(0, ROOT)
+    import json
+
+    import bench
+    with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
+        d = json.load(f)
+    assert d["kernel_source_sha"] == bench.kernel_source_sha()
+    for k, e in d["kernels"].items():
+        assert os.path.exists(os.path.join(ROOT, "profiles", e["summary"])), k
+
+
+def test_ut_parity_summary():
+    sys.path.insert(0, ROOT)
+    import bench
+    import numpy as np
+    rng = np.random.default_rng(0)
+    ref = rng.normal(size=(50, 4))
+    p = bench._ut_parity(ref.copy(), ref)
+    assert p["bitwise"] and p["max_rel_err"] == 0.0
+    got = ref.copy()
+    got[7, 3] += 1e-9 * np.abs(ref[:, 3]).max()
+    p = bench._ut_parity(got, ref)
+    assert not p["bitwise"]
+    assert abs(p["max_rel_err"] - 1e-9) < 1e-12
+    assert p["density_ut_rel_l2"] == 0.0 and p["energy_ut_rel_linf"] > 0.0

@@ -314,3 +314,39 @@ def test_fused_equals_split_large_delaunay(n, periodic):
         assert np.array_equal(r, out[0][1])
         assert np.array_equal(cons, out[0][2])
         assert np.array_equal(ut, out[0][3])
+
+
+@pytest.fixture(scope="module")
+def large_delaunay():
+    """250,632 periodic jittered-Delaunay triangles (355 points per side),
+    the smooth seeded state of the 4M-cell workload"""
+    mesh = cases.delaunay_mesh(355, seed=2, periodic=True)
+    return mesh, M.smooth_state(mesh.bary, seed=2)
+
+
+@pytest.mark.parametrize("flux", [2, 3, 5, 9], ids=["roe", "hll", "hllc",
+                                                     "ausmd"])
+def test_large_delaunay_matches_the_oracle(large_delaunay, flux):
+    """fvTimeDerivative (2nd order, VK K = 5) on 250k periodic triangles,
+    the default (fused face-once) operator: dU/dt equal to the oracle's bit
+    for bit, then two RK4 steps with device time steps and the residual of
+    each step, equal to the oracle's steps"""
+    mesh, prim = large_delaunay
+    kw = dict(flux=flux, order=2, limiter=2, venk_k=5.0, nrk=4, torder=2)
+    gs = G.Solver(mesh, G.params(**kw))
+    assert gs.variant() == G.KV_FUSED_FACE
+    orc = B.Oracle(mesh, B.make_cfg(**kw))
+    gs.set_prim(prim)
+    orc.set_prim(prim)
+    gs.fvtd(0.0)
+    assert np.array_equal(gs.ut(), orc.fvtd(0.0))
+    t, to = 0.0, 0.0
+    for _ in range(2):
+        t, r = gs.advance(1, t, 1e9, 1e9)
+        dto, _, _ = orc.calc_dt(to, 1e9, 1e9)
+        ro = orc.explicit_step(to, dto)
+        to += dto
+        assert t == to
+        assert np.allclose(r, ro, rtol=1e-12, atol=0.0)
+    assert np.array_equal(gs.cons(), orc.cons())
+    gs.close()

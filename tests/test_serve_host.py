@@ -100,9 +100,10 @@ def test_all_requests_complete_within_budget_in_fifo_order():
         assert r["converged"] and r["drop"] <= 1e-6
         assert math.isclose(r["latency"], r["done"] - r["arrival"])
     s = S.summary(res)
-    assert s["solves"] == 60 and s["converged"] == 60
+    assert s["requests"] == 60 and s["solves"] == 60
+    assert s["failed_step_cap"] == s["failed_nonfinite"] == 0
     assert s["latency_p50_s"] <= s["latency_p99_s"] <= s["latency_max_s"]
-    assert s["solves_per_s"] > 0
+    assert s["solves_per_s"] == s["retirements_per_s"] > 0
 
 
 def test_step_cap_retires_stalled_solves():
@@ -116,6 +117,11 @@ def test_step_cap_retires_stalled_solves():
     assert all(not r["converged"] and r["steps"] >= 5000 for r in stalled)
     assert all(r["converged"] for r in res if r["id"] not in (3, 7))
     assert fb.used == 0 and not fb.inst
+    # capped requests are failures: they do not count as solves
+    s = S.summary(res)
+    assert s["requests"] == 10 and s["solves"] == 8
+    assert s["failed_step_cap"] == 2 and s["failed_nonfinite"] == 0
+    assert math.isclose(s["solves_per_s"], 0.8 * s["retirements_per_s"])
 
 
 def test_request_larger_than_the_arena_waits_for_room():
