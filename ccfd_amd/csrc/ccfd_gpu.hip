@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1232,6 +1233,74 @@ extern "C" int ccfd_gpu_riemann_batch(int code, int precision, double gam,
   }
   hipError_t e = hipMemcpy(F, dF, b, hipMemcpyDeviceToHost);
   hipFree(dL); hipFree(dR); hipFree(dF);
+  return e == hipSuccess ? CCFD_OK : fail(CCFD_EHIP, "riemann batch failed");
+}
+
+// ---- convectiveFlux on n states without allocating (staging in `scratch`)
+//
+// The device staging of L, R and F lives in a file-scope device array, so
+// the call allocates nothing: for callers that cannot call hipMalloc (a
+// per-face convectiveFlux loop, code under graph capture, serving loops).
+// Host code must not pass the array's name to a kernel or a copy: its
+// device address comes from hipGetSymbolAddress, once per device, and is
+// kept in g_scratch[device]. Every call on one device uses that device's
+// array; calls that overlap on one device (other host threads) race.
+constexpr long kNoAllocMaxStates = CCFD_NOALLOC_MAX_STATES;
+static __device__ __align__(256) double scratch[kNoAllocMaxStates * 12];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+}  // namespace
+
+// device address of this device's staging array (set up on first use)
+static int scratch_ptr(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(CCFD_EHIP, "no usable current device");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CCFD_EHIP, "staging array address: %s", hipGetErrorString(e));
+    g_scratch[dev] = (double*)p;
+  }
+  *out = g_scratch[dev];
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_riemann_batch_noalloc(int code, int precision,
+                                              double gam, long n,
+                                              const double* L, const double* R,
+                                              double* F) {
+  if (code < 1 || code > 11) return fail(CCFD_EINVAL, "unknown flux %d", code);
+  if (n < 0 || (n > 0 && (!L || !R || !F))) return fail(CCFD_EINVAL, "bad arrays");
+  if (n > kNoAllocMaxStates)
+    return fail(CCFD_EINVAL,
+                "%ld states exceed the %ld of the preallocated staging array",
+                n, kNoAllocMaxStates);
+  if (n == 0) return CCFD_OK;
+  double* base = nullptr;
+  int rc = scratch_ptr(&base);
+  if (rc) return rc;
+  double* dL = base;
+  double* dR = base + 4 * (size_t)kNoAllocMaxStates;
+  double* dF = base + 8 * (size_t)kNoAllocMaxStates;
+  const size_t b = 4 * (size_t)n * 8;
+  if (hipMemcpy(dL, L, b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dR, R, b, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(CCFD_EHIP, "riemann staging upload failed");
+  dim3 g((n + 255) / 256), bl(256);
+  if (precision == CCFD_FP64) {
+    Eq<double> q = {gam, gam - 1.0, 1.0 / (gam - 1.0), 0.0, 0.72};
+    hipLaunchKernelGGL(k_riemann<double>, g, bl, 0, 0, code, n, dL, dR, dF, q);
+  } else {
+    Eq<float> q = {float(gam), float(gam - 1.0), float(1.0 / (gam - 1.0)), 0.f, 0.72f};
+    hipLaunchKernelGGL(k_riemann<float>, g, bl, 0, 0, code, n, dL, dR, dF, q);
+  }
+  hipError_t e = hipMemcpy(F, dF, b, hipMemcpyDeviceToHost);
   return e == hipSuccess ? CCFD_OK : fail(CCFD_EHIP, "riemann batch failed");
 }
 

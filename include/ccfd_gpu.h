@@ -236,6 +236,18 @@ int ccfd_gpu_jacobian_blocks(ccfd_gpu_ctx *ctx,
  */
 int ccfd_gpu_riemann_batch(int code, int precision, double gam, long n,
 		const double *L, const double *R, double *F);
+/*
+ * The same without allocating device memory (no counterpart in the
+ * reference): for callers that cannot allocate, such as a per-face
+ * convectiveFlux loop, code under HIP graph capture or a serving loop.
+ * L, R and F are staged through a device array the library holds for the
+ * life of the process (CCFD_NOALLOC_MAX_STATES states, 1.5 MiB per device);
+ * larger n is refused with CCFD_EINVAL. Runs on the current device, whose
+ * array every call shares: calls that overlap on one device race.
+ */
+#define CCFD_NOALLOC_MAX_STATES 16384
+int ccfd_gpu_riemann_batch_noalloc(int code, int precision, double gam, long n,
+		const double *L, const double *R, double *F);
 
 /* ---- kernel variants ---- */
 /*
