@@ -158,6 +158,10 @@ def reference_run(mesh, prim, flux, venk_k, seconds, parity_fluxes):
             calls += 1
         el = time.time() - t0
     finally:
+        # the reference's printf output sits in the C library's buffer:
+        # flush it while stdout still points at /dev/null
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
         os.dup2(saved, 1)
         os.close(devnull)
         os.close(saved)
@@ -189,6 +193,23 @@ def reference_run(mesh, prim, flux, venk_k, seconds, parity_fluxes):
                      os.environ.get("OMP_PLACES"), _cpu_model()),
     }
     return base, uts
+
+
+def _reference_child(raw, bcs, prim, flux, venk_k, seconds, parity_fluxes):
+    """reference_run in a process of its own (spawned): its OpenMP runtime,
+    thread binding and host arrays stay out of the process that drives the
+    GPU"""
+    from ccfd_amd import mesh as M
+    mesh = M.Mesh(raw, bcs)
+    return reference_run(mesh, prim, flux, venk_k, seconds, parity_fluxes)
+
+
+def reference_in_child(raw, bcs, prim, flux, venk_k, seconds, parity_fluxes):
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    with cf.ProcessPoolExecutor(1, mp_context=mp.get_context("spawn")) as ex:
+        return ex.submit(_reference_child, raw, bcs, prim, flux, venk_k,
+                         seconds, parity_fluxes).result()
 
 
 def kernel_source_sha():
@@ -410,7 +431,9 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
            "cell_stage_updates_per_s": ncell * 3 * steps / el,
            "solves_per_s": len(ids) / el}
     if baseline:
+        import numpy as np
         tot, errs, eerrs = 0.0, [], []
+        nan32 = nan64 = 0
         for k, (m, p) in zip(ids, cases):
             s = G.Solver(m, sch64)
             s.set_prim(p)
@@ -420,6 +443,8 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
             s.sync()
             tot += time.perf_counter() - w0
             q32, q64 = b.prim(k), s.prim()
+            nan32 += int(not np.isfinite(q32).all())
+            nan64 += int(not np.isfinite(q64).all())
             errs.append(_density_err(q32, q64))
             eerrs.append(_energy_err(q32, q64))
             s.close()
@@ -440,6 +465,7 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
                  "density_rel_l2": errs[k][0], "density_rel_linf": errs[k][1],
                  "max_mach": _max_mach(cases[k][1])} for k in bad]})
         out["energy_err_vs_fp64"] = _err_summary(eerrs)
+        out["nonfinite_states"] = {"batched_fp32": nan32, "single_fp64": nan64}
     b.close()
     return out
 
@@ -539,6 +565,9 @@ def main():
     ap.add_argument("--variant", choices=["auto", "split", "fused",
                                           "fused_face"],
                     default="auto", help="operator implementation")
+    ap.add_argument("--tile", type=int, default=0, choices=[0, 256, 512],
+                    help="elements per block of the face-once kernel (0: "
+                         "automatic)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -642,7 +671,6 @@ def main():
         return
 
     raw, bcs, mesh, prim = build_case(args.n, args.seed)
-    del raw, bcs
     prec = G.FP64 if args.precision == "fp64" else G.FP32_FLUX
 
     def prm_of(flux, precision):
@@ -653,7 +681,16 @@ def main():
     solver.set_variant({"auto": G.KV_AUTO, "split": G.KV_SPLIT,
                         "fused": G.KV_FUSED,
                         "fused_face": G.KV_FUSED_FACE}[args.variant])
+    solver.set_tile(args.tile)
+    tuned = None
+    if args.variant == "auto" and args.tile == 0:
+        # the autotuner times the operator implementations and tiles on the
+        # workload's own state and keeps the fastest
+        solver.set_prim(prim)
+        _, _, tuned = solver.autotune(10)
+        log("autotune (ms per operator call): %s" % json.dumps(tuned))
     kv = solver.variant()
+    tile = solver.tile()
     fused = kv in (G.KV_FUSED, G.KV_FUSED_FACE)
     solver.set_prim(prim)
     log("upload %.1f s" % (time.time() - t0))
@@ -751,11 +788,12 @@ def main():
     cpu, refut = None, {}
     if not args.no_cpu:
         try:
-            cpu, refut = reference_run(
-                mesh, prim, args.flux, args.venk_k, args.cpu_seconds,
+            cpu, refut = reference_in_child(
+                raw, bcs, prim, args.flux, args.venk_k, args.cpu_seconds,
                 [args.flux] + flux_list)
         except Exception as e:   # must not hide the GPU result
             log("reference run failed:", e)
+    del raw, bcs
     parity = {}
     for (f, p), ut in uts.items():
         if f not in refut:
@@ -790,6 +828,8 @@ def main():
             "precision": args.precision,
             "operator": {G.KV_SPLIT: "split", G.KV_FUSED: "fused",
                          G.KV_FUSED_FACE: "fused_face"}[kv],
+            "tile_elements": tile,
+            "autotune_ms_per_call": tuned,
         },
         "kernel_ms": ms,
         "residual_last_step": main["residual"],

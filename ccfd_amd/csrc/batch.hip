@@ -542,7 +542,8 @@ extern "C" int ccfd_batch_create(ccfd_batch** out, int device,
   // arena sizing: elements, plus ghosts at 1/8 of the elements
   const size_t perElem = elem_bytes(b->S, b->ts) + ghost_bytes(b->ts) / 8;
   long capE = (long)(budget / perElem);
-  capE = std::min(capE, (1L << 28));
+  // the kernels index [4][S][E] arrays (side states) with 32-bit ints
+  capE = std::min(capE, (long)kMaxIndexedElems / b->S);
   b->capBlocks = (int)(capE / kBlock);
   if (b->capBlocks < 1) {
     hipStreamDestroy(b->st);
@@ -784,9 +785,17 @@ extern "C" int ccfd_batch_add(ccfd_batch* b, const ccfd_mesh* m,
   set_dbl(b, b->areaQInst + k, m->totalArea_q);
   set_dbl(b, b->stepsInst + k, 0.0);
   set_dbl(b, b->nfInst + k, 0.0);
+  // a failure from here on leaves the slot as ccfd_batch_remove would:
+  // ranges released, tables rebuilt without it, the first error reported
+  auto undo = [&](int code) {
+    char msg[512];
+    snprintf(msg, sizeof(msg), "%s", ccfd_gpu_last_error());
+    ccfd_batch_remove(b, k);
+    return fail(code, "%s", msg);
+  };
   // initial state (reference order -> renumbered [4][np] staging)
   int rc = ensure_stage(b, 4 * (size_t)np * 8);
-  if (rc) return rc;
+  if (rc) return undo(rc);
   std::vector<double> st(4 * (size_t)np, 1.0);
   for (int i = 0; i < N; ++i)
     for (int v = 0; v < 4; ++v) st[(size_t)v * np + i] = prim[4 * (size_t)h.perm[i] + v];
@@ -798,8 +807,10 @@ extern "C" int ccfd_batch_add(ccfd_batch* b, const ccfd_mesh* m,
   else
     hipLaunchKernelGGL(k_bset_prim<float>, dim3((np + 255) / 256), dim3(256), 0, b->st,
                        np, e0, E, b->stage, (float*)b->prim[b->cur], b->Ubuf[b->ucur], q);
-  if (hipStreamSynchronize(b->st) != hipSuccess)
-    return fail(CCFD_EHIP, "instance upload failed");
+  if (hipStreamSynchronize(b->st) != hipSuccess) {
+    fail(CCFD_EHIP, "instance upload failed");
+    return undo(CCFD_EHIP);
+  }
   *id = k;
   return rebuild_tables(b);
 }

@@ -55,17 +55,17 @@ static const flux_fn_f kFluxF[12] = {
 
 #define DECL_FUSED(k)                                                        \
   void ccfd_launch_fused_##k##_d(const OpArgs<double>&, const FusedArgs&, int, \
-                                 int, int, size_t, hipStream_t);              \
+                                 int, int, size_t, hipStream_t, int);         \
   void ccfd_launch_fused_##k##_f(const OpArgs<float>&, const FusedArgs&, int,  \
-                                 int, int, size_t, hipStream_t);
+                                 int, int, size_t, hipStream_t, int);
 DECL_FUSED(1) DECL_FUSED(2) DECL_FUSED(3) DECL_FUSED(4) DECL_FUSED(5)
 DECL_FUSED(6) DECL_FUSED(7) DECL_FUSED(8) DECL_FUSED(9) DECL_FUSED(10)
 DECL_FUSED(11)
 
 typedef void (*fused_fn_d)(const OpArgs<double>&, const FusedArgs&, int, int,
-                           int, size_t, hipStream_t);
+                           int, size_t, hipStream_t, int);
 typedef void (*fused_fn_f)(const OpArgs<float>&, const FusedArgs&, int, int,
-                           int, size_t, hipStream_t);
+                           int, size_t, hipStream_t, int);
 static const fused_fn_d kFusedD[12] = {
     nullptr, ccfd_launch_fused_1_d, ccfd_launch_fused_2_d, ccfd_launch_fused_3_d,
     ccfd_launch_fused_4_d, ccfd_launch_fused_5_d, ccfd_launch_fused_6_d,
@@ -80,14 +80,16 @@ static const fused_fn_f kFusedF[12] = {
 template <>
 void ccfd::launch_fused_code<double>(int code, const OpArgs<double>& a,
                                      const FusedArgs& f, int lim, int face,
-                                     int nb, size_t lds, hipStream_t st) {
-  kFusedD[code](a, f, lim, face, nb, lds, st);
+                                     int nb, size_t lds, hipStream_t st,
+                                     int tb) {
+  kFusedD[code](a, f, lim, face, nb, lds, st, tb);
 }
 template <>
 void ccfd::launch_fused_code<float>(int code, const OpArgs<float>& a,
                                     const FusedArgs& f, int lim, int face,
-                                    int nb, size_t lds, hipStream_t st) {
-  kFusedF[code](a, f, lim, face, nb, lds, st);
+                                    int nb, size_t lds, hipStream_t st,
+                                    int tb) {
+  kFusedF[code](a, f, lim, face, nb, lds, st, tb);
 }
 
 template <>
@@ -507,8 +509,9 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
                                  const ccfd_gpu_params* p) {
   if (!c || !m || !p) return fail(CCFD_EINVAL, "null argument");
   if (m->nElems <= 0) return fail(CCFD_EINVAL, "empty mesh");
-  if (m->nElems > (1L << 28))
-    return fail(CCFD_ENOTSUP, "mesh too large (%ld elements)", m->nElems);
+  if (m->nElems > kMaxIndexedElems / kMaxSlots)
+    return fail(CCFD_ENOTSUP, "mesh too large (%ld elements; at most %ld)",
+                m->nElems, kMaxIndexedElems / kMaxSlots);
   if (m->nBC > kMaxBC)
     return fail(CCFD_ENOTSUP, "at most %d boundary conditions", kMaxBC);
   hipSetDevice(c->device);
@@ -537,6 +540,13 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   c->hArea.assign(h.area.begin(), h.area.begin() + N);
   c->tile = kBlock + h.maxHalo;
   if (h.haloCells.empty()) h.haloCells.push_back(0);
+  // halo lists of two-block tiles for the wide face-once kernel
+  std::vector<int> hs2, hc2;
+  int mh2 = 0;
+  build_halo(h, 2 * kBlock, 0, hs2, hc2, mh2);
+  c->tile2 = 2 * kBlock + mh2;
+  c->nt2 = (int)hs2.size() - 1;
+  if (hc2.empty()) hc2.push_back(0);
   std::vector<int> ghostBC(std::max(G, 1), 0), ghostCell(std::max(G, 1), 0),
       ghostK(std::max(G, 1), 0);
   std::vector<double> ghostBary(2 * (size_t)std::max(G, 1), 0.0);
@@ -576,7 +586,9 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
       (rc = upd(h.sx.data(), h.sx.size() * 8, (void**)&c->sx)) ||
       (rc = upd(h.sy.data(), h.sy.size() * 8, (void**)&c->sy)) ||
       (rc = upd(h.haloStart.data(), h.haloStart.size() * 4, (void**)&c->haloStart)) ||
-      (rc = upd(h.haloCells.data(), h.haloCells.size() * 4, (void**)&c->haloCells)))
+      (rc = upd(h.haloCells.data(), h.haloCells.size() * 4, (void**)&c->haloCells)) ||
+      (rc = upd(hs2.data(), hs2.size() * 4, (void**)&c->haloStart2)) ||
+      (rc = upd(hc2.data(), hc2.size() * 4, (void**)&c->haloCells2)))
     return rc;
   if ((rc = upload_geom<double>(c, h.w, h.gp, h.fn, h.flen, h.bbv, h.bbd, h.areaq, venk)) ||
       (rc = upload_geom<float>(c, h.w, h.gp, h.fn, h.flen, h.bbv, h.bbd, h.areaq, venk)))
@@ -846,6 +858,16 @@ bool ccfd::fused_applies(const ccfd_gpu_ctx* c, size_t tsize) {
   return fused_lds_bytes(tsize, c->S, c->tile) + 256 <= c->ldsLimit;
 }
 
+// elements per block of the face-once kernel: two layout blocks when asked
+// for and their LDS fits, else one (the automatic choice: on the 4M-cell
+// periodic Delaunay mesh one block of 256 runs 4% faster than 512, whose
+// 8-wave barriers cost more than its smaller halo saves)
+int ccfd::face_tile(const ccfd_gpu_ctx* c, size_t tsize) {
+  const bool wide = fused_lds_bytes(tsize, c->S, c->tile2, 2 * kBlock) + 512 <=
+                    c->ldsLimit;
+  return c->tileElems == 2 * kBlock && wide ? 2 * kBlock : kBlock;
+}
+
 int ccfd::active_variant(const ccfd_gpu_ctx* c, size_t tsize) {
   if (c->variant == CCFD_KV_SPLIT || !fused_applies(c, tsize))
     return CCFD_KV_SPLIT;
@@ -875,11 +897,19 @@ void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
       exact_ghosts<T>(c, a, 1);
       tspan_end(c, eb, KT_OTHER);
     }
-    FusedArgs f{c->haloStart, c->haloStart + 1, c->haloCells, c->tile};
+    const int face = kv == CCFD_KV_FUSED_FACE ? 1 : 0;
+    const int tb = face ? face_tile(c, sizeof(T)) : kBlock;
     tspan_begin(c, &e1);
-    launch_fused_code<T>(c->prm.iFlux, a, f, c->prm.limiter,
-                         kv == CCFD_KV_FUSED_FACE ? 1 : 0, c->nb,
-                         fused_lds_bytes(sizeof(T), c->S, c->tile), c->st);
+    if (tb == kBlock) {
+      FusedArgs f{c->haloStart, c->haloStart + 1, c->haloCells, c->tile};
+      launch_fused_code<T>(c->prm.iFlux, a, f, c->prm.limiter, face, c->nb,
+                           fused_lds_bytes(sizeof(T), c->S, c->tile), c->st);
+    } else {
+      FusedArgs f{c->haloStart2, c->haloStart2 + 1, c->haloCells2, c->tile2};
+      launch_fused_code<T>(c->prm.iFlux, a, f, c->prm.limiter, face, c->nt2,
+                           fused_lds_bytes(sizeof(T), c->S, c->tile2, tb),
+                           c->st, tb);
+    }
     tspan_end(c, e1, KT_FLUX);
     return;
   }
@@ -1418,6 +1448,77 @@ extern "C" int ccfd_gpu_copy_probe(int device, size_t bytes, int reps,
   hipFree(b);
   if (err != hipSuccess) return fail(CCFD_EHIP, "copy probe failed");
   *gbs = best;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_set_tile(ccfd_gpu_ctx* c, int elems) {
+  if (!c || !c->hasMesh) return fail(CCFD_EINVAL, "no mesh");
+  if (elems != 0 && elems != kBlock && elems != 2 * kBlock)
+    return fail(CCFD_EINVAL, "tile must be 0 (automatic), %d or %d elements",
+                kBlock, 2 * kBlock);
+  c->tileElems = elems;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_gpu_get_tile(ccfd_gpu_ctx* c, int* elems) {
+  if (!c || !c->hasMesh || !elems) return fail(CCFD_EINVAL, "bad arguments");
+  const size_t ts = c->prm.precision == CCFD_FP64 ? 8 : 4;
+  *elems = active_variant(c, ts) == CCFD_KV_FUSED_FACE ? face_tile(c, ts)
+                                                       : kBlock;
+  return CCFD_OK;
+}
+
+// time each applicable operator implementation and tile on the resident
+// state (reps fvTimeDerivative calls each, after one untimed call; the
+// state is not changed) and keep the fastest (a later candidate must beat
+// the one kept by 2%); ms[k] gets the time per call
+// of candidate k: split, fused, face-once 256, face-once 512 (-1 where it
+// does not apply)
+extern "C" int ccfd_gpu_autotune(ccfd_gpu_ctx* c, int reps, int* variant,
+                                 int* tile, double ms[4]) {
+  if (!c || !c->hasMesh || reps < 1 || !variant || !tile || !ms)
+    return fail(CCFD_EINVAL, "bad autotune arguments");
+  hipSetDevice(c->device);
+  const size_t ts = c->prm.precision == CCFD_FP64 ? 8 : 4;
+  const int cand[4][2] = {{CCFD_KV_SPLIT, kBlock}, {CCFD_KV_FUSED, kBlock},
+                          {CCFD_KV_FUSED_FACE, kBlock},
+                          {CCFD_KV_FUSED_FACE, 2 * kBlock}};
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(CCFD_EHIP, "event creation failed");
+  const int sv = c->variant, st = c->tileElems;
+  int best = -1;
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = -1.0;
+    c->variant = cand[k][0];
+    c->tileElems = cand[k][1];
+    if (active_variant(c, ts) != cand[k][0] ||
+        (cand[k][0] == CCFD_KV_FUSED_FACE && face_tile(c, ts) != cand[k][1]))
+      continue;
+    int rc = ccfd_gpu_fvtd(c, c->lastTime);
+    hipEventRecord(e0, c->st);
+    for (int r = 0; rc == CCFD_OK && r < reps; ++r) rc = ccfd_gpu_fvtd(c, c->lastTime);
+    hipEventRecord(e1, c->st);
+    float x = 0.f;
+    if (rc || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&x, e0, e1) != hipSuccess) {
+      c->variant = sv;
+      c->tileElems = st;
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      return rc ? rc : fail(CCFD_EHIP, "autotune timing failed");
+    }
+    ms[k] = x / reps;
+    // a later candidate has to be 2% faster to replace an earlier one
+    // (timing noise must not pick the wider tile over an equal default)
+    if (best < 0 || ms[k] < 0.98 * ms[best]) best = k;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->variant = cand[best][0];
+  c->tileElems = cand[best][1];
+  *variant = c->variant;
+  *tile = cand[best][1];
   return CCFD_OK;
 }
 

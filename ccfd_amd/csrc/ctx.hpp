@@ -76,6 +76,11 @@ struct ccfd_gpu_ctx {
   int* haloStart = nullptr;
   int* haloCells = nullptr;
   int tile = 0;
+  // the same for tiles of two layout blocks (wide face-once kernel)
+  int* haloStart2 = nullptr;
+  int* haloCells2 = nullptr;
+  int tile2 = 0, nt2 = 0;
+  int tileElems = 0;        // face-once tile: 0 automatic, 256 or 512
   int variant = 0;          // ccfd_kernel_variant
   size_t ldsLimit = 65536;  // per-block LDS of the device
   // optional per-kernel timing with HIP events on the context stream
@@ -121,6 +126,7 @@ void tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind);
 // can the fused kernels run (2nd order Euler, LDS tile fits), and which
 // ccfd_kernel_variant the next operator call of type T runs
 bool fused_applies(const ccfd_gpu_ctx* c, size_t tsize);
+int face_tile(const ccfd_gpu_ctx* c, size_t tsize);
 int active_variant(const ccfd_gpu_ctx* c, size_t tsize);
 
 }  // namespace ccfd

@@ -367,7 +367,8 @@ void convectiveFlux(double rhoL, double rhoR, double vxL, double vxR,
 	double L[4] = {rhoL, vxL, vyL, pL}, R[4] = {rhoR, vxR, vyR, pR};
 	int code = C.ready ? C.prm.iFlux : 5;
 	double gam = C.ready ? C.prm.gam : 1.4;
-	if (ccfd_gpu_riemann_batch(code, CCFD_FP64, gam, 1, L, R, fluxLoc))
+	/* the reference calls this once per face: no allocation per call */
+	if (ccfd_gpu_riemann_batch_noalloc(code, CCFD_FP64, gam, 1, L, R, fluxLoc))
 		FAILX("convectiveFlux: %s", ccfd_gpu_last_error());
 }
 

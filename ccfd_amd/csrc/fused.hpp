@@ -135,22 +135,25 @@ __global__ void bc_side_recon_kernel(OpArgs<T> a) {
   for (int v = 0; v < 4; ++v) a.ghostS[v * a.G + g] = out[v];
 }
 
-// LDS column of neighbour element j: its offset in the block, or kBlock +
-// its position in the block's ascending halo list
+// LDS column of neighbour element j: its offset in the tile of TB
+// elements, or TB + its position in the tile's ascending halo list
+template <int TB = kBlock>
 __device__ inline int tile_column(int j, int base, const int* hcell, int nh) {
   int col = j - base;
-  if (col < 0 || col >= kBlock) {
+  if (col < 0 || col >= TB) {
     int lo = 0, hi = nh - 1;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
       if (hcell[mid] < j) lo = mid + 1; else hi = mid;
     }
-    col = kBlock + lo;
+    col = TB + lo;
   }
   return col;
 }
 
-// exclusive prefix sum of x over the block; *total gets the block sum
+// exclusive prefix sum of x over the block of TB threads; *total gets the
+// block sum
+template <int TB = kBlock>
 __device__ inline int block_scan_excl(int x, int* wsum, int* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int inc = x;
@@ -163,7 +166,7 @@ __device__ inline int block_scan_excl(int x, int* wsum, int* total) {
   __syncthreads();
   int pre = 0, tot = 0;
 #pragma unroll
-  for (int k = 0; k < kBlock / 64; ++k) {
+  for (int k = 0; k < TB / 64; ++k) {
     const int v = wsum[k];
     if (k < w) pre += v;
     tot += v;
@@ -265,23 +268,34 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(OpArgs<T> a,
 // the listed side's own LDS slot, which no other list entry reads; phase 3
 // gathers them per element in the reference's side order with the sign of
 // the side (src/fluxCalculation.c:993-996), the same sums as fused_kernel.
-template <int CODE, typename T, int LIM, int S>
-__global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
-                                                            FusedArgs f) {
+//
+// TB: elements (and threads) of a block's tile, a multiple of the layout's
+// block size. A wider tile of one mesh (TB = 2 blocks) has a smaller halo
+// per element; its residual partials are still written per layout block,
+// with the sums of the 256-thread form, so every result is the same.
+template <int CODE, typename T, int LIM, int S, int TB>
+__global__ __launch_bounds__(TB) void fused_face_kernel(OpArgs<T> a,
+                                                        FusedArgs f) {
   extern __shared__ __align__(16) unsigned char fused_smem[];
-  __shared__ int wsum[kBlock / 64];
+  __shared__ int wsum[TB / 64];
   T* lds = reinterpret_cast<T*>(fused_smem);
   const int tile = f.tile;
   int* hcell = reinterpret_cast<int*>(lds + 4 * S * tile);
   unsigned short* ent =
-      reinterpret_cast<unsigned short*>(hcell + (tile - kBlock));
-  const BlockPos bp = block_pos(a);
+      reinterpret_cast<unsigned short*>(hcell + (tile - TB));
+  BlockPos bp;
+  if (TB == kBlock) {
+    bp = block_pos(a);
+  } else {   // one mesh (no arena block map)
+    const int lbt = logical_block();
+    bp = {lbt, lbt * TB, min(TB, a.N - lbt * TB)};
+  }
   const int lb = bp.lb, base = bp.base;
   const int i = base + threadIdx.x;
   const bool active = (int)threadIdx.x < bp.n;
   const int Np = a.Np;
-  const int h0 = f.haloStart[base / kBlock];
-  const int nh = f.haloEnd[base / kBlock] - h0;
+  const int h0 = f.haloStart[base / TB];
+  const int nh = f.haloEnd[base / TB] - h0;
 
   // phase 1: side states into LDS; which of the element's sides it owns
   // and which face fluxes this block evaluates for it
@@ -297,27 +311,27 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
       const bool owner = a.flen[s * Np + i] > T(0.0);
       const int jc = (lk[s] >> 2) - base;
       if (owner) ownMask |= 1u << s;
-      if (owner || (lk[s] >= 0 && (jc < 0 || jc >= kBlock)))
+      if (owner || (lk[s] >= 0 && (jc < 0 || jc >= TB)))
         faceMask |= 1u << s;
     }
   }
   // halo elements on the last threads: the first threads take the extra
   // list entries of the flux phase
-  for (int h = kBlock - 1 - (int)threadIdx.x; h < nh; h += kBlock) {
+  for (int h = TB - 1 - (int)threadIdx.x; h < nh; h += TB) {
     const int e = f.haloCells[h0 + h];
     hcell[h] = e;
     int lkh[S];
-    fused_recon<T, LIM, S>(a, e, kBlock + h, tile, lds, lkh);
+    fused_recon<T, LIM, S>(a, e, TB + h, tile, lds, lkh);
   }
   int nf;
-  int k = block_scan_excl(__popc(faceMask), wsum, &nf);
+  int k = block_scan_excl<TB>(__popc(faceMask), wsum, &nf);
 #pragma unroll
   for (int s = 0; s < S; ++s)
     if (faceMask & (1u << s)) ent[k++] = (unsigned short)((threadIdx.x << 2) | s);
   __syncthreads();
 
   // phase 2: one flux per listed side, into that side's LDS slot
-  for (int e = threadIdx.x; e < nf; e += kBlock) {
+  for (int e = threadIdx.x; e < nf; e += TB) {
     const int c = ent[e] >> 2, s = ent[e] & 3;
     const int ie = base + c;
     const int l = a.link[s * Np + ie];
@@ -327,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
     const T len = owner ? fl : -fl;
     T L[4], R[4];   // owning side left
     if (l >= 0) {
-      const int col = tile_column(l >> 2, base, hcell, nh), ps = l & 3;
+      const int col = tile_column<TB>(l >> 2, base, hcell, nh), ps = l & 3;
       const int cL = owner ? c : col, sL = owner ? s : ps;
       const int cR = owner ? col : c, sR = owner ? ps : s;
 #pragma unroll
@@ -380,21 +394,25 @@ __global__ __launch_bounds__(kBlock) void fused_face_kernel(OpArgs<T> a,
     }
     stage_epilogue<T>(a, i, acc, r2, pre);
   }
-  if (a.res_part) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
+  if (a.res_part) {
+    if (TB == kBlock) block_sum4<T>(r2, a.res_part, lb, gridDim.x);
+    else block_sum4_tile<TB>(r2, a.res_part, base, a.N, a.Np / kBlock);
+  }
 }
 
-// dynamic LDS bytes of one fused block: side states of tile columns, the
-// halo id list and the face list
-inline size_t fused_lds_bytes(size_t tsize, int S, int tile) {
-  return 4 * (size_t)S * tile * tsize + (size_t)(tile - kBlock) * 4 +
-         (size_t)S * kBlock * 2;
+// dynamic LDS bytes of one fused block of tb elements: side states of the
+// tile columns (tb + the largest halo), the halo id list and the face list
+inline size_t fused_lds_bytes(size_t tsize, int S, int tile, int tb = kBlock) {
+  return 4 * (size_t)S * tile * tsize + (size_t)(tile - tb) * 4 +
+         (size_t)S * tb * 2;
 }
 
 // launch entry of one flux code (defined in op_flux.hip)
-// face: 0 fused_kernel, 1 fused_face_kernel
+// face: 0 fused_kernel, 1 fused_face_kernel; tb: tile elements (kBlock, or
+// 2 kBlock for the face-once kernel of one mesh)
 template <typename T>
 void launch_fused_code(int code, const OpArgs<T>& a, const FusedArgs& f,
                        int limiter, int face, int nblocks, size_t lds,
-                       hipStream_t st);
+                       hipStream_t st, int tb = kBlock);
 
 }  // namespace ccfd

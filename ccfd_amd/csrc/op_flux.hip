@@ -33,10 +33,13 @@ static void dispatch(const OpArgs<T>& a, int nb, hipStream_t st) {
 
 template <typename T, int LIM, int S>
 static void go_fused(const OpArgs<T>& a, const FusedArgs& f, int face, int nb,
-                     size_t lds, hipStream_t st) {
-  if (face)
-    hipLaunchKernelGGL((fused_face_kernel<FLUX_CODE, T, LIM, S>), dim3(nb),
-                       dim3(kBlock), lds, st, a, f);
+                     size_t lds, hipStream_t st, int tb) {
+  if (face && tb == 2 * kBlock)
+    hipLaunchKernelGGL((fused_face_kernel<FLUX_CODE, T, LIM, S, 2 * kBlock>),
+                       dim3(nb), dim3(2 * kBlock), lds, st, a, f);
+  else if (face)
+    hipLaunchKernelGGL((fused_face_kernel<FLUX_CODE, T, LIM, S, kBlock>),
+                       dim3(nb), dim3(kBlock), lds, st, a, f);
   else
     hipLaunchKernelGGL((fused_kernel<FLUX_CODE, T, LIM, S>), dim3(nb),
                        dim3(kBlock), lds, st, a, f);
@@ -44,13 +47,14 @@ static void go_fused(const OpArgs<T>& a, const FusedArgs& f, int face, int nb,
 
 template <typename T>
 static void dispatch_fused(const OpArgs<T>& a, const FusedArgs& f, int lim,
-                           int face, int nb, size_t lds, hipStream_t st) {
+                           int face, int nb, size_t lds, hipStream_t st,
+                           int tb) {
   if (a.S == 3) {
-    if (lim == 1) go_fused<T, 1, 3>(a, f, face, nb, lds, st);
-    else go_fused<T, 2, 3>(a, f, face, nb, lds, st);
+    if (lim == 1) go_fused<T, 1, 3>(a, f, face, nb, lds, st, tb);
+    else go_fused<T, 2, 3>(a, f, face, nb, lds, st, tb);
   } else {
-    if (lim == 1) go_fused<T, 1, 4>(a, f, face, nb, lds, st);
-    else go_fused<T, 2, 4>(a, f, face, nb, lds, st);
+    if (lim == 1) go_fused<T, 1, 4>(a, f, face, nb, lds, st, tb);
+    else go_fused<T, 2, 4>(a, f, face, nb, lds, st, tb);
   }
 }
 
@@ -93,12 +97,12 @@ void CCFD_CAT(CCFD_CAT(ccfd_launch_flux_, FLUX_CODE), _f)(
 
 void CCFD_CAT(CCFD_CAT(ccfd_launch_fused_, FLUX_CODE), _d)(
     const ccfd::OpArgs<double>& a, const ccfd::FusedArgs& f, int lim, int face,
-    int nb, size_t lds, hipStream_t st) {
-  ccfd::dispatch_fused<double>(a, f, lim, face, nb, lds, st);
+    int nb, size_t lds, hipStream_t st, int tb) {
+  ccfd::dispatch_fused<double>(a, f, lim, face, nb, lds, st, tb);
 }
 
 void CCFD_CAT(CCFD_CAT(ccfd_launch_fused_, FLUX_CODE), _f)(
     const ccfd::OpArgs<float>& a, const ccfd::FusedArgs& f, int lim, int face,
-    int nb, size_t lds, hipStream_t st) {
-  ccfd::dispatch_fused<float>(a, f, lim, face, nb, lds, st);
+    int nb, size_t lds, hipStream_t st, int tb) {
+  ccfd::dispatch_fused<float>(a, f, lim, face, nb, lds, st, tb);
 }

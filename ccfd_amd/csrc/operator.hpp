@@ -710,6 +710,34 @@ __global__ __launch_bounds__(kBlock) void jac1_kernel(
   }
 }
 
+// the partials of block_sum4 for a tile of TB threads covering TB / kBlock
+// layout blocks from element `base`: one partial per layout block that
+// holds elements, summed exactly as block_sum4 sums a block of kBlock
+// threads (so the residual does not depend on the tile)
+template <int TB>
+__device__ inline void block_sum4_tile(double v[4], double* res_part, int base,
+                                       int N, int nblocks) {
+  __shared__ double red[4][TB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[k][wid] = x;
+  }
+  __syncthreads();
+  constexpr int W = kBlock / 64, H = TB / kBlock;
+  if (threadIdx.x < 4 * H) {
+    const int v = threadIdx.x & 3, hb = threadIdx.x >> 2;
+    if (base + hb * kBlock < N) {
+      double s = 0.0;
+      for (int w = 0; w < W; ++w) s += red[v][hb * W + w];
+      res_part[v * nblocks + base / kBlock + hb] = s;
+    }
+  }
+}
+
 // launch entry of one flux code (defined in op_flux.hip, one object per code)
 template <typename T>
 void launch_flux_code(int code, const OpArgs<T>& a, int nblocks,

@@ -133,14 +133,22 @@ void build_soa(const ccfd_mesh* m, int S, int elemBase, int ghostBase,
   }
 
   // halo of each block: elements of other blocks sharing a face with it
-  o.haloStart.assign(o.nb + 1, 0);
-  o.haloCells.clear();
-  o.maxHalo = 0;
+  build_halo(o, kBlock, elemBase, o.haloStart, o.haloCells, o.maxHalo);
+}
+
+void build_halo(const MeshSoA& o, int tileElems, int elemBase,
+                std::vector<int>& start, std::vector<int>& cells,
+                int& maxHalo) {
+  const int N = o.N, Np = o.Np, S = o.S;
+  const int nt = (N + tileElems - 1) / tileElems;
+  start.assign(nt + 1, 0);
+  cells.clear();
+  maxHalo = 0;
   std::vector<int> tmp;
-  for (int b = 0; b < o.nb; ++b) {
+  for (int b = 0; b < nt; ++b) {
     tmp.clear();
-    const int lo = b * kBlock + elemBase, hi = lo + kBlock;
-    for (int i = b * kBlock; i < std::min(N, (b + 1) * kBlock); ++i)
+    const int lo = b * tileElems + elemBase, hi = lo + tileElems;
+    for (int i = b * tileElems; i < std::min(N, (b + 1) * tileElems); ++i)
       for (int k = 0; k < S; ++k) {
         const int l = o.link[(size_t)k * Np + i];
         if (l < 0) continue;
@@ -149,9 +157,9 @@ void build_soa(const ccfd_mesh* m, int S, int elemBase, int ghostBase,
       }
     std::sort(tmp.begin(), tmp.end());
     tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-    o.haloCells.insert(o.haloCells.end(), tmp.begin(), tmp.end());
-    o.haloStart[b + 1] = (int)o.haloCells.size();
-    o.maxHalo = std::max(o.maxHalo, (int)tmp.size());
+    cells.insert(cells.end(), tmp.begin(), tmp.end());
+    start[b + 1] = (int)cells.size();
+    maxHalo = std::max(maxHalo, (int)tmp.size());
   }
 }
 

@@ -38,4 +38,12 @@ void sfc_order(const ccfd_mesh* m, std::vector<int>& perm);
 void build_soa(const ccfd_mesh* m, int S, int elemBase, int ghostBase,
                int bcBase, MeshSoA& out);
 
+// halo lists of tiles of tileElems consecutive elements (a multiple of the
+// block size): per tile the ascending ids (offset by elemBase) of the
+// elements outside it that share a face with it; start has one entry per
+// tile plus one
+void build_halo(const MeshSoA& o, int tileElems, int elemBase,
+                std::vector<int>& start, std::vector<int>& cells,
+                int& maxHalo);
+
 }  // namespace ccfd

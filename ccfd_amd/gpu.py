@@ -97,6 +97,10 @@ def lib():
                                                C.c_double, dp, dp, ip, ip,
                                                ip]),
         "ccfd_gpu_get_variant": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "ccfd_gpu_set_tile": (C.c_int, [vp, C.c_int]),
+        "ccfd_gpu_get_tile": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "ccfd_gpu_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), dp]),
         "ccfd_gpu_profile_steps": (C.c_int, [vp, C.c_int, C.c_double,
                                              C.c_double, C.c_int, dp, ip,
                                              dp]),
@@ -164,6 +168,28 @@ class Solver:
         """The implementation the next operator call runs."""
         v = C.c_int()
         check(self.L.ccfd_gpu_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    def set_tile(self, elems):
+        """elements per block of the face-once kernel: 0 (automatic), 256
+        or 512"""
+        check(self.L.ccfd_gpu_set_tile(self.h, elems))
+
+    def autotune(self, reps=5):
+        """time the operator implementations and tiles on the resident
+        state and keep the fastest: (variant, tile, {name: ms per call})"""
+        v, t = C.c_int(), C.c_int()
+        ms = np.zeros(4)
+        check(self.L.ccfd_gpu_autotune(self.h, reps, C.byref(v), C.byref(t),
+                                       N.dptr(ms)))
+        names = ("split", "fused", "fused_face_256", "fused_face_512")
+        return v.value, t.value, {n: (m if m >= 0 else None)
+                                  for n, m in zip(names, ms.tolist())}
+
+    def tile(self):
+        """the tile the next operator call runs"""
+        v = C.c_int()
+        check(self.L.ccfd_gpu_get_tile(self.h, C.byref(v)))
         return v.value
 
     def close(self):

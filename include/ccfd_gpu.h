@@ -269,6 +269,26 @@ enum ccfd_kernel_variant {
 };
 int ccfd_gpu_set_variant(ccfd_gpu_ctx *ctx, int variant);
 int ccfd_gpu_get_variant(ccfd_gpu_ctx *ctx, int *variant);
+/*
+ * Elements per block of the face-once kernel: 256, 512 (a wider tile, whose
+ * halo is a smaller share of the elements it reconstructs, used where its
+ * LDS fits the device) or 0 (automatic: 256). Every tile gives the same
+ * results.
+ * get reports the tile the next operator call runs (256 for the other
+ * implementations).
+ */
+int ccfd_gpu_set_tile(ccfd_gpu_ctx *ctx, int elems);
+int ccfd_gpu_get_tile(ccfd_gpu_ctx *ctx, int *elems);
+/*
+ * Autotuner: time every applicable operator implementation and face-once
+ * tile (reps fvTimeDerivative calls each on the resident state, which is
+ * not changed) and keep the fastest (a candidate later in the list has to
+ * be 2% faster than the one kept). ms[0..3]: time per call of split,
+ * fused, face-once 256 and face-once 512 (-1 where one does not apply);
+ * variant and tile get the choice.
+ */
+int ccfd_gpu_autotune(ccfd_gpu_ctx *ctx, int reps, int *variant, int *tile,
+		double ms[4]);
 
 /* ---- timing helpers ---- */
 /*

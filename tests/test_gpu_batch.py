@@ -254,3 +254,43 @@ def test_wide_halo_instance_switches_to_the_split_operator():
         assert np.array_equal(b.cons(k), ref.cons(r))
     b.close()
     ref.close()
+
+
+@pytest.mark.parametrize("sides,prec", [(3, G.FP32_FLUX), (4, G.FP32_FLUX),
+                                        (3, G.FP64)])
+def test_arena_capacity_fits_32_bit_indices(sides, prec):
+    """the kernels index the [4][S][E] side-state arrays with 32-bit ints:
+    however large the budget, an arena holds at most E elements with
+    4 S E < 2^31 (a 64 GiB FP32 triangle arena would otherwise hold 214M
+    elements, whose split-operator indices wrap)"""
+    b = BT.Batch(G.params(flux=5, order=2, limiter=1, precision=prec),
+                 max_sides=sides, hbm_bytes=64 << 30, max_instances=8)
+    _, cap, _ = b.usage()
+    b.close()
+    print("capacity %d elements" % cap)
+    assert cap > 0 and 4 * sides * cap < 2 ** 31
+
+
+def test_split_steps_after_a_freed_context_match_a_fresh_arena():
+    """a context that ran and was freed does not change what an arena
+    computes afterwards, on every operator implementation the autotuner
+    times (the split operator's side states included)"""
+    mesh = cases.delaunay_mesh(60, seed=3, periodic=False)
+    prim = BT.riemann4_state(mesh.bary, 11)
+    prm = G.params(flux=5, order=2, limiter=1, nrk=3, torder=2, cfl=0.9,
+                   precision=G.FP32_FLUX)
+    big = cases.delaunay_mesh(300, seed=4, periodic=True)
+    s = G.Solver(big, G.params(flux=5, order=2, limiter=2, venk_k=5.0))
+    s.set_prim(M.smooth_state(big.bary, seed=1))
+    s.advance(3, 0.0, 1e300, 1e300)
+    s.close()
+    out = []
+    for variant in (G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE):
+        b = BT.Batch(prm, max_sides=3, hbm_bytes=64 << 30, max_instances=8)
+        b.set_variant(variant)
+        k = b.add(mesh, prim)
+        b.step(20)
+        out.append(b.prim(k))
+        b.close()
+    assert np.isfinite(out[0]).all()
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])

@@ -294,17 +294,22 @@ def test_advance_device_dt_matches_host_loop(variant):
 @pytest.mark.parametrize("n,periodic", [(120, True), (300, True),
                                         (200, False)])
 def test_fused_equals_split_large_delaunay(n, periodic):
-    """Fused and split operators agree bit for bit on meshes with many
-    blocks (every block has a halo) through RK4 steps; transmissive
-    boundaries when not periodic."""
+    """Fused and split operators, and the face-once kernel with tiles of
+    256 and 512 elements, agree bit for bit on meshes with many blocks
+    (every block has a halo) through RK4 steps, residuals included;
+    transmissive boundaries when not periodic."""
     mesh = cases.delaunay_mesh(n, seed=7, periodic=periodic)
     prm = G.params(flux=5, order=2, limiter=2, venk_k=5.0, nrk=4, torder=2)
     prim = M.smooth_state(mesh.bary, seed=9)
     out = []
-    for v in (G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE):
+    for v, tile in ((G.KV_SPLIT, 0), (G.KV_FUSED, 0), (G.KV_FUSED_FACE, 256),
+                    (G.KV_FUSED_FACE, 512)):
         a = G.Solver(mesh, prm)
         a.set_variant(v)
         assert a.variant() == v
+        if tile:
+            a.set_tile(tile)
+            assert a.tile() == tile
         a.set_prim(prim)
         t, r = a.advance(3, 0.0, 1e9, 1e9)
         out.append((t, r, a.cons(), a.ut()))
@@ -349,4 +354,30 @@ def test_large_delaunay_matches_the_oracle(large_delaunay, flux):
         assert t == to
         assert np.allclose(r, ro, rtol=1e-12, atol=0.0)
     assert np.array_equal(gs.cons(), orc.cons())
+    gs.close()
+
+
+def test_autotune_keeps_the_state_and_picks_a_timed_candidate():
+    """ccfd_gpu_autotune times every applicable implementation and tile on
+    the resident state without changing it, keeps the fastest, and the
+    operator then gives the same dU/dt as before"""
+    mesh = cases.delaunay_mesh(150, seed=5, periodic=True)
+    prim = M.smooth_state(mesh.bary, seed=4)
+    gs = G.Solver(mesh, G.params(flux=5, order=2, limiter=2, venk_k=5.0))
+    gs.set_prim(prim)
+    gs.fvtd(0.0)
+    ut0 = gs.ut()
+    v, tile, ms = gs.autotune(3)
+    print("autotune: variant %d tile %d %s" % (v, tile, ms))
+    timed = {k: t for k, t in ms.items() if t is not None}
+    assert {"split", "fused", "fused_face_256"} <= set(timed)
+    assert all(t > 0 for t in timed.values())
+    name = {G.KV_SPLIT: "split", G.KV_FUSED: "fused"}.get(
+        v, "fused_face_%d" % tile)
+    assert timed[name] <= min(timed.values()) / 0.98
+    assert gs.variant() == v and gs.tile() == tile
+    assert np.array_equal(gs.prim(), prim) or np.allclose(gs.prim(), prim,
+                                                          rtol=1e-15)
+    gs.fvtd(0.0)
+    assert np.array_equal(gs.ut(), ut0)
     gs.close()
