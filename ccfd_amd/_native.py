@@ -81,11 +81,13 @@ def host():
         lib.ccfd_gmsh_write.argtypes = [
             C.c_char_p, C.c_long, c_double_p, C.c_long, c_long_p, C.c_long,
             c_long_p, C.c_long, c_long_p]
-        lib.ccfd_gmsh_read.restype = C.c_int
-        lib.ccfd_gmsh_read.argtypes = [
-            C.c_char_p, c_long_p, C.POINTER(c_double_p), c_long_p,
-            C.POINTER(c_long_p), c_long_p, C.POINTER(c_long_p), c_long_p,
-            C.POINTER(c_long_p), C.c_char_p]
+        for name in ("ccfd_gmsh_read", "ccfd_emc2_read"):
+            f = getattr(lib, name)
+            f.restype = C.c_int
+            f.argtypes = [
+                C.c_char_p, c_long_p, C.POINTER(c_double_p), c_long_p,
+                C.POINTER(c_long_p), c_long_p, C.POINTER(c_long_p), c_long_p,
+                C.POINTER(c_long_p), C.c_char_p]
         lib._typed = True
     return lib
 

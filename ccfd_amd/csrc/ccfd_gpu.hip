@@ -342,6 +342,16 @@ void* ccfd::dmalloc(ccfd_gpu_ctx* c, size_t bytes) {
   return p;
 }
 
+void ccfd::dfree(ccfd_gpu_ctx* c, void* p) {
+  if (!p) return;
+  for (size_t k = 0; k < c->allocs.size(); ++k)
+    if (c->allocs[k] == p) {
+      hipFree(p);
+      c->allocs.erase(c->allocs.begin() + k);
+      return;
+    }
+}
+
 static void free_all(ccfd_gpu_ctx* c) {
   for (void* p : c->allocs) hipFree(p);
   c->allocs.clear();
@@ -519,6 +529,12 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   ccfd::implicit_free(c);
   free_all(c);
   c->hasMesh = false;
+  c->bcIn = nullptr; c->src = nullptr; c->q5 = nullptr;
+  c->wingG = nullptr; c->wingN = nullptr; c->wingOut = nullptr;
+  c->nWing = c->nWingP = 0;
+  c->recIdx = nullptr; c->recOut = nullptr; c->recCap = 0;
+  c->errPart = nullptr;
+  c->hq5.clear();
   int rc = apply_params(c, p);
   if (rc) return rc;
 
@@ -610,9 +626,32 @@ extern "C" int ccfd_gpu_set_mesh(ccfd_gpu_ctx* c, const ccfd_mesh* m,
   // per-block partials (4 per block) and the residual's middle stage
   c->part = (double*)dmalloc(c, (4 * (size_t)c->nb + 4 * kResMid + 8) * 8);
   c->scal = (double*)dmalloc(c, 64 * 8);
+  c->bcIn = (double*)dmalloc(c, 4 * (size_t)std::max(G, 1) * 8);
+  if (c->srcFunc) c->src = dmalloc(c, e4 * 8);
   if (!c->prim[0] || !c->prim[1] || !c->U || !c->U0 || !c->ut || !c->ux ||
-      !c->uy || !c->side || !c->ghostP || !c->ghostS || !c->stage || !c->part || !c->scal)
+      !c->uy || !c->side || !c->ghostP || !c->ghostS || !c->stage || !c->part ||
+      !c->scal || !c->bcIn || (c->srcFunc && !c->src))
     return fail(CCFD_ENOMEM, "device state allocation failed");
+  // centre points of the quadrilaterals (the fifth quadrature point,
+  // src/mesh.c:157-158), for error norms and source terms
+  if (m->nQuads > 0) {
+    c->hq5.assign(2 * (size_t)Np, 0.0);
+    for (int e = 0; e < N; ++e) {
+      if (m->elemType[e] != 4) continue;
+      const double* a0 = m->xy + 2 * m->elemNode[4 * (size_t)e];
+      const double* a2 = m->xy + 2 * m->elemNode[4 * (size_t)e + 2];
+      const int i = c->inv[e];
+      c->hq5[i] = 0.5 * (a0[0] + a2[0]);
+      c->hq5[Np + i] = 0.5 * (a0[1] + a2[1]);
+    }
+    // the source kernel reads them at every operator call
+    if (c->srcFunc) {
+      c->q5 = (double*)dmalloc(c, c->hq5.size() * 8);
+      if (!c->q5 || hipMemcpyAsync(c->q5, c->hq5.data(), c->hq5.size() * 8,
+                                   hipMemcpyHostToDevice, c->st) != hipSuccess)
+        return fail(CCFD_ENOMEM, "quadrature point upload failed");
+    }
+  }
   c->cur = 0;
   c->hasMesh = true;
   if ((rc = upload_venk(c))) return rc;
@@ -735,13 +774,13 @@ static void exact_ghosts(ccfd_gpu_ctx* c, const OpArgs<T>& a, int side) {
                        0, c->st, a, side);
 }
 
-// limited gradients and side states of the current state at the time of
-// the last ccfd_gpu_fvtd call, into c->ux, c->uy, c->side (the fused
-// operator does not materialise them)
+// limited gradients and side states of the current state at the given
+// time, into c->ux, c->uy, c->side (the fused operator does not materialise
+// them)
 template <typename T>
-static void recompute_recon(ccfd_gpu_ctx* c) {
+void ccfd::recompute_recon(ccfd_gpu_ctx* c, double time) {
   OpArgs<T> a = op_args<T>(c);
-  a.time0 = c->lastTime;
+  a.time0 = time;
   if (c->G > 0)
     hipLaunchKernelGGL(bc_bary_kernel<T>, dim3((c->G + 127) / 128), dim3(128),
                        0, c->st, a);
@@ -756,6 +795,8 @@ static void recompute_recon(ccfd_gpu_ctx* c) {
     else hipLaunchKernelGGL((recon_kernel<T, 2, 4>), g, b, 0, c->st, a);
   }
 }
+template void ccfd::recompute_recon<double>(ccfd_gpu_ctx*, double);
+template void ccfd::recompute_recon<float>(ccfd_gpu_ctx*, double);
 
 extern "C" int ccfd_gpu_get_recon(ccfd_gpu_ctx* c, double* u_x, double* u_y,
                                   double* sideP) {
@@ -763,8 +804,8 @@ extern "C" int ccfd_gpu_get_recon(ccfd_gpu_ctx* c, double* u_x, double* u_y,
   hipSetDevice(c->device);
   int rc;
   if (c->prm.spatialOrder == 2) {
-    if (c->prm.precision == CCFD_FP64) recompute_recon<double>(c);
-    else recompute_recon<float>(c);
+    if (c->prm.precision == CCFD_FP64) recompute_recon<double>(c, c->lastTime);
+    else recompute_recon<float>(c, c->lastTime);
   }
   if (u_x && (rc = download_aos(c, as_double(c, c->ux, 4 * (size_t)c->Np), u_x))) return rc;
   if (u_y && (rc = download_aos(c, as_double(c, c->uy, 4 * (size_t)c->Np), u_y))) return rc;
@@ -814,6 +855,7 @@ OpArgs<T> ccfd::op_args(ccfd_gpu_ctx* c) {
   a.ghostBC = c->ghostBC; a.ghostBary = c->ghostBary;
   a.ghostCell = c->ghostCell; a.ghostK = c->ghostK;
   a.ghostS = (T*)c->ghostS;
+  a.bcIn = c->bcIn;
   a.prim = (const T*)c->prim[c->cur];
   a.side = (T*)c->side;
   a.ux = (T*)c->ux; a.uy = (T*)c->uy;
@@ -878,6 +920,7 @@ template <typename T>
 void ccfd::launch_operator(ccfd_gpu_ctx* c, OpArgs<T>& a) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int kv = active_variant(c, sizeof(T));
+  if (c->srcFunc) launch_source<T>(c, a);
   if (kv != CCFD_KV_SPLIT) {
     if (c->G > 0) {
       hipEvent_t eb = nullptr;

@@ -91,6 +91,23 @@ struct ccfd_gpu_ctx {
   std::vector<Span> spans;
   std::vector<void*> allocs;
   ccfd::ImplicitState* imp = nullptr;
+  // analysis consumers and source term (analyze.hip)
+  double* bcIn = nullptr;     // [4][G] interior side states at the boundary
+                              // sides from the last operator evaluation
+  int srcFunc = 0;            // source function (0: none)
+  void* src = nullptr;        // [4][Np] source term of the last evaluation
+  std::vector<double> hq5;    // [2][Np] quadrilateral centre points (host;
+                              // empty for triangle meshes)
+  double* q5 = nullptr;       // the same on the device, uploaded on first use
+  int nWing = 0, nWingP = 0;  // calcCoef sides (pressure side first)
+  int* wingG = nullptr;       // ghost of each listed side
+  double* wingN = nullptr;    // [3][nWing] nx, ny, len of the interior side
+  double* wingOut = nullptr;  // [2 + nWing] cl, cd, cp
+  double refLength = 1.0;
+  int recCap = 0;             // record points (gather buffers)
+  int* recIdx = nullptr;
+  double* recOut = nullptr;
+  double* errPart = nullptr;  // [12][nb] error-norm partials
 };
 
 namespace ccfd {
@@ -128,5 +145,15 @@ void tspan_end(ccfd_gpu_ctx* c, hipEvent_t a, int kind);
 bool fused_applies(const ccfd_gpu_ctx* c, size_t tsize);
 int face_tile(const ccfd_gpu_ctx* c, size_t tsize);
 int active_variant(const ccfd_gpu_ctx* c, size_t tsize);
+// limited gradients (c->ux, c->uy) and side states (c->side) of the current
+// state at the given time, by the split kernels
+template <typename T>
+void recompute_recon(ccfd_gpu_ctx* c, double time);
+// source term of the operator call a is about to make (analyze.hip); a.src
+// is set when the context has a source function
+template <typename T>
+void launch_source(ccfd_gpu_ctx* c, OpArgs<T>& a);
+// free one allocation made by dmalloc
+void dfree(ccfd_gpu_ctx* c, void* p);
 
 }  // namespace ccfd

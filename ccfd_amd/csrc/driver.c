@@ -6,7 +6,9 @@
  * initBoundary, initMesh, initInitialCondition, initFV, initTimeDisc,
  * initLinearSolver), the mesh is preprocessed on the host (mesh.c) and the
  * state lives on the device; timeDisc reproduces the reference's loop
- * (src/timeDiscretization.c:559-777) with CSV output.
+ * (src/timeDiscretization.c:559-777) with CSV output and the analysis
+ * consumers of src/analyze.c (residual file, CL / CD / CP, record points,
+ * error norms).
  */
 #include <math.h>
 #include <stdio.h>
@@ -37,6 +39,24 @@ static struct {
 	double abortResidual;
 	int abortVariable;
 	long nNewton, nGMRES;
+	double dt;                 /* the last time step (elem->dt) */
+	int sourceFunc;
+	int icType, exactFunc;
+	double alpha, ref0[4];     /* refState[0] and alpha of the IC */
+	int haveRef;
+	/* analysis (src/analyze.c, src/timeDiscretization.c:180-213) */
+	int hasExactSolution, calcWing;
+	double refLength;
+	int wallId;
+	double clAbortResidual, cdAbortResidual;
+	int abortOnCl, abortOnCd;
+	char abortName[4];
+	long nWingP, nWingS;
+	double *wingXY, *cp, cl, cd;
+	long nRec;
+	long *recElem;
+	FILE **recFile;
+	FILE *resFile;
 } C;
 
 static double wall(void)
@@ -50,6 +70,76 @@ ccfd_gpu_ctx *ccfd_case_context(void) { return C.ctx; }
 const ccfd_mesh *ccfd_case_mesh(void) { return C.mesh; }
 double ccfd_case_time(void) { return C.t; }
 long ccfd_case_iterations(void) { return C.iter; }
+
+/* initAnalyze's files (src/analyze.c:200-282): the residual history for
+ * stationary or wing runs, gnuplot scripts, one file per record point */
+static int open_analysis_files(char *e)
+{
+	char name[600];
+	if (C.calcWing || C.stationary) {
+		snprintf(name, sizeof(name), "%s_analysis.csv", C.fileName);
+		C.resFile = fopen(name, "w");
+		if (!C.resFile) {
+			snprintf(e, 256, "cannot open %s", name);
+			return -1;
+		}
+		if (C.calcWing)
+			fprintf(C.resFile, "Iter, Time, Residual(%s), CL, CD\n", C.abortName);
+		else
+			fprintf(C.resFile, "Iter, Time, Residual(RHO), Residual(VX), Residual(VY), Residual(E)\n");
+		char res[600];
+		snprintf(res, sizeof(res), "%s", name);
+		snprintf(name, sizeof(name), "%s_residuals.dem", C.fileName);
+		FILE *d = fopen(name, "w");
+		if (d) {
+			fprintf(d, "set title 'Residual Plot'\n");
+			fprintf(d, "set logscale y\n");
+			if (C.calcWing) {
+				fprintf(d, "plot '%s' using 1:3 title '%s' with lines\n", res, C.abortName);
+			} else {
+				fprintf(d, "plot '%s' using 1:3 title 'RHO' with lines, \\\n", res);
+				fprintf(d, "     '%s' using 1:4 title 'MX' with lines, \\\n", res);
+				fprintf(d, "     '%s' using 1:5 title 'MY' with lines, \\\n", res);
+				fprintf(d, "     '%s' using 1:6 title 'E' with lines\n", res);
+			}
+			fprintf(d, "pause -1");
+			fclose(d);
+		}
+		if (C.calcWing) {
+			snprintf(name, sizeof(name), "%s_CLCD.dem", C.fileName);
+			d = fopen(name, "w");
+			if (d) {
+				fprintf(d, "set title 'CL/CD Plot'\n");
+				fprintf(d, "plot '%s' using 1:4 title 'CL' with lines, \\\n", res);
+				fprintf(d, "     '%s' using 1:5 title 'CD' with lines\n", res);
+				fprintf(d, "pause -1");
+				fclose(d);
+			}
+		}
+	}
+	for (long k = 0; k < C.nRec; ++k) {
+		snprintf(name, sizeof(name), "%s_recordPoint_%ld.csv", C.fileName, k);
+		C.recFile[k] = fopen(name, "w");
+		if (!C.recFile[k]) {
+			snprintf(e, 256, "cannot open %s", name);
+			return -1;
+		}
+		fprintf(C.recFile[k], "Time, Density, VelocityX, VelocityY, Pressure\n");
+	}
+	return 0;
+}
+
+static void close_analysis_files(void)
+{
+	if (C.resFile)
+		fclose(C.resFile);
+	C.resFile = NULL;
+	for (long k = 0; k < C.nRec; ++k)
+		if (C.recFile && C.recFile[k]) {
+			fclose(C.recFile[k]);
+			C.recFile[k] = NULL;
+		}
+}
 
 #define NEED(x, what) do { if ((x) != 0) { snprintf(e, 256, "missing ini key %s", what); goto fail; } } while (0)
 
@@ -91,8 +181,9 @@ int ccfd_init(const char *iniFile, int device, char *err)
 		int src;
 		NEED(ccfd_ini_bool(ini, "calcSource", "F", &src), "calcSource");
 		if (src) {
-			snprintf(e, 256, "source terms are not supported");
-			goto fail;
+			long sf;
+			NEED(ccfd_ini_long(ini, "sourceFunction", NULL, &sf), "sourceFunction");
+			C.sourceFunc = (int)sf;
 		}
 	}
 	p->mu = 0.0;
@@ -193,8 +284,12 @@ int ccfd_init(const char *iniFile, int device, char *err)
 		char fmt[64], file[256], path[600];
 		NEED(ccfd_ini_str(ini, "meshFormat", NULL, fmt, sizeof(fmt)), "meshFormat");
 		NEED(ccfd_ini_str(ini, "meshFile", NULL, file, sizeof(file)), "meshFile");
-		if (strncmp(fmt, ".msh", 4) != 0) {
-			snprintf(e, 256, "mesh format %s not supported (gmsh v2 only)", fmt);
+		/* src/mesh.c:1419-1440: .msh / .msh2 / .msh4 gmsh, .mesh EMC2;
+		 * CGNS (.cgns) is not read on this path */
+		int gmsh = !strcmp(fmt, ".msh") || !strcmp(fmt, ".msh2") ||
+			!strcmp(fmt, ".msh4");
+		if (!gmsh && strcmp(fmt, ".mesh") != 0) {
+			snprintf(e, 256, "mesh format %s not supported (gmsh or EMC2)", fmt);
 			goto fail;
 		}
 		/* relative to the ini file's directory */
@@ -204,7 +299,8 @@ int ccfd_init(const char *iniFile, int device, char *err)
 					iniFile, file, fmt);
 		else
 			snprintf(path, sizeof(path), "%s%s", file, fmt);
-		if (ccfd_gmsh_read(path, &nv, &xy, &nt, &tri, &nq, &quad, &nb, &bce, e))
+		if ((gmsh ? ccfd_gmsh_read : ccfd_emc2_read)(path, &nv, &xy, &nt,
+				&tri, &nq, &quad, &nb, &bce, e))
 			goto fail;
 	}
 	C.mesh = ccfd_mesh_build(nv, xy, nt, tri, nq, quad, nb, bce, (int)nBC, bcs, e);
@@ -215,25 +311,30 @@ int ccfd_init(const char *iniFile, int device, char *err)
 	NEED(ccfd_ini_long(ini, "icType", NULL, &icType), "icType");
 	long N = C.mesh->nElems;
 	prim = calloc(4 * N, sizeof(double));
+	C.icType = (int)icType;
 	if (icType == 2) {
 		long ef;
 		NEED(ccfd_ini_long(ini, "exactFunc", NULL, &ef), "exactFunc");
-		if (ef != 5) {
-			snprintf(e, 256, "initial exact function %ld not supported", ef);
+		if (ef < 1 || ef > 6) {
+			snprintf(e, 256, "unknown exact function %ld", ef);
 			goto fail;
 		}
-		NEED(ccfd_ini_dbl(ini, "RP_1D_interface", "0.5", &p->rp1Dinterface), "RP_1D_interface");
-		NEED(ccfd_ini_dbl_array(ini, "StateLeft", 4, NULL, p->refState[0]), "StateLeft");
-		NEED(ccfd_ini_dbl_array(ini, "StateRight", 4, NULL, p->refState[1]), "StateRight");
-		/* exactFunc 5 at t = 0 (src/exactFunction.c:131-150) */
-		for (long i = 0; i < N; ++i) {
-			const double *s = C.mesh->bary[2 * i] <= p->rp1Dinterface ?
-				p->refState[0] : p->refState[1];
-			prim[4 * i] = s[0];
-			prim[4 * i + 1] = s[1];
-			prim[4 * i + 2] = 0.0;
-			prim[4 * i + 3] = s[3];
+		C.exactFunc = (int)ef;
+		if (ef == 5) {
+			NEED(ccfd_ini_dbl(ini, "RP_1D_interface", "0.5", &p->rp1Dinterface), "RP_1D_interface");
+			NEED(ccfd_ini_dbl_array(ini, "StateLeft", 4, NULL, p->refState[0]), "StateLeft");
+			NEED(ccfd_ini_dbl_array(ini, "StateRight", 4, NULL, p->refState[1]), "StateRight");
+			memcpy(C.ref0, p->refState[0], sizeof(C.ref0));
+			C.haveRef = 1;
 		}
+		/* setInitialCondition (src/initialCondition.c:205-211): the
+		 * exact function at the barycentres, t = 0 */
+		for (long i = 0; i < N; ++i)
+			if (ccfd_exact_func((int)ef, p, C.mesh, C.mesh->bary + 2 * i,
+					0.0, prim + 4 * i)) {
+				snprintf(e, 256, "exact function %ld failed", ef);
+				goto fail;
+			}
 	} else if (icType == 1) {
 		long nd;
 		NEED(ccfd_ini_long(ini, "nDomains", "1", &nd), "nDomains");
@@ -255,7 +356,10 @@ int ccfd_init(const char *iniFile, int device, char *err)
 			st[d][1] = v * cos(alpha * pi / 180.0);
 			st[d][2] = v * sin(alpha * pi / 180.0);
 			st[d][3] = pr;
+			C.alpha = alpha;   /* the global keeps the last domain's */
 		}
+		memcpy(C.ref0, st[0], sizeof(C.ref0));
+		C.haveRef = 1;
 		for (long i = 0; i < N; ++i) {
 			int d = nd == 1 ? 0 : C.mesh->domain[i] - 1;
 			if (d < 0 || d >= nd) {
@@ -292,9 +396,27 @@ int ccfd_init(const char *iniFile, int device, char *err)
 	p->nRKstages = (int)nrk;
 	if (C.stationary) {
 		long av;
+		static const char *names[4] = {"RHO", "MX", "MY", "E"};
 		NEED(ccfd_ini_dbl(ini, "abortResidual", "1e-6", &C.abortResidual), "abortResidual");
 		NEED(ccfd_ini_long(ini, "abortVariable", "1", &av), "abortVariable");
+		if (av < 1 || av > 4) {
+			snprintf(e, 256, "abortVariable %ld out of range", av);
+			goto fail;
+		}
 		C.abortVariable = (int)av - 1;
+		snprintf(C.abortName, sizeof(C.abortName), "%s", names[av - 1]);
+		/* src/timeDiscretization.c:202-217 */
+		NEED(ccfd_ini_dbl(ini, "cl_abortResidual", "0.0", &C.clAbortResidual), "cl_abortResidual");
+		NEED(ccfd_ini_dbl(ini, "cd_abortResidual", "0.0", &C.cdAbortResidual), "cd_abortResidual");
+		if (C.clAbortResidual == 0.0 && C.cdAbortResidual == 0.0) {
+		} else if (C.clAbortResidual > 0.0) {
+			C.abortOnCl = 1;
+		} else if (C.cdAbortResidual > 0.0) {
+			C.abortOnCd = 1;
+		} else {
+			snprintf(e, 256, "wrong definition of abort residual");
+			goto fail;
+		}
 	}
 	NEED(ccfd_ini_long(ini, "maxIter", "100000", &C.maxIter), "maxIter");
 	NEED(ccfd_ini_dbl(ini, "tEnd", NULL, &C.stopTime), "tEnd");
@@ -319,10 +441,68 @@ int ccfd_init(const char *iniFile, int device, char *err)
 		rc = ccfd_gpu_set_mesh(C.ctx, C.mesh, p);
 	if (!rc)
 		rc = ccfd_gpu_set_prim(C.ctx, prim);
+	if (!rc && C.sourceFunc)
+		rc = ccfd_gpu_set_source(C.ctx, C.sourceFunc);
 	if (rc) {
 		snprintf(e, 256, "%s", ccfd_gpu_last_error());
 		goto fail;
 	}
+	/* initAnalyze (src/analyze.c:197-282) */
+	NEED(ccfd_ini_bool(ini, "exactSolution", "F", &C.hasExactSolution), "exactSolution");
+	NEED(ccfd_ini_bool(ini, "calcWing", "F", &C.calcWing), "calcWing");
+	if (C.hasExactSolution && C.icType != 2) {
+		snprintf(e, 256, "exactSolution needs an exact-function initial condition");
+		goto fail;
+	}
+	if (C.calcWing) {
+		long wid;
+		NEED(ccfd_ini_dbl(ini, "referenceLength", "1.0", &C.refLength), "referenceLength");
+		NEED(ccfd_ini_long(ini, "wall_id", NULL, &wid), "wall_id");
+		C.wallId = (int)wid;
+		if (!C.haveRef) {
+			snprintf(e, 256, "calcWing needs a freestream (icType 1) state");
+			goto fail;
+		}
+	}
+	{
+		long nr;
+		NEED(ccfd_ini_long(ini, "nRecordPoints", "0", &nr), "nRecordPoints");
+		if (nr < 0) {
+			snprintf(e, 256, "nRecordPoints %ld", nr);
+			goto fail;
+		}
+		C.nRec = nr;
+		C.recElem = calloc(nr + 1, sizeof(long));
+		C.recFile = calloc(nr + 1, sizeof(FILE *));
+		if (!C.recElem || !C.recFile) {
+			snprintf(e, 256, "out of memory");
+			goto fail;
+		}
+		for (long k = 0; k < nr; ++k) {
+			double x[2];
+			NEED(ccfd_ini_dbl_array(ini, "pointCoords", 2, NULL, x), "pointCoords");
+			C.recElem[k] = ccfd_mesh_locate(C.mesh, x);
+			if (C.recElem[k] < 0) {
+				snprintf(e, 256, "Record Point # %ld is not in Domain", k);
+				goto fail;
+			}
+		}
+	}
+	if (C.calcWing) {
+		C.wingXY = malloc(sizeof(double) * 3 * (C.mesh->nGhosts + 1));
+		C.cp = malloc(sizeof(double) * (C.mesh->nGhosts + 1));
+		if (!C.wingXY || !C.cp) {
+			snprintf(e, 256, "out of memory");
+			goto fail;
+		}
+		if (ccfd_gpu_wing_init(C.ctx, C.mesh, C.wallId, C.refLength,
+				&C.nWingP, &C.nWingS, C.wingXY)) {
+			snprintf(e, 256, "%s", ccfd_gpu_last_error());
+			goto fail;
+		}
+	}
+	if (open_analysis_files(e))
+		goto fail;
 	C.t = 0.0;
 	C.ready = 1;
 	free(prim); free(xy); free(tri); free(quad); free(bce);
@@ -339,6 +519,16 @@ fail:
 
 void ccfd_finalize(void)
 {
+	close_analysis_files();
+	free(C.recElem);
+	free(C.recFile);
+	free(C.wingXY);
+	free(C.cp);
+	C.recElem = NULL;
+	C.recFile = NULL;
+	C.wingXY = NULL;
+	C.cp = NULL;
+	C.nRec = 0;
 	if (C.ctx)
 		ccfd_gpu_destroy(C.ctx);
 	if (C.mesh)
@@ -491,6 +681,133 @@ static void data_output(double time, long iter)
 		FAILX("could not write %s", name);
 }
 
+/* ---- analysis consumers, src/analyze.c ---- */
+
+void initAnalyze(void) { need_ready("initAnalyze"); }
+
+/* globalResidual (src/analyze.c:521-539) of the last dU/dt; entries 4 and
+ * 5 (the CL / CD residuals) are cleared, as the reference's memset does */
+void globalResidual(double resIter[6])
+{
+	need_ready("globalResidual");
+	memset(resIter, 0, 6 * sizeof(double));
+	if (ccfd_gpu_residual(C.ctx, resIter))
+		FAILX("globalResidual: %s", ccfd_gpu_last_error());
+}
+
+/* evalRecordPoints (src/analyze.c:393-402) */
+static void eval_record_points(double time)
+{
+	double *pv = malloc(sizeof(double) * 4 * (C.nRec + 1));
+	if (!pv)
+		FAILX("record points: out of memory");
+	if (ccfd_gpu_get_prim_at(C.ctx, (int)C.nRec, C.recElem, pv))
+		FAILX("record points: %s", ccfd_gpu_last_error());
+	for (long k = 0; k < C.nRec; ++k)
+		fprintf(C.recFile[k], "%20.12f,%20.12f,%20.12f,%20.12f,%20.12f\n",
+				time + C.dt, pv[4 * k], pv[4 * k + 1], pv[4 * k + 2],
+				pv[4 * k + 3]);
+	free(pv);
+}
+
+static void write_cp(const char *suffix, const char *head, long k0, long n)
+{
+	char name[600];
+	snprintf(name, sizeof(name), "%s_CP_%s.csv", C.fileName, suffix);
+	FILE *f = fopen(name, "w");
+	if (!f)
+		FAILX("Cannot open Output File for CP I/O");
+	fprintf(f, "x, y, phi, %s\n", head);
+	for (long k = k0; k < k0 + n; ++k)
+		fprintf(f, "%15.7f,%15.7f,%15.7f,%15.7f\n", C.wingXY[3 * k],
+				C.wingXY[3 * k + 1], C.wingXY[3 * k + 2], C.cp[k]);
+	fclose(f);
+}
+
+/* calcCoef (src/analyze.c:292-387): CL and CD on the device, CP files */
+static void calc_coef(void)
+{
+	double coef[2];
+	if (ccfd_gpu_wing_coef(C.ctx, C.ref0, C.alpha, coef, C.cp))
+		FAILX("calcCoef: %s", ccfd_gpu_last_error());
+	write_cp("pressureSide", "CP_pressureSide", 0, C.nWingP);
+	write_cp("suctionSide", "CP_suctionSide", C.nWingP, C.nWingS);
+	C.cl = coef[0];
+	C.cd = coef[1];
+	char name[600], ps[600], ss[600];
+	snprintf(name, sizeof(name), "%s_CP.dem", C.fileName);
+	snprintf(ps, sizeof(ps), "%s_CP_pressureSide.csv", C.fileName);
+	snprintf(ss, sizeof(ss), "%s_CP_suctionSide.csv", C.fileName);
+	FILE *d = fopen(name, "r");
+	if (!d) {
+		d = fopen(name, "w");
+		if (d) {
+			fprintf(d, "unset key\n");
+			fprintf(d, "set xlabel 'x'\n");
+			fprintf(d, "set ylabel 'cp'\n");
+			fprintf(d, "set yrange [*:*] reverse\n");
+			fprintf(d, "plot '%s' using 1:4 w l lc rgb 'black', \\\n", ps);
+			fprintf(d, "     '%s' using 1:4 w l lc rgb 'black'\n", ss);
+			fprintf(d, "pause -1");
+		}
+	}
+	if (d)
+		fclose(d);
+}
+
+/* analyze (src/analyze.c:411-438), after each step at the new time */
+void analyze(double time, long iter, double resIter[6])
+{
+	need_ready("analyze");
+	if (C.nRec > 0)
+		eval_record_points(time);
+	if (C.calcWing) {
+		resIter[4] = C.cl;
+		resIter[5] = C.cd;
+		calc_coef();
+		resIter[4] = fabs(resIter[4] - C.cl) / C.dt;
+		resIter[5] = fabs(resIter[5] - C.cd) / C.dt;
+		fprintf(C.resFile, "%7ld, %13.8f, %15.8e, %15.10f, %15.10f\n", iter,
+				time + C.dt, resIter[C.abortVariable], C.cl, C.cd);
+	} else if (C.stationary) {
+		fprintf(C.resFile, "%7ld, %13.8f, %15.8e, %15.8e, %15.8e, %15.8e\n",
+				iter, time + C.dt, resIter[0], resIter[1], resIter[2],
+				resIter[3]);
+	}
+}
+
+/* calcErrors (src/analyze.c:445-514) against the initial condition's exact
+ * function */
+void calcErrors(double time)
+{
+	need_ready("calcErrors");
+	double L1[4], L2[4], Li[4];
+	if (ccfd_gpu_calc_errors(C.ctx, C.exactFunc, time, L1, L2, Li))
+		FAILX("calcErrors: %s", ccfd_gpu_last_error());
+	printf("\nError Analysis at t = %g\n", time);
+	printf("|                RHO           VX           VY            P\n");
+	printf("| L1  :  %7.5e  %7.5e  %7.5e  %7.5e\n", L1[0], L1[1], L1[2], L1[3]);
+	printf("| L2  :  %7.5e  %7.5e  %7.5e  %7.5e\n", L2[0], L2[1], L2[2], L2[3]);
+	printf("| Linf:  %7.5e  %7.5e  %7.5e  %7.5e\n", Li[0], Li[1], Li[2], Li[3]);
+}
+
+void freeAnalyze(void) { }
+
+static void print_coefficients(void)
+{
+	if (C.calcWing) {
+		printf("| Final Lift and Drag Coefficients:\n");
+		printf("|   CL: %.10g\n", C.cl);
+		printf("|   CD: %.10g\n", C.cd);
+	}
+}
+
+static void calc_dt(double t, double pTime, double *dt, int *vd)
+{
+	if (ccfd_gpu_calc_dt(C.ctx, t, pTime, C.stopTime, dt, vd))
+		FAILX("%s", ccfd_gpu_last_error());
+}
+
 void timeDisc(void)
 {
 	need_ready("timeDisc");
@@ -498,19 +815,19 @@ void timeDisc(void)
 	double printTime = (floor(t / C.IOtimeInterval) + 1) * C.IOtimeInterval;
 	long printIter = (0 / C.IOiterInterval + 1) * C.IOiterInterval;
 	int hasConverged = C.stationary ? 0 : 1;
-	(void)hasConverged;
 	data_output(t, 0);
 	double dt;
 	int vd;
-	double t0 = wall();
-	if (ccfd_gpu_calc_dt(C.ctx, t, printTime, C.stopTime, &dt, &vd))
-		FAILX("%s", ccfd_gpu_last_error());
+	double t0 = wall(), tIO = t0;
+	calc_dt(t, printTime, &dt, &vd);
 	printf("| Initial Time Step: %.10g\n", dt);
+	if (vd)
+		printf("| Viscous Time Step Dominates!\n");
 	long iter;
 	for (iter = 1; iter <= C.maxIter; ++iter) {
-		if (ccfd_gpu_calc_dt(C.ctx, t, printTime, C.stopTime, &dt, &vd))
-			FAILX("%s", ccfd_gpu_last_error());
-		double res[4];
+		calc_dt(t, printTime, &dt, &vd);
+		C.dt = dt;
+		double res[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 		if (!C.implicit) {
 			if (ccfd_gpu_explicit_step(C.ctx, t, dt, res))
 				FAILX("%s", ccfd_gpu_last_error());
@@ -525,25 +842,75 @@ void timeDisc(void)
 		t += dt;
 		C.t = t;
 		C.iter = iter;
+		analyze(t, iter, res);
 		if (C.stopTime - t <= 1e-15) {
 			printf("\nTime Limit Reached - Computation Complete!\n");
 			printf("| Final Time      : %.10g\n", t);
 			printf("| Iteration Number: %ld\n", iter);
+			printf("| Writing Final State to Disk\n");
 			data_output(t, iter);
+			if (C.hasExactSolution)
+				calcErrors(t);
 			break;
 		}
+		/* residual abort criteria, src/timeDiscretization.c:629-688 */
 		if (C.stationary && fabs(res[C.abortVariable]) <= C.abortResidual) {
-			printf("\nConverged - Calculation complete\n");
-			printf("| Iteration number: %ld\n", iter);
+			if (C.abortOnCl) {
+				if (fabs(res[4]) <= C.clAbortResidual) {
+					printf("\nConverged in CL - Calculation complete\n");
+					printf("| Iteration number: %ld\n", iter);
+					printf("| Writing final state to disk\n");
+					print_coefficients();
+					hasConverged = 1;
+				}
+			} else if (C.abortOnCd) {
+				if (fabs(res[5]) <= C.cdAbortResidual) {
+					printf("\nConverged in CD - Calculation complete\n");
+					printf("| Iteration number: %ld\n", iter);
+					printf("| Writing final state to disk\n");
+					print_coefficients();
+					hasConverged = 1;
+				}
+			} else {
+				printf("\nConverged in '%s' - Calculation complete\n", C.abortName);
+				printf("| Iteration number: %ld\n", iter);
+				printf("| Writing final state to disk\n");
+				print_coefficients();
+				hasConverged = 1;
+			}
+		}
+		if (C.stationary && hasConverged) {
 			data_output(t, iter);
+			if (C.hasExactSolution)
+				calcErrors(t);
 			break;
 		}
 		if ((printTime - t <= 1e-15) || (iter == printIter)) {
 			printf("\nData Output at Iteration %ld\n", iter);
-			if (C.stationary)
-				printf("|            RHO: %20.14e\n", res[0]);
-			else
+			double tNow = wall();
+			printf("| Time since last I/O: %.10g s\n", tNow - tIO);
+			tIO = tNow;
+			if (C.stationary) {
+				if (C.calcWing) {
+					printf("| Residuals: %s: %20.14e\n", C.abortName, res[C.abortVariable]);
+					printf("|            CL : %20.14e\n", res[4]);
+					printf("|            CD : %20.14e\n", res[5]);
+				} else {
+					printf("|            RHO: %20.14e\n", res[0]);
+					printf("|            MX : %20.14e\n", res[1]);
+					printf("|            MY : %20.14e\n", res[2]);
+					printf("|            E  : %20.14e\n", res[3]);
+				}
+			} else {
 				printf("| Time     : %.10g\n", t);
+				double dtp;
+				calc_dt(t, printTime + 1e150, &dtp, &vd);
+				printf("| Time Step: %.10g\n", dtp);
+				if (vd)
+					printf("| Viscous Time Step Dominates!\n");
+			}
+			if (C.hasExactSolution)
+				calcErrors(t);
 			data_output(t, iter);
 			if (printTime - t <= 1e-15)
 				printTime += C.IOtimeInterval;
@@ -555,13 +922,19 @@ void timeDisc(void)
 	if (iter > C.maxIter) {
 		printf("\nERROR - ERROR - ERROR\n");
 		printf("| Maximum Iteration Number Reached. Calculation Aborted!\n");
+		printf("| Final Time %g has not been reached.\n", C.stopTime);
+		printf("| Current Time: %.10g\n", t);
+		printf("| Final State will be written to disk\n");
 		data_output(t, iter - 1);
+		if (C.hasExactSolution)
+			calcErrors(t);
 	}
 	printf("\nComputation Time: %.10g s\n", t1 - t0);
 	if (C.implicit) {
 		printf("| Newton Iterations: %ld\n", C.nNewton);
 		printf("| GMRES Iterations : %ld\n", C.nGMRES);
 	}
+	close_analysis_files();
 }
 
 int ccfd_main(const char *iniFile, int device)

@@ -128,6 +128,10 @@ __global__ void bc_side_recon_kernel(OpArgs<T> a) {
     if (s == k) { gpx = GPx[s]; gpy = GPy[s]; }
 #pragma unroll
   for (int v = 0; v < 4; ++v) in[v] = p[v] + gpx * gx[v] + gpy * gy[v];
+  if (a.bcIn) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.bcIn[v * a.G + g] = double(in[v]);
+  }
   T x[2] = {gpx + T(a.bary[i]), gpy + T(a.bary[Np + i])};
   bc_ghost<T, false>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
               a.fn[(S + k) * Np + i], op_time(a, i), in, out, x);

@@ -57,6 +57,31 @@ void ccfd_mesh_free(ccfd_mesh *m)
 	free(m);
 }
 
+/* initRecordPoints' element search, src/analyze.c:43-76 */
+long ccfd_mesh_locate(const ccfd_mesh *m, const double x[2])
+{
+	if (!m || !x)
+		return -1;
+	for (long e = 0; e < m->nElems; ++e) {
+		const int nv = m->elemType[e];
+		int inside = 1;
+		for (int k = 0; k < nv; ++k) {
+			const double *p = m->xy + 2 * m->elemNode[4 * e + k];
+			const double *q = m->xy + 2 * m->elemNode[4 * e + (k + 1) % nv];
+			double nx = q[1] - p[1], ny = p[0] - q[0];
+			/* the reference's norm: sqrt(nx^2 + nx^2) */
+			double l = sqrt(nx * nx + nx * nx);
+			nx /= l;
+			ny /= l;
+			if (nx * (x[0] - p[0]) + ny * (x[1] - p[1]) > 0.0)
+				inside = 0;
+		}
+		if (inside)
+			return e;
+	}
+	return -1;
+}
+
 /* createElemInfo, src/mesh.c:296-341 */
 static void elem_info(ccfd_mesh *m, long e)
 {
@@ -713,6 +738,317 @@ int ccfd_gmsh_write(const char *path, long nVert, const double *xy,
 	return fclose(f) == 0 ? 0 : -1;
 }
 
+/* find the line after `header` (a full line); returns 0 if absent */
+static int seek_section(FILE *f, const char *header, char *line, int n)
+{
+	rewind(f);
+	while (fgets(line, n, f))
+		if (!strcmp(line, header))
+			return fgets(line, n, f) != NULL;
+	return 0;
+}
+
+/*
+ * gmsh format 4 (readGmsh case 4, src/mesh.c:742-975). As the reference:
+ * boundary codes are the first physical tag of each curve entity that has
+ * one, assigned to the line-element blocks in block order; triangles and
+ * quadrilaterals get domain 1; nodes must be numbered 1..n in file order;
+ * element blocks of other types are not read past (their element lines
+ * are taken as the next block headers, as the reference's loop does).
+ */
+static int gmsh4_read(FILE *f, long *nVertOut, double **xyOut,
+		long *nTriOut, long **triOut, long *nQuadOut, long **quadOut,
+		long *nBCedgeOut, long **bcOut, char *err)
+{
+	char line[4096];
+	long nP = 0, nC = 0;
+	if (!seek_section(f, "$Entities\n", line, sizeof(line)) ||
+			sscanf(line, "%ld %ld", &nP, &nC) != 2 || nC <= 0) {
+		SETERR("no curves in mesh file");
+		return -1;
+	}
+	for (long l = 0; l < nP; ++l)
+		if (!fgets(line, sizeof(line), f)) {
+			SETERR("truncated $Entities section");
+			return -1;
+		}
+	long *pTag = malloc(sizeof(long) * nC), nTag = 0;
+	if (!pTag)
+		return -1;
+	for (long l = 0; l < nC; ++l) {
+		if (!fgets(line, sizeof(line), f)) {
+			SETERR("truncated $Entities section");
+			free(pTag);
+			return -1;
+		}
+		/* tag, bounding box (6), number of physical tags, tags ... */
+		char *tok = strtok(line, " ");
+		for (int k = 0; k < 7 && tok; ++k)
+			tok = strtok(NULL, " ");
+		if (!tok)
+			continue;
+		if (strtol(tok, NULL, 10)) {
+			tok = strtok(NULL, " ");
+			if (tok)
+				pTag[nTag++] = strtol(tok, NULL, 10);
+		}
+	}
+	long nBlocks = 0, nv = 0;
+	if (!seek_section(f, "$Nodes\n", line, sizeof(line)) ||
+			sscanf(line, "%ld %ld", &nBlocks, &nv) != 2 || nv <= 0) {
+		SETERR("no nodes in mesh file");
+		free(pTag);
+		return -1;
+	}
+	double *xy = malloc(sizeof(double) * 2 * nv);
+	long *id = malloc(sizeof(long) * nv);
+	if (!xy || !id) {
+		free(pTag); free(xy); free(id);
+		return -1;
+	}
+	long iv = 0;
+	for (long b = 0; b < nBlocks; ++b) {
+		long nin = 0;
+		if (!fgets(line, sizeof(line), f) ||
+				sscanf(line, "%*d %*d %*d %ld", &nin) != 1 ||
+				nin < 0 || iv + nin > nv) {
+			SETERR("bad $Nodes block %ld", b);
+			goto fail_nodes;
+		}
+		for (long i = 0; i < nin; ++i)
+			if (!fgets(line, sizeof(line), f) ||
+					sscanf(line, "%ld", &id[iv + i]) != 1) {
+				SETERR("bad node tag in block %ld", b);
+				goto fail_nodes;
+			}
+		for (long i = 0; i < nin; ++i, ++iv) {
+			double x, y;
+			if (!fgets(line, sizeof(line), f) ||
+					sscanf(line, "%lg %lg", &x, &y) != 2 ||
+					id[iv] != iv + 1) {
+				SETERR("node %ld does not match its position", iv + 1);
+				goto fail_nodes;
+			}
+			xy[2 * iv] = x;
+			xy[2 * iv + 1] = y;
+		}
+	}
+	if (iv != nv) {
+		SETERR("$Nodes holds %ld of %ld nodes", iv, nv);
+		goto fail_nodes;
+	}
+	free(id);
+	id = NULL;
+	long ne = 0;
+	if (!seek_section(f, "$Elements\n", line, sizeof(line)) ||
+			sscanf(line, "%ld %ld", &nBlocks, &ne) != 2 || ne <= 0) {
+		SETERR("no elements in mesh file");
+		free(pTag); free(xy);
+		return -1;
+	}
+	long *bce = malloc(sizeof(long) * 3 * ne);
+	long *tri = malloc(sizeof(long) * 4 * ne);
+	long *quad = malloc(sizeof(long) * 5 * ne);
+	long nb = 0, nt = 0, nq = 0;
+	if (!bce || !tri || !quad)
+		goto fail_elems;
+	for (long b = 0; b < nBlocks; ++b) {
+		int type = 0;
+		long nin = 0;
+		if (!fgets(line, sizeof(line), f) ||
+				sscanf(line, "%*d %*d %d %ld", &type, &nin) != 2 ||
+				nin < 0 || nb + nin > ne || nt + nin > ne ||
+				nq + nin > ne) {
+			SETERR("bad $Elements block %ld", b);
+			goto fail_elems;
+		}
+		if (type == 1 && b >= nTag) {
+			SETERR("line block %ld has no curve with a physical tag", b);
+			goto fail_elems;
+		}
+		for (long i = 0; i < nin && type >= 1 && type <= 3; ++i) {
+			long n[4];
+			int need = type + 1;
+			if (!fgets(line, sizeof(line), f) ||
+					sscanf(line, "%*d %ld %ld %ld %ld", &n[0], &n[1],
+						&n[2], &n[3]) < need) {
+				SETERR("bad element line in block %ld", b);
+				goto fail_elems;
+			}
+			for (int k = 0; k < need; ++k)
+				if (n[k] < 1 || n[k] > nv) {
+					SETERR("element node %ld out of range", n[k]);
+					goto fail_elems;
+				}
+			if (type == 1) {
+				bce[3 * nb] = n[0] - 1;
+				bce[3 * nb + 1] = n[1] - 1;
+				bce[3 * nb + 2] = pTag[b];
+				nb++;
+			} else if (type == 2) {
+				for (int k = 0; k < 3; ++k)
+					tri[4 * nt + k] = n[k] - 1;
+				tri[4 * nt + 3] = 1;
+				nt++;
+			} else {
+				for (int k = 0; k < 4; ++k)
+					quad[5 * nq + k] = n[k] - 1;
+				quad[5 * nq + 4] = 1;
+				nq++;
+			}
+		}
+	}
+	free(pTag);
+	*nVertOut = nv; *xyOut = xy;
+	*nTriOut = nt; *triOut = tri;
+	*nQuadOut = nq; *quadOut = quad;
+	*nBCedgeOut = nb; *bcOut = bce;
+	return 0;
+fail_elems:
+	free(bce); free(tri); free(quad);
+	free(pTag); free(xy);
+	return -1;
+fail_nodes:
+	free(pTag); free(xy); free(id);
+	return -1;
+}
+
+/* first line containing `word` (strstr, as the reference), stopping at a
+ * line containing "End"; returns 0 if not found */
+static int emc2_seek(FILE *f, const char *word, char *line, int n)
+{
+	rewind(f);
+	while (fgets(line, n, f)) {
+		if (strstr(line, word))
+			return fgets(line, n, f) != NULL;
+		if (strstr(line, "End"))
+			return 0;
+	}
+	return 0;
+}
+
+/* EMC2 / medit .mesh (readEMC2, src/mesh.c:1004-1141): vertices, edges with
+ * a nonzero reference as boundary edges (their reference is the BC code),
+ * triangles and quadrilaterals with their reference as domain */
+int ccfd_emc2_read(const char *path, long *nVertOut, double **xyOut,
+		long *nTriOut, long **triOut, long *nQuadOut, long **quadOut,
+		long *nBCedgeOut, long **bcOut, char *err)
+{
+	FILE *f = fopen(path, "r");
+	if (!f) {
+		SETERR("could not open mesh file '%s'", path);
+		return -1;
+	}
+	char line[1024];
+	long nv = 0, nE = 0, nt = 0, nq = 0, nb = 0;
+	double *xy = NULL;
+	long *bce = NULL, *tri = NULL, *quad = NULL;
+	if (!emc2_seek(f, "Vertices", line, sizeof(line)) ||
+			sscanf(line, "%ld", &nv) != 1 || nv <= 0) {
+		SETERR("no vertices in mesh file");
+		goto fail;
+	}
+	xy = malloc(sizeof(double) * 2 * nv);
+	if (!xy)
+		goto fail;
+	for (long i = 0; i < nv; ++i)
+		if (!fgets(line, sizeof(line), f) ||
+				sscanf(line, "%lg %lg", &xy[2 * i], &xy[2 * i + 1]) != 2) {
+			SETERR("bad vertex %ld", i + 1);
+			goto fail;
+		}
+	if (emc2_seek(f, "Edges", line, sizeof(line)) &&
+			sscanf(line, "%ld", &nE) == 1 && nE > 0) {
+		bce = malloc(sizeof(long) * 3 * nE);
+		if (!bce)
+			goto fail;
+		for (long i = 0; i < nE; ++i) {
+			long a, b, r;
+			if (!fgets(line, sizeof(line), f) ||
+					sscanf(line, "%ld %ld %ld", &a, &b, &r) != 3) {
+				SETERR("bad edge %ld", i + 1);
+				goto fail;
+			}
+			if (r == 0)
+				continue;
+			if (a < 1 || a > nv || b < 1 || b > nv) {
+				SETERR("edge node out of range");
+				goto fail;
+			}
+			bce[3 * nb] = a - 1;
+			bce[3 * nb + 1] = b - 1;
+			bce[3 * nb + 2] = r;
+			nb++;
+		}
+	}
+	if (emc2_seek(f, "Triangles", line, sizeof(line)) &&
+			sscanf(line, "%ld", &nt) == 1 && nt > 0) {
+		tri = malloc(sizeof(long) * 4 * nt);
+		if (!tri)
+			goto fail;
+		for (long i = 0; i < nt; ++i) {
+			long n[4];
+			if (!fgets(line, sizeof(line), f) ||
+					sscanf(line, "%ld %ld %ld %ld", &n[0], &n[1], &n[2],
+						&n[3]) != 4) {
+				SETERR("bad triangle %ld", i + 1);
+				goto fail;
+			}
+			for (int k = 0; k < 3; ++k) {
+				if (n[k] < 1 || n[k] > nv) {
+					SETERR("triangle node out of range");
+					goto fail;
+				}
+				tri[4 * i + k] = n[k] - 1;
+			}
+			tri[4 * i + 3] = n[3];
+		}
+	} else {
+		nt = 0;
+	}
+	if (emc2_seek(f, "Quadrangles", line, sizeof(line)) &&
+			sscanf(line, "%ld", &nq) == 1 && nq > 0) {
+		quad = malloc(sizeof(long) * 5 * nq);
+		if (!quad)
+			goto fail;
+		for (long i = 0; i < nq; ++i) {
+			long n[5];
+			if (!fgets(line, sizeof(line), f) ||
+					sscanf(line, "%ld %ld %ld %ld %ld", &n[0], &n[1],
+						&n[2], &n[3], &n[4]) != 5) {
+				SETERR("bad quadrangle %ld", i + 1);
+				goto fail;
+			}
+			for (int k = 0; k < 4; ++k) {
+				if (n[k] < 1 || n[k] > nv) {
+					SETERR("quadrangle node out of range");
+					goto fail;
+				}
+				quad[5 * i + k] = n[k] - 1;
+			}
+			quad[5 * i + 4] = n[4];
+		}
+	} else {
+		nq = 0;
+	}
+	fclose(f);
+	if (!bce)
+		bce = malloc(sizeof(long) * 3);
+	if (!tri)
+		tri = malloc(sizeof(long) * 4);
+	if (!quad)
+		quad = malloc(sizeof(long) * 5);
+	*nVertOut = nv; *xyOut = xy;
+	*nTriOut = nt; *triOut = tri;
+	*nQuadOut = nq; *quadOut = quad;
+	*nBCedgeOut = nb; *bcOut = bce;
+	return 0;
+fail:
+	fclose(f);
+	free(xy); free(bce); free(tri); free(quad);
+	return -1;
+}
+
 int ccfd_gmsh_read(const char *path, long *nVertOut, double **xyOut,
 		long *nTriOut, long **triOut, long *nQuadOut, long **quadOut,
 		long *nBCedgeOut, long **bcOut, char *err)
@@ -731,8 +1067,14 @@ int ccfd_gmsh_read(const char *path, long *nVertOut, double **xyOut,
 			break;
 		}
 	}
+	if (fmt == 4) {
+		int rc = gmsh4_read(f, nVertOut, xyOut, nTriOut, triOut, nQuadOut,
+				quadOut, nBCedgeOut, bcOut, err);
+		fclose(f);
+		return rc;
+	}
 	if (fmt != 2) {
-		SETERR("only gmsh format 2 is supported (found %d)", fmt);
+		SETERR("gmsh format %d is not supported (2 or 4)", fmt);
 		fclose(f);
 		return -1;
 	}

@@ -64,6 +64,9 @@ struct OpArgs {
   T* __restrict__ uy;
   T* __restrict__ ghostP;       // [4][G] ghost element states (barycentres)
   T* __restrict__ ghostS;       // [4][G] ghost side states
+  double* __restrict__ bcIn;    // [4][G] interior side states at the
+                                // boundary sides (null: not kept)
+  const T* __restrict__ src;    // [4][Np] source term (null: none)
   T* __restrict__ ut;           // [4][Np]
   // time: time = tdev ? tdev[0] + tcoef * tdev[1] : time0
   double time0;
@@ -277,6 +280,10 @@ __global__ void bc_side_kernel(OpArgs<T> a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) in[v] = a.side[(v * S + k) * Np + i];
   }
+  if (a.bcIn) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.bcIn[v * a.G + g] = double(in[v]);
+  }
   T x[2] = {a.gp[k * Np + i] + T(a.bary[i]),
             a.gp[(S + k) * Np + i] + T(a.bary[Np + i])};
   bc_ghost<T, false>(a.eq, op_bc(a, a.ghostBC[g]), a.ex, a.fn[k * Np + i],
@@ -364,8 +371,14 @@ __device__ inline void stage_epilogue(const OpArgs<T>& a, int i,
   const int Np = a.Np;
   const T aq = pre.aq;
   T ut[4];
+  if (a.src) {
+    // src/finiteVolume.c:116-119 with the source term
 #pragma unroll
-  for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
+    for (int v = 0; v < 4; ++v) ut[v] = (a.src[v * Np + i] - acc[v]) * aq;
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ut[v] = (T(0.0) - acc[v]) * aq;
+  }
   if (a.write_ut) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) a.ut[v * Np + i] = ut[v];

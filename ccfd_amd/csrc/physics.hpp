@@ -751,6 +751,30 @@ CCFD_HD void exact_func(const Eq<T>& q, const ExactCfg& ex, int f,
   }
 }
 
+// source term at one point (evalSource, src/source.c:22-60): function 1 is
+// the manufactured sine-wave source; any other number leaves src as it was
+// (the reference's switch has no default). ns: the Navier-Stokes variant's
+// extra viscous term of the energy equation.
+template <typename T>
+CCFD_HD void eval_source(const Eq<T>& q, int f, int ns, const T x[2], T time,
+                         T src[4]) {
+  if (f != 1) return;
+  const T pi = T(3.141592653589793238462643383279502884);
+  const T amp = T(0.1), om = pi * T(1.0), a = T(2.0) * pi;
+  const T ph = om * (x[0] + x[1]) - a * time;
+  const T t1 = cos(ph);
+  const T t2 = sin(T(2.0) * (om * (x[0] + x[1]) - a * time));
+  src[0] = (-a + T(2.0) * om) * t1;
+  src[1] = (-a + om * (q.gam * T(3.0) - T(1.0))) * t1 + amp * om * q.gam1 * t2;
+  src[2] = src[1];
+  src[3] = ((T(2.0) + q.gam * T(6.0)) * om - T(4.0) * a) * t1 +
+           amp * (T(2.0) * om * q.gam - a) * t2;
+  if (ns)
+    src[3] = src[3] + T(2.0) * q.mu * q.gam * om * om / q.Pr * sin(ph);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) src[v] *= amp;
+}
+
 // characteristic ghost state (type 5; src/boundary.c:217-271)
 template <typename T>
 CCFD_HD void bc_ghost_char(const Eq<T>& q, const BCDev& b, T nx, T ny,

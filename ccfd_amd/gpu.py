@@ -112,6 +112,16 @@ def lib():
         "ccfd_gpu_jacobian_blocks": (C.c_int, [vp, C.POINTER(ImplicitCfg),
                                                C.c_double, C.c_double,
                                                C.c_int, dp, dp]),
+        "ccfd_gpu_set_source": (C.c_int, [vp, C.c_int]),
+        "ccfd_gpu_get_bc_sides": (C.c_int, [vp, dp]),
+        "ccfd_gpu_wing_init": (C.c_int, [vp, C.c_void_p, C.c_int, C.c_double,
+                                         C.POINTER(C.c_long),
+                                         C.POINTER(C.c_long), dp]),
+        "ccfd_gpu_wing_coef": (C.c_int, [vp, dp, C.c_double, dp, dp]),
+        "ccfd_gpu_calc_errors": (C.c_int, [vp, C.c_int, C.c_double, dp, dp,
+                                           dp]),
+        "ccfd_gpu_get_prim_at": (C.c_int, [vp, C.c_int, C.POINTER(C.c_long),
+                                           dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -237,6 +247,52 @@ class Solver:
 
     def fvtd(self, time=0.0):
         check(self.L.ccfd_gpu_fvtd(self.h, time))
+
+    # ---- source term and analysis consumers (src/source.c, src/analyze.c)
+    def set_source(self, func):
+        """calcSource with sourceFunction func (0: off)"""
+        check(self.L.ccfd_gpu_set_source(self.h, func))
+
+    def bc_sides(self):
+        """interior side states at the boundary sides (ghost order) from
+        the last operator evaluation"""
+        out = np.empty((max(self.mesh.n_ghosts, 1), 4))
+        check(self.L.ccfd_gpu_get_bc_sides(self.h, N.dptr(out)))
+        return out[:self.mesh.n_ghosts]
+
+    def wing_init(self, wall_code, ref_length=1.0):
+        """initWing: (n_pressure, n_suction, xyphi)"""
+        nP, nS = C.c_long(), C.c_long()
+        xy = np.zeros((max(self.mesh.n_ghosts, 1), 3))
+        check(self.L.ccfd_gpu_wing_init(self.h, C.cast(self.mesh.ptr, C.c_void_p),
+                                        wall_code, ref_length, C.byref(nP),
+                                        C.byref(nS), N.dptr(xy)))
+        return nP.value, nS.value, xy[:nP.value + nS.value]
+
+    def wing_coef(self, ref, alpha, n_sides):
+        """calcCoef: (cl, cd, cp)"""
+        ref = np.ascontiguousarray(ref, dtype=np.float64)
+        coef = np.zeros(2)
+        cp = np.zeros(max(n_sides, 1))
+        check(self.L.ccfd_gpu_wing_coef(self.h, N.dptr(ref), alpha,
+                                        N.dptr(coef), N.dptr(cp)))
+        return coef[0], coef[1], cp[:n_sides]
+
+    def calc_errors(self, func, time):
+        """calcErrors: (L1, L2, Linf)"""
+        L1, L2, Li = np.zeros(4), np.zeros(4), np.zeros(4)
+        check(self.L.ccfd_gpu_calc_errors(self.h, func, time, N.dptr(L1),
+                                          N.dptr(L2), N.dptr(Li)))
+        return L1, L2, Li
+
+    def prim_at(self, elems):
+        """primitive state of the given elements (reference numbering)"""
+        e = np.ascontiguousarray(elems, dtype=np.int64)
+        out = np.zeros((max(len(e), 1), 4))
+        check(self.L.ccfd_gpu_get_prim_at(self.h, len(e),
+                                          e.ctypes.data_as(C.POINTER(C.c_long)),
+                                          N.dptr(out)))
+        return out[:len(e)]
 
     def calc_dt(self, t, ptime, tend):
         dt = C.c_double()

@@ -88,14 +88,23 @@ class Raw:
 
     @staticmethod
     def read_gmsh(path):
-        lib = N.host()
+        """gmsh ASCII, format 2 or 4"""
+        return Raw._read(N.host().ccfd_gmsh_read, path)
+
+    @staticmethod
+    def read_emc2(path):
+        """EMC2 / medit .mesh"""
+        return Raw._read(N.host().ccfd_emc2_read, path)
+
+    @staticmethod
+    def _read(fn, path):
         nv, nt, nq, nb = C.c_long(), C.c_long(), C.c_long(), C.c_long()
         xy, tri, quad, bce = N.c_double_p(), N.c_long_p(), N.c_long_p(), \
             N.c_long_p()
         err = C.create_string_buffer(256)
-        rc = lib.ccfd_gmsh_read(path.encode(), C.byref(nv), C.byref(xy),
-                                C.byref(nt), C.byref(tri), C.byref(nq),
-                                C.byref(quad), C.byref(nb), C.byref(bce), err)
+        rc = fn(path.encode(), C.byref(nv), C.byref(xy), C.byref(nt),
+                C.byref(tri), C.byref(nq), C.byref(quad), C.byref(nb),
+                C.byref(bce), err)
         if rc != 0:
             raise IOError(err.value.decode())
         r = Raw(N.view(xy, 2 * nv.value, np.float64),
@@ -138,6 +147,7 @@ class Mesh:
     # copied out of the C structure on first access
     _ARRAYS = {
         "elem_type": ("elemType", "e", 1, np.int32),
+        "elem_node": ("elemNode", "e", 4, np.int64),
         "bary": ("bary", "e", 2, np.float64),
         "area": ("area", "e", 1, np.float64),
         "areaq": ("areaq", "e", 1, np.float64),

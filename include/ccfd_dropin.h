@@ -15,6 +15,12 @@
  *   vectorDotProduct   src/linearSolver.h:40
  *   GMRES_M            src/linearSolver.h:41-42
  *   freeLinearSolver   src/linearSolver.h:43
+ *   initAnalyze        src/analyze.h:50 (the analysis keys are read by
+ *                      ccfd_init; this only checks the case is set up)
+ *   analyze            src/analyze.h:51
+ *   calcErrors         src/analyze.h:52
+ *   globalResidual     src/analyze.h:53
+ *   freeAnalyze        src/analyze.h:54
  *   main()             src/main.c:35-130 -> ccfd_main
  */
 #ifndef CCFD_DROPIN_H
@@ -73,6 +79,13 @@ double vectorDotProduct(double **A, double **B);
 void GMRES_M(double time, double dt, double alpha, double **B, double normB,
 		double *abortCrit, double **deltaX);
 void freeLinearSolver(void);
+void initAnalyze(void);
+/* record points, CL / CD / CP (calcWing) and the residual file after a step
+ * to `time`; resIter[4], resIter[5] receive the CL / CD residuals */
+void analyze(double time, long iter, double resIter[6]);
+void calcErrors(double time);
+void globalResidual(double resIter[6]);
+void freeAnalyze(void);
 
 #ifdef __cplusplus
 }

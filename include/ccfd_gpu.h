@@ -110,6 +110,66 @@ int ccfd_gpu_get_recon(ccfd_gpu_ctx *ctx, double *u_x, double *u_y,
 /* fvTimeDerivative (src/finiteVolume.c:78-121) at the given time on the
  * resident primitive state; dU/dt stays on the device */
 int ccfd_gpu_fvtd(ccfd_gpu_ctx *ctx, double time);
+/*
+ * Source term (calcSource, src/source.c:66-85; the reference's calcSource /
+ * sourceFunction keys, src/equation.c:119-123): sourceFunc 0 none, 1 the
+ * manufactured sine-wave source. Every later operator evaluation (fvtd,
+ * explicit stages, implicit residuals) adds the quadrature of the source
+ * at its own time. Survives ccfd_gpu_set_mesh.
+ */
+int ccfd_gpu_set_source(ccfd_gpu_ctx *ctx, int sourceFunc);
+
+/* exact function exactFunc (1..6, src/exactFunction.c:35-183) at point x
+ * and time on the host (the C library's transcendental functions, as the
+ * reference): refState / rp1Dinterface come from p, the bounding box and
+ * dxRef (src/mesh.c:1923) from the mesh. Returns CCFD_EINVAL for an unknown
+ * function. */
+int ccfd_exact_func(int exactFunc, const ccfd_gpu_params *p,
+		const ccfd_mesh *mesh, const double x[2], double time,
+		double pVar[4]);
+
+/* ---- analysis (src/analyze.c) ---- */
+/*
+ * Interior side states at the boundary: for every boundary side (mesh
+ * ghost order), the reconstructed primitive state of the interior side as
+ * the last operator evaluation left it (side->pVar of the connection of the
+ * reference's BC side, which calcCoef reads, src/analyze.c:314,344).
+ * 4 doubles per ghost.
+ */
+int ccfd_gpu_get_bc_sides(ccfd_gpu_ctx *ctx, double *pVar);
+/*
+ * initWing (src/analyze.c:97-181, readWing :187-192): the sides of the wall
+ * whose BC code (100 * type + id) is wallCode, pressure side first, each
+ * list ordered as the reference's insertion orders it. mesh is the one given
+ * to ccfd_gpu_set_mesh. nPressure / nSuction (may be NULL) get the list
+ * lengths; xyphi (may be NULL, 3 per side) the side midpoint x, y and
+ * atan2(y, x) that the reference writes to its CP files.
+ */
+int ccfd_gpu_wing_init(ccfd_gpu_ctx *ctx, const ccfd_mesh *mesh, int wallCode,
+		double refLength, long *nPressure, long *nSuction, double *xyphi);
+/*
+ * calcCoef (src/analyze.c:292-387) from the wall states of the last operator
+ * evaluation: ref = freestream (rho, vx, vy, p) (the reference's
+ * refState[0]), alpha in degrees. coef = (CL, CD) rotated by alpha; cp (may
+ * be NULL) one pressure coefficient per listed side. Bit for bit the
+ * reference's sums (list order, one thread).
+ */
+int ccfd_gpu_wing_coef(ccfd_gpu_ctx *ctx, const double ref[4], double alpha,
+		double coef[2], double *cp);
+/*
+ * calcErrors (src/analyze.c:445-514): reconstruct the current state at
+ * `time` and compare it with exact function exactFunc (1..6,
+ * src/exactFunction.c) at the quadrature points. The sums run in a fixed
+ * order that differs from the reference's element order (relative
+ * difference ~1e-15); Linf is exact. Unlike the reference, dU/dt is left
+ * as it was.
+ */
+int ccfd_gpu_calc_errors(ccfd_gpu_ctx *ctx, int exactFunc, double time,
+		double L1[4], double L2[4], double Linf[4]);
+/* record points (evalRecordPoints, src/analyze.c:393-402): primitive state
+ * of n elements given in reference numbering, 4 doubles each */
+int ccfd_gpu_get_prim_at(ccfd_gpu_ctx *ctx, int n, const long *elems,
+		double *pVar);
 
 /* ---- time stepping (src/timeDiscretization.c) ---- */
 /* calcTimeStep (:249-329): global CFL time step with the output / stop

@@ -125,9 +125,18 @@ int ccfd_cart_connectivity(long iMax, long jMax, double x0, double y0,
 		long *nVert, double **xy, long *nQuad, long **quad,
 		long *nBCedge, long **bcEdge);
 
-/* gmsh v2 ASCII reader (readGmsh, src/mesh.c:498-742): lines with physical
- * group > 100 become boundary edges with that code. */
+/* gmsh ASCII reader (readGmsh, src/mesh.c:498-975), formats 2 and 4.
+ * Format 2: lines with physical group > 100 become boundary edges with that
+ * code. Format 4: the line-element blocks take, in block order, the first
+ * physical tag of the curve entities that have one; elements get domain 1. */
 int ccfd_gmsh_read(const char *path, long *nVert, double **xy,
+		long *nTri, long **tri, long *nQuad, long **quad,
+		long *nBCedge, long **bcEdge, char *err);
+
+/* EMC2 / medit .mesh reader (readEMC2, src/mesh.c:1004-1141): edges with a
+ * nonzero reference become boundary edges with that code; element
+ * references become domains. Same outputs as ccfd_gmsh_read. */
+int ccfd_emc2_read(const char *path, long *nVert, double **xy,
 		long *nTri, long **tri, long *nQuad, long **quad,
 		long *nBCedge, long **bcEdge, char *err);
 
@@ -137,6 +146,16 @@ int ccfd_gmsh_write(const char *path, long nVert, const double *xy,
 		long nBCedge, const long *bcEdge);
 
 void ccfd_mesh_free(ccfd_mesh *m);
+
+/*
+ * The element a record point lies in, as initRecordPoints finds it
+ * (src/analyze.c:33-87): the first element, in element order, for which the
+ * point is on the inner side of every edge. The reference normalises each
+ * edge normal with the x component twice, so a horizontal edge (0 / 0)
+ * never rejects a point; this keeps that behaviour. Returns -1 if no
+ * element accepts the point.
+ */
+long ccfd_mesh_locate(const ccfd_mesh *m, const double x[2]);
 
 /* Venkatakrishnan epsilon^2 = (K sqrt(A))^3 per element, initFV
  * (src/finiteVolume.c:47-52). */

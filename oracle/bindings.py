@@ -29,6 +29,7 @@ class OracleCfg(C.Structure):
         ("rp1Dinterface", C.c_double), ("dxRef", C.c_double),
         ("cfl", C.c_double), ("dfl", C.c_double), ("timeStep1D", C.c_int),
         ("nRKstages", C.c_int), ("timeOrder", C.c_int),
+        ("sourceFunc", C.c_int),
     ]
 
 
@@ -40,7 +41,8 @@ class ImplicitCfg(C.Structure):
 
 def make_cfg(gamma=1.4, mu=0.0, Pr=0.72, navierstokes=0, flux=5, order=2,
              limiter=1, venk_k=1.0, left=(1, 0, 0, 1), right=(0.125, 0, 0, 0.1),
-             x0=0.5, dxref=0.0, cfl=0.9, dfl=0.9, ts1d=0, nrk=1, torder=1):
+             x0=0.5, dxref=0.0, cfl=0.9, dfl=0.9, ts1d=0, nrk=1, torder=1,
+             source=0):
     c = OracleCfg()
     c.gam, c.mu, c.Pr = gamma, mu, Pr
     c.navierstokes, c.iFlux, c.spatialOrder = navierstokes, flux, order
@@ -51,6 +53,7 @@ def make_cfg(gamma=1.4, mu=0.0, Pr=0.72, navierstokes=0, flux=5, order=2,
     c.rp1Dinterface, c.dxRef = x0, dxref
     c.cfl, c.dfl, c.timeStep1D = cfl, dfl, ts1d
     c.nRKstages, c.timeOrder = nrk, torder
+    c.sourceFunc = source
     return c
 
 
@@ -93,6 +96,18 @@ def oracle_lib():
                                    C.c_double, C.c_double, dp, dp, ip]
         L.oracle_rk_coeffs.restype = C.c_int
         L.oracle_rk_coeffs.argtypes = [C.c_int, C.c_int, dp]
+        lp = C.POINTER(C.c_long)
+        L.oracle_quadrature.argtypes = [C.c_void_p, C.c_long, dp, dp]
+        L.oracle_quadrature.restype = C.c_int
+        L.oracle_calc_errors.argtypes = [C.c_void_p, C.c_int, C.c_double,
+                                         dp, dp, dp]
+        L.oracle_wing_lists.argtypes = [C.c_void_p, C.c_int, lp, lp, lp]
+        L.oracle_wing_lists.restype = C.c_long
+        L.oracle_wing_coef.argtypes = [C.c_void_p, lp, C.c_long, C.c_long,
+                                       C.c_double, dp, C.c_double, dp, dp,
+                                       dp, dp]
+        L.oracle_locate.argtypes = [C.c_void_p, dp]
+        L.oracle_locate.restype = C.c_long
         L.oracle_exact_func.argtypes = [C.POINTER(OracleCfg), C.c_void_p,
                                         C.c_int, dp, C.c_double, dp]
         _olib = L
@@ -196,11 +211,61 @@ class Oracle:
             raise RuntimeError("oracle_gmres failed (%d)" % rc)
         return dX, n.value
 
+    def calc_errors(self, func, time):
+        """calcErrors: (L1, L2, Linf), 4 values each"""
+        L1, L2, Li = np.zeros(4), np.zeros(4), np.zeros(4)
+        self.L.oracle_calc_errors(self.p, func, time, _d(L1), _d(L2), _d(Li))
+        return L1, L2, Li
+
+    def wing_lists(self, wall_code):
+        """initWing: (slots, n_pressure, n_suction)"""
+        slots = np.zeros(max(self.mesh.n_ghosts, 1), dtype=np.int64)
+        nP, nS = C.c_long(), C.c_long()
+        n = self.L.oracle_wing_lists(self.p, wall_code,
+                                     slots.ctypes.data_as(C.POINTER(C.c_long)),
+                                     C.byref(nP), C.byref(nS))
+        return slots[:n].copy(), nP.value, nS.value
+
+    def wing_coef(self, slots, n_pressure, n_suction, ref_length, ref, alpha):
+        """calcCoef on the side states of the last fvtd: (cl, cd, cp, xyphi)"""
+        slots = np.ascontiguousarray(slots, dtype=np.int64)
+        n = len(slots)
+        cp, xyphi = np.zeros(max(n, 1)), np.zeros((max(n, 1), 3))
+        cl, cd = C.c_double(), C.c_double()
+        ref = np.asarray(ref, dtype=np.float64)
+        self.L.oracle_wing_coef(self.p, slots.ctypes.data_as(C.POINTER(C.c_long)),
+                                n_pressure, n_suction, ref_length, _d(ref),
+                                alpha, C.byref(cl), C.byref(cd), _d(cp),
+                                _d(xyphi))
+        return cl.value, cd.value, cp[:n], xyphi[:n]
+
     def residual(self, ut):
         ut = np.ascontiguousarray(ut, dtype=np.float64)
         r = np.zeros(4)
         self.L.oracle_global_residual(self.p, _d(ut), _d(r))
         return r
+
+
+def exact_func(cfg, mesh, f, x, time):
+    """exact function f at point x and time (src/exactFunction.c)"""
+    out = np.zeros(4)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    oracle_lib().oracle_exact_func(C.byref(cfg), C.cast(mesh.ptr, C.c_void_p),
+                                   f, _d(x), time, _d(out))
+    return out
+
+
+def quadrature(mesh, e):
+    """quadrature points (n, 2) and weights (n) of element e"""
+    x, w = np.zeros((5, 2)), np.zeros(5)
+    n = oracle_lib().oracle_quadrature(C.cast(mesh.ptr, C.c_void_p), e, _d(x), _d(w))
+    return x[:n], w[:n]
+
+
+def locate(mesh, x):
+    """initRecordPoints' element search: element index or -1"""
+    x = np.asarray(x, dtype=np.float64)
+    return oracle_lib().oracle_locate(C.cast(mesh.ptr, C.c_void_p), _d(x))
 
 
 def dot(a, b):

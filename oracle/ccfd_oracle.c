@@ -26,6 +26,7 @@ struct oracle_ctx {
 	double *ghostElemP;            /* 4 / ghost: ghost cell state */
 	double *ghostSideP;            /* 4 / ghost: ghost side state */
 	double *venkEps;
+	double *src;                   /* 4 / elem: source term */
 	double rk[6];
 	/* implicit work arrays */
 	double *XK, *RXK, *Q, *FX0, *FXK, *dX;
@@ -759,7 +760,8 @@ oracle_ctx *oracle_create(const ccfd_mesh *m, const oracle_cfg *cfg)
 	o->ghostElemP = calloc(4 * G + 4, sizeof(double));
 	o->ghostSideP = calloc(4 * G + 4, sizeof(double));
 	o->venkEps = calloc(N + 1, sizeof(double));
-	if (!o->prim || !o->cons || !o->cons0 || !o->ux || !o->uy || !o->ut ||
+	o->src = calloc(4 * N + 4, sizeof(double));
+	if (!o->src || !o->prim || !o->cons || !o->cons0 || !o->ux || !o->uy || !o->ut ||
 			!o->sideP || !o->flux || !o->ghostElemP ||
 			!o->ghostSideP || !o->venkEps) {
 		oracle_destroy(o);
@@ -784,7 +786,7 @@ void oracle_destroy(oracle_ctx *o)
 		return;
 	free(o->prim); free(o->cons); free(o->cons0); free(o->ux); free(o->uy);
 	free(o->ut); free(o->sideP); free(o->flux); free(o->ghostElemP);
-	free(o->ghostSideP); free(o->venkEps);
+	free(o->ghostSideP); free(o->venkEps); free(o->src);
 	free(o->XK); free(o->RXK); free(o->Q); free(o->FX0); free(o->FXK);
 	free(o->dX);
 	free(o);
@@ -1085,6 +1087,80 @@ static void face_fluxes(oracle_ctx *o)
 	}
 }
 
+/* ---------- quadrature points, src/mesh.c:117-165 ---------- */
+
+int oracle_quadrature(const ccfd_mesh *m, long e, double xGP[5][2],
+		double wGP[5])
+{
+	long s0 = m->slotStart[e];
+	if (m->elemType[e] == 3) {
+		for (int k = 0; k < 3; ++k) {
+			xGP[k][0] = m->bary[2 * e] + m->GP[2 * (s0 + k)];
+			xGP[k][1] = m->bary[2 * e + 1] + m->GP[2 * (s0 + k) + 1];
+			wGP[k] = m->area[e] / 3.0;
+		}
+		return 3;
+	}
+	for (int k = 0; k < 4; ++k) {
+		xGP[k][0] = m->bary[2 * e] + m->GP[2 * (s0 + k)];
+		xGP[k][1] = m->bary[2 * e + 1] + m->GP[2 * (s0 + k) + 1];
+		wGP[k] = m->area[e] / 6.0;
+	}
+	const double *a = m->xy + 2 * m->elemNode[4 * e];
+	const double *c = m->xy + 2 * m->elemNode[4 * e + 2];
+	xGP[4][0] = 0.5 * (a[0] + c[0]);
+	xGP[4][1] = 0.5 * (a[1] + c[1]);
+	wGP[4] = m->area[e] / 3.0;
+	return 5;
+}
+
+/* ---------- source term, src/source.c ---------- */
+
+/* evalSource (src/source.c:22-60); function 1 is the manufactured sine
+ * wave, other numbers leave the values as they were */
+static void eval_source(const oracle_ctx *o, int f, const double x[2],
+		double time, double src[4])
+{
+	if (f != 1)
+		return;
+	double gam = o->c.gam, gam1 = o->gam1, pi = o->pi;
+	double freq = 1.0, amp = 0.1, om = pi * freq, a = 2.0 * pi;
+	double t1 = cos(om * (x[0] + x[1]) - a * time);
+	double t2 = sin(2.0 * (om * (x[0] + x[1]) - a * time));
+	src[0] = (-a + 2.0 * om) * t1;
+	src[1] = (-a + om * (gam * 3.0 - 1.0)) * t1 + amp * om * gam1 * t2;
+	src[2] = src[1];
+	if (o->c.navierstokes) {
+		double t3 = sin(om * (x[0] + x[1]) - a * time);
+		src[3] = ((2.0 + gam * 6.0) * om - 4.0 * a) * t1
+			+ amp * (2.0 * om * gam - a) * t2
+			+ 2.0 * o->c.mu * gam * om * om / o->c.Pr * t3;
+	} else {
+		src[3] = ((2.0 + gam * 6.0) * om - 4.0 * a) * t1
+			+ amp * (2.0 * om * gam - a) * t2;
+	}
+	for (int v = 0; v < 4; ++v)
+		src[v] *= amp;
+}
+
+/* calcSource (src/source.c:66-85): quadrature of evalSource per element;
+ * the point values carry over between points as the reference's local
+ * array does */
+static void calc_source(oracle_ctx *o, double time)
+{
+	for (long e = 0; e < o->N; ++e) {
+		double xg[5][2], wg[5], pt[4] = {0.0, 0.0, 0.0, 0.0};
+		double *S = o->src + 4 * e;
+		S[0] = S[1] = S[2] = S[3] = 0.0;
+		int n = oracle_quadrature(o->m, e, xg, wg);
+		for (int k = 0; k < n; ++k) {
+			eval_source(o, o->c.sourceFunc, xg[k], time, pt);
+			for (int v = 0; v < 4; ++v)
+				S[v] += pt[v] * wg[k];
+		}
+	}
+}
+
 void oracle_fvtd(oracle_ctx *o, double time, double *u_t, double *u_x,
 		double *u_y, double *sideP, double *flux)
 {
@@ -1100,14 +1176,18 @@ void oracle_fvtd(oracle_ctx *o, double time, double *u_t, double *u_x,
 		bc_ghost(o, g, time, o->sideP + 4 * s, o->ghostSideP + 4 * g, x);
 	}
 	face_fluxes(o);
-	/* residual gather, src/finiteVolume.c:96-120 (source term = 0) */
+	/* source term, src/finiteVolume.c:91-93 (zero without calcSource,
+	 * src/finiteVolume.c:58-64) */
+	if (o->c.sourceFunc)
+		calc_source(o, time);
+	/* residual gather, src/finiteVolume.c:96-120 */
 	for (long e = 0; e < N; ++e) {
 		double acc[4] = {0.0, 0.0, 0.0, 0.0};
 		for (long s = m->slotStart[e]; s < m->slotStart[e + 1]; ++s)
 			for (int v = 0; v < 4; ++v)
 				acc[v] += o->flux[4 * s + v];
 		for (int v = 0; v < 4; ++v)
-			o->ut[4 * e + v] = (0.0 - acc[v]) * m->areaq[e];
+			o->ut[4 * e + v] = (o->src[4 * e + v] - acc[v]) * m->areaq[e];
 	}
 	if (u_t) memcpy(u_t, o->ut, sizeof(double) * 4 * N);
 	if (u_x) memcpy(u_x, o->ux, sizeof(double) * 4 * N);
@@ -1651,4 +1731,159 @@ int oracle_gmres(oracle_ctx *o, const oracle_implicit_cfg *ic, double t,
 	o->RXK = saved[1];
 	free(xk); free(rxk); free(prim);
 	return rc;
+}
+
+/* ---------- analysis consumers, src/analyze.c ---------- */
+
+void oracle_calc_errors(oracle_ctx *o, int f, double time, double L1[4],
+		double L2[4], double Linf[4])
+{
+	const ccfd_mesh *m = o->m;
+	for (int v = 0; v < 4; ++v)
+		L1[v] = L2[v] = Linf[v] = 0.0;
+	reconstruct(o, time);
+	for (long e = 0; e < o->N; ++e) {
+		double xg[5][2], wg[5];
+		int n = oracle_quadrature(m, e, xg, wg);
+		const double *p = o->prim + 4 * e;
+		const double *gx = o->ux + 4 * e, *gy = o->uy + 4 * e;
+		for (int k = 0; k < n; ++k) {
+			double ex[4];
+			oracle_exact_func(&o->c, m, f, xg[k], time, ex);
+			double dx[2] = {xg[k][0] - m->bary[2 * e],
+				xg[k][1] - m->bary[2 * e + 1]};
+			for (int v = 0; v < 4; ++v) {
+				double pv = p[v] + dx[0] * gx[v] + dx[1] * gy[v];
+				double err = fabs(ex[v] - pv);
+				Linf[v] = fmax(Linf[v], err);
+				L1[v] += err * wg[k];
+				L2[v] += err * err * wg[k];
+			}
+		}
+	}
+	for (int v = 0; v < 4; ++v) {
+		L1[v] *= m->totalArea_q;
+		L2[v] = sqrt(L2[v] * m->totalArea_q);
+	}
+}
+
+/* the x of a side's midpoint as initWing / calcCoef compute it */
+static double side_x(const ccfd_mesh *m, long s, int d)
+{
+	long e = slot_owner_elem(m, s);
+	return m->GP[2 * s + d] + m->bary[2 * e + d];
+}
+
+/* insert slot s into a list (stable in x: after entries of equal x), the
+ * reference's linked-list insertion (src/analyze.c:119-178) */
+static void wing_insert(const ccfd_mesh *m, long *list, long *n, long s)
+{
+	double xi = side_x(m, s, 0);
+	long k = 0;
+	while (k < *n && !(xi < side_x(m, list[k], 0)))
+		++k;
+	for (long j = *n; j > k; --j)
+		list[j] = list[j - 1];
+	list[k] = s;
+	++*n;
+}
+
+long oracle_wing_lists(const oracle_ctx *o, int wallCode, long *slots,
+		long *nPressure, long *nSuction)
+{
+	const ccfd_mesh *m = o->m;
+	long G = m->nGhosts, nP = 0, nS = 0;
+	/* the wing BC: the reference walks its BC list, built by prepending
+	 * (src/boundary.c:44-45), and keeps the last match, i.e. the first in
+	 * parameter-file order */
+	int wing = -1;
+	for (int b = m->nBC - 1; b >= 0; --b)
+		if (m->bc[b].type * 100 + m->bc[b].id == wallCode)
+			wing = b;
+	long *pr = malloc(sizeof(long) * (G + 1));
+	long *su = malloc(sizeof(long) * (G + 1));
+	if (!pr || !su) {
+		free(pr);
+		free(su);
+		*nPressure = *nSuction = 0;
+		return -1;
+	}
+	/* firstBCside is built by prepending (src/mesh.c:1739-1747): walk the
+	 * ghosts from the last to the first */
+	for (long g = G - 1; g >= 0; --g) {
+		if (m->ghostBC[g] != wing)
+			continue;
+		long s = m->ghostSlot[g];
+		/* the ghost side's normal is minus the interior one (src/mesh.c:
+		 * 285-286); n_y <= 0 goes to the suction list */
+		if (-m->n[2 * s + 1] <= 0.0)
+			wing_insert(m, su, &nS, s);
+		else
+			wing_insert(m, pr, &nP, s);
+	}
+	for (long k = 0; k < nP; ++k)
+		slots[k] = pr[k];
+	for (long k = 0; k < nS; ++k)
+		slots[nP + k] = su[k];
+	free(pr);
+	free(su);
+	*nPressure = nP;
+	*nSuction = nS;
+	return nP + nS;
+}
+
+void oracle_wing_coef(const oracle_ctx *o, const long *slots, long nPressure,
+		long nSuction, double refLength, const double ref[4], double alpha,
+		double *cl, double *cd, double *cp, double *xyphi)
+{
+	const ccfd_mesh *m = o->m;
+	double pi = o->pi;
+	double c_l = 0.0, c_d = 0.0;
+	double v = ref[1] / cos(alpha * pi / 180.0);
+	double qInfQ = 1.0 / (ref[0] * 0.5 * v * v);
+	double qInfLq = qInfQ / refLength;
+	double pInf = ref[3];
+	for (long k = 0; k < nPressure + nSuction; ++k) {
+		long s = slots[k];
+		double p0 = o->sideP[4 * s + 3];
+		double nx = m->n[2 * s], ny = m->n[2 * s + 1], len = m->len[s];
+		c_l += ny * p0 * len;
+		c_d += nx * p0 * len;
+		if (cp)
+			cp[k] = (p0 - pInf) * qInfQ;
+		if (xyphi) {
+			double x = side_x(m, s, 0), y = side_x(m, s, 1);
+			xyphi[3 * k] = x;
+			xyphi[3 * k + 1] = y;
+			xyphi[3 * k + 2] = atan2(y, x);
+		}
+	}
+	c_l *= qInfLq;
+	c_d *= qInfLq;
+	double aL = alpha * pi / 180.0;
+	*cl = c_l * cos(aL) - c_d * sin(aL);
+	*cd = c_d * cos(aL) + c_l * sin(aL);
+}
+
+long oracle_locate(const ccfd_mesh *m, const double x[2])
+{
+	for (long e = 0; e < m->nElems; ++e) {
+		int nv = m->elemType[e], inside = 1;
+		for (int i = 0; i < nv; ++i) {
+			const double *a = m->xy + 2 * m->elemNode[4 * e + i];
+			const double *b = m->xy + 2 * m->elemNode[4 * e + (i + 1) % nv];
+			double u[2] = {b[1] - a[1], a[0] - b[0]};
+			/* the reference normalises with u_x twice (src/analyze.c:55):
+			 * a horizontal edge gives 0 / 0 and never rejects */
+			double t = sqrt(u[0] * u[0] + u[0] * u[0]);
+			u[0] /= t;
+			u[1] /= t;
+			double r[2] = {x[0] - a[0], x[1] - a[1]};
+			if (u[0] * r[0] + u[1] * r[1] > 0.0)
+				inside = 0;
+		}
+		if (inside)
+			return e;
+	}
+	return -1;
 }

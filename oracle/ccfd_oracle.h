@@ -25,6 +25,15 @@
  *   globalResidual             area-weighted RMS (test_make_check_oracle.py)
  *   exact Riemann solution     Toro's published Sod solution, isentropic
  *                              fan (test_oracle_golden.py)
+ *   source term (calcSource)   bit for bit against the reference's compiled
+ *                              source.c + fvTimeDerivative (harness) and the
+ *                              integral of a constant (test_oracle_analyze.py)
+ *   calcErrors                 closed forms: a uniform offset, quadrature
+ *                              points and weights rebuilt from the nodes
+ *   calcCoef / initWing        closed form: a linear pressure field on a
+ *                              closed polygon; list order and partition
+ *   initRecordPoints           point-in-polygon by brute force, and the
+ *                              reference's horizontal-edge behaviour
  */
 #ifndef CCFD_ORACLE_H
 #define CCFD_ORACLE_H
@@ -50,6 +59,8 @@ typedef struct oracle_cfg {
 	double cfl, dfl;
 	int timeStep1D;
 	int nRKstages, timeOrder;
+	/* source term (src/source.c, src/equation.c:119-123): 0 none */
+	int sourceFunc;
 } oracle_cfg;
 
 typedef struct oracle_ctx oracle_ctx;
@@ -132,6 +143,40 @@ int oracle_rk_coeffs(int nRKstages, int timeOrder, double coeff[6]);
  * (src/exactRiemann.c:59-214) */
 void oracle_exact_func(const oracle_cfg *cfg, const ccfd_mesh *m, int f,
 		const double x[2], double time, double pVar[4]);
+
+/* ---- analysis consumers (src/analyze.c) ---- */
+
+/* quadrature points and weights of element e (src/mesh.c:117-165): returns
+ * nGP (3 for triangles, 5 for quadrilaterals) */
+int oracle_quadrature(const ccfd_mesh *m, long e, double xGP[5][2],
+		double wGP[5]);
+
+/* calcErrors (src/analyze.c:445-514): spatialReconstruction(time) on the
+ * current state, then the L1, L2 and Linf norms of (exact function f at the
+ * quadrature points) - (reconstructed state there), summed in element and
+ * point order */
+void oracle_calc_errors(oracle_ctx *o, int f, double time, double L1[4],
+		double L2[4], double Linf[4]);
+
+/* initWing (src/analyze.c:97-181): the interior slots of the sides of the
+ * wall whose BC code (100 * type + id) is wallCode, as the pressure-side
+ * list (nPressure entries) followed by the suction-side list, each sorted by
+ * x of its midpoint the way the reference inserts them. slots holds
+ * nGhosts entries. Returns the number listed. */
+long oracle_wing_lists(const oracle_ctx *o, int wallCode, long *slots,
+		long *nPressure, long *nSuction);
+
+/* calcCoef (src/analyze.c:292-387) from the side states of the last
+ * oracle_fvtd: ref = freestream (rho, vx, vy, p), alpha in degrees. cl, cd
+ * get the rotated coefficients; cp (may be NULL) one value per listed side,
+ * xyphi (may be NULL) its x, y and atan2(y, x). */
+void oracle_wing_coef(const oracle_ctx *o, const long *slots, long nPressure,
+		long nSuction, double refLength, const double ref[4], double alpha,
+		double *cl, double *cd, double *cp, double *xyphi);
+
+/* initRecordPoints (src/analyze.c:33-87): the first element (in element
+ * order) the reference's test accepts for point x; -1 if none */
+long oracle_locate(const ccfd_mesh *m, const double x[2]);
 
 #ifdef __cplusplus
 }

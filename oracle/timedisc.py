@@ -10,7 +10,9 @@ TEST INFRASTRUCTURE ONLY (only tests/ use it).
                  / initFV / initTimeDisc (src/main.c:101-121) for Cartesian and
                  gmsh meshes, icType 1 and 2
   run_case       timeDisc (src/timeDiscretization.c:559-777) with CSV output
-                 (csvOutput, src/output.c:56-113)
+                 (csvOutput, src/output.c:56-113) and, on request, the
+                 stationary abort criteria and analyze.c's files (residual
+                 history with CL / CD, CP files, record points)
 
 Pinned by the reference's own make check outputs (tests/test_make_check_oracle.py)
 and, for Ini, by the reference's compiled readInTools on every case file
@@ -96,7 +98,9 @@ def setup_case(ini_path):
     gam = ini.dbl("gamma", "1.4")
     ini.dbl("R", "287")
     c["flux"] = ini.int("fluxFunction")
-    ini.bool("calcSource", "F")
+    c["source"] = 0
+    if ini.bool("calcSource", "F"):
+        c["source"] = ini.int("sourceFunction")
     # initBoundary
     nBC = ini.int("nBC")
     bcs = []
@@ -139,17 +143,15 @@ def setup_case(ini_path):
     left = right = (1.0, 0.0, 0.0, 1.0)
     x0i = 0.5
     prim = np.zeros((mesh.n_elems, 4))
+    ef = 0
+    c["ref0"], c["alpha"] = None, 0.0
     if ic == 2:
         ef = ini.int("exactFunc")
-        if ef != 5:
-            raise NotImplementedError("exact function %d initial state" % ef)
-        x0i = ini.dbl("RP_1D_interface", "0.5")
-        left = tuple(ini.arr("StateLeft", 4))
-        right = tuple(ini.arr("StateRight", 4))
-        x = mesh.bary[:, 0]
-        prim[x <= x0i] = left
-        prim[x > x0i] = right
-        prim[:, 2] = 0.0
+        if ef == 5:
+            x0i = ini.dbl("RP_1D_interface", "0.5")
+            left = tuple(ini.arr("StateLeft", 4))
+            right = tuple(ini.arr("StateRight", 4))
+            c["ref0"] = left
     elif ic == 1:
         nd = ini.int("nDomains", "1")
         if nd != 1:
@@ -161,6 +163,7 @@ def setup_case(ini_path):
         pi = math.acos(-1.0)
         prim[:] = (rho, cc * math.cos(alpha * pi / 180.0),
                    cc * math.sin(alpha * pi / 180.0), p)
+        c["ref0"], c["alpha"] = tuple(prim[0]), alpha
     # initFV
     order = ini.int("spatialOrder", "1")
     lim, vk = 1, 1.0
@@ -175,6 +178,14 @@ def setup_case(ini_path):
     c["timeOrder"] = ini.int("timeOrder", "1")
     c["implicit"] = ini.bool("implicit", "F")
     c["nRK"] = ini.int("nRKstages", "1")
+    c["abortVariable"], c["abortName"] = 0, ""
+    c["clAbort"] = c["cdAbort"] = 0.0
+    if c["stationary"]:
+        c["abortResidual"] = ini.dbl("abortResidual", "1e-6")
+        c["abortVariable"] = ini.int("abortVariable", "1") - 1
+        c["abortName"] = ("RHO", "MX", "MY", "E")[c["abortVariable"]]
+        c["clAbort"] = ini.dbl("cl_abortResidual", "0.0")
+        c["cdAbort"] = ini.dbl("cd_abortResidual", "0.0")
     c["maxIter"] = ini.int("maxIter", "100000")
     c["tEnd"] = ini.dbl("tEnd")
     if c["implicit"]:
@@ -188,7 +199,21 @@ def setup_case(ini_path):
                      venk_k=vk, left=left, right=right, x0=x0i,
                      cfl=c["cfl"], dfl=c["dfl"], ts1d=int(c["ts1d"]),
                      nrk=c["nRK"], torder=c["timeOrder"],
-                     dxref=math.sqrt(1.0 / (mesh.total_area_q * mesh.n_elems)))
+                     dxref=math.sqrt(1.0 / (mesh.total_area_q * mesh.n_elems)),
+                     source=c["source"])
+    if ic == 2:
+        # setInitialCondition (src/initialCondition.c:205-211)
+        for e in range(mesh.n_elems):
+            prim[e] = B.exact_func(cfg, mesh, ef, mesh.bary[e], 0.0)
+    # initAnalyze (src/analyze.c:197-282)
+    c["exactFunc"] = ef
+    c["exactSolution"] = ini.bool("exactSolution", "F")
+    c["calcWing"] = ini.bool("calcWing", "F")
+    if c["calcWing"]:
+        c["refLength"] = ini.dbl("referenceLength", "1.0")
+        c["wallId"] = ini.int("wall_id")
+    c["recordPoints"] = [tuple(ini.arr("pointCoords", 2))
+                         for _ in range(ini.int("nRecordPoints", "0"))]
     c.update(gam=gam, order=order, limiter=lim, venk_k=vk, left=left,
              right=right, x0=x0i)
     return mesh, cfg, prim, c
@@ -205,12 +230,79 @@ def csv_rows(mesh, prim):
     return rows, text
 
 
-def run_case(ini_path, max_iter=None):
+class Analysis:
+    """analyze.c's consumers over the oracle, writing the reference's files
+    (src/analyze.c:200-282, 292-438) into directory d"""
+
+    def __init__(self, orc, mesh, c, d):
+        self.orc, self.mesh, self.c = orc, mesh, c
+        self.base = os.path.join(d, c["fileName"])
+        self.cl = self.cd = 0.0
+        self.res = None
+        if c["calcWing"] or c["stationary"]:
+            self.res = open(self.base + "_analysis.csv", "w")
+            if c["calcWing"]:
+                self.res.write("Iter, Time, Residual(%s), CL, CD\n" % c["abortName"])
+            else:
+                self.res.write("Iter, Time, Residual(RHO), Residual(VX), "
+                               "Residual(VY), Residual(E)\n")
+        if c["calcWing"]:
+            self.slots, self.nP, self.nS = orc.wing_lists(c["wallId"])
+        self.rec = []
+        for k, x in enumerate(c["recordPoints"]):
+            e = B.locate(mesh, np.array(x))
+            if e < 0:
+                raise ValueError("Record Point # %d is not in Domain" % k)
+            f = open("%s_recordPoint_%d.csv" % (self.base, k), "w")
+            f.write("Time, Density, VelocityX, VelocityY, Pressure\n")
+            self.rec.append((e, f))
+
+    def _cp_file(self, suffix, rng, cp, xy):
+        with open("%s_CP_%s.csv" % (self.base, suffix), "w") as f:
+            f.write("x, y, phi, CP_%s\n" % suffix)
+            for k in rng:
+                f.write("%15.7f,%15.7f,%15.7f,%15.7f\n" % (
+                    xy[k, 0], xy[k, 1], xy[k, 2], cp[k]))
+
+    def analyze(self, t, it, dt, res):
+        c = self.c
+        if self.rec:
+            prim = self.orc.prim()
+            for e, f in self.rec:
+                f.write("%20.12f,%20.12f,%20.12f,%20.12f,%20.12f\n" % (
+                    (t + dt,) + tuple(prim[e])))
+        if c["calcWing"]:
+            old = (self.cl, self.cd)
+            self.cl, self.cd, cp, xy = self.orc.wing_coef(
+                self.slots, self.nP, self.nS, c["refLength"], c["ref0"],
+                c["alpha"])
+            self._cp_file("pressureSide", range(self.nP), cp, xy)
+            self._cp_file("suctionSide", range(self.nP, self.nP + self.nS),
+                          cp, xy)
+            res[4] = abs(old[0] - self.cl) / dt
+            res[5] = abs(old[1] - self.cd) / dt
+            self.res.write("%7d, %13.8f, %15.8e, %15.10f, %15.10f\n" % (
+                it, t + dt, res[c["abortVariable"]], self.cl, self.cd))
+        elif c["stationary"]:
+            self.res.write("%7d, %13.8f, %15.8e, %15.8e, %15.8e, %15.8e\n" % (
+                it, t + dt, res[0], res[1], res[2], res[3]))
+
+    def close(self):
+        if self.res:
+            self.res.close()
+        for _, f in self.rec:
+            f.close()
+
+
+def run_case(ini_path, max_iter=None, analysis_dir=None):
     """timeDisc of the oracle; returns (final time, iterations, final prim,
-    mesh, case dict)"""
+    mesh, case dict). With analysis_dir, also the residual-abort criteria
+    of stationary runs (src/timeDiscretization.c:629-688) and analyze.c's
+    files written there."""
     mesh, cfg, prim, c = setup_case(ini_path)
     orc = B.Oracle(mesh, cfg)
     orc.set_prim(prim)
+    an = Analysis(orc, mesh, c, analysis_dir) if analysis_dir else None
     t = 0.0
     printTime = (math.floor(t / c["IOtimeInterval"]) + 1) * c["IOtimeInterval"]
     printIter = (0 // c["IOiterInterval"] + 1) * c["IOiterInterval"]
@@ -226,19 +318,34 @@ def run_case(ini_path, max_iter=None):
         if nonfinite:
             raise FloatingPointError("convective time step NaN")
         if ic is None:
-            orc.explicit_step(t, dt)
+            r4 = orc.explicit_step(t, dt)
         else:
-            rc, _, _, _ = orc.implicit_step(ic, t, dt)
+            rc, r4, _, _ = orc.implicit_step(ic, t, dt)
             if rc:
                 raise RuntimeError("implicit step failed: %d" % rc)
         t += dt
+        res = list(r4) + [0.0, 0.0]
+        if an:
+            an.analyze(t, it, dt, res)
         if tEnd - t <= 1e-15:
             break
+        if an and c["stationary"] and \
+                abs(res[c["abortVariable"]]) <= c["abortResidual"]:
+            if c["clAbort"] > 0.0:
+                done = abs(res[4]) <= c["clAbort"]
+            elif c["cdAbort"] > 0.0:
+                done = abs(res[5]) <= c["cdAbort"]
+            else:
+                done = True
+            if done:
+                break
         if (printTime - t <= 1e-15) or (it == printIter):
             if printTime - t <= 1e-15:
                 printTime += c["IOtimeInterval"]
             if it == printIter:
                 printIter += c["IOiterInterval"]
+    if an:
+        an.close()
     return t, it, orc.prim(), mesh, c
 
 

@@ -18,7 +18,10 @@ static void mesh_file(const char *path)
 	long nv, nt, nq, nb;
 	double *xy = NULL;
 	long *tri = NULL, *quad = NULL, *bce = NULL;
-	if (ccfd_gmsh_read(path, &nv, &xy, &nt, &tri, &nq, &quad, &nb, &bce, err)) {
+	size_t L = strlen(path);
+	int emc2 = L > 5 && !strcmp(path + L - 5, ".mesh");
+	if ((emc2 ? ccfd_emc2_read : ccfd_gmsh_read)(path, &nv, &xy, &nt, &tri,
+			&nq, &quad, &nb, &bce, err)) {
 		printf("read %s: %s\n", path, err);
 		return;
 	}

@@ -23,7 +23,10 @@ static void mesh_file(const char* path) {
   long nv, nt, nq, nb;
   double* xy = nullptr;
   long *tri = nullptr, *quad = nullptr, *bce = nullptr;
-  if (ccfd_gmsh_read(path, &nv, &xy, &nt, &tri, &nq, &quad, &nb, &bce, err)) {
+  const size_t L = std::strlen(path);
+  const bool emc2 = L > 5 && !std::strcmp(path + L - 5, ".mesh");
+  if ((emc2 ? ccfd_emc2_read : ccfd_gmsh_read)(path, &nv, &xy, &nt, &tri, &nq,
+                                               &quad, &nb, &bce, err)) {
     std::printf("read %s: %s\n", path, err);
     return;
   }

@@ -114,18 +114,28 @@ def test_command_line_errors(tmp_path):
 
 
 def test_unsupported_settings_are_refused(tmp_path, monkeypatch):
-    """settings outside this path end set-up with a message, not a wrong
-    run: source terms, and a gmsh file of another format"""
+    """settings outside this path, or incomplete, end set-up with a
+    message, not a wrong run: a source term without its function (the
+    reference's mandatory sourceFunction, src/equation.c:121), an
+    exact-function initial condition the reference does not know, a gmsh
+    file of another format, and a CGNS mesh"""
     L = _lib()
     monkeypatch.chdir(tmp_path)
     base = open(FILES[0]).read()
     (tmp_path / "src.ini").write_text(base + "\ncalcSource = T\n")
     assert _init(L, str(tmp_path / "src.ini")) == \
-        "source terms are not supported"
-    (tmp_path / "m4.msh").write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n")
+        "missing ini key sourceFunction"
+    lines = [x for x in base.splitlines() if _key(x) != "exactfunc"]
+    (tmp_path / "ef.ini").write_text("\n".join(lines + ["exactFunc = 7"]) + "\n")
+    assert _init(L, str(tmp_path / "ef.ini")) == "unknown exact function 7"
+    (tmp_path / "m3.msh").write_text("$MeshFormat\n3.0 0 8\n$EndMeshFormat\n")
     lines = [x for x in base.splitlines()
              if _key(x) not in ("meshtype", "meshformat", "meshfile")]
-    (tmp_path / "m4.ini").write_text("\n".join(
-        lines + ["meshType = 2", "meshFormat = .msh", "meshFile = m4"]) + "\n")
-    msg = _init(L, str(tmp_path / "m4.ini"))
-    assert "format 2" in msg, msg
+    (tmp_path / "m3.ini").write_text("\n".join(
+        lines + ["meshType = 2", "meshFormat = .msh", "meshFile = m3"]) + "\n")
+    msg = _init(L, str(tmp_path / "m3.ini"))
+    assert "format 3 is not supported" in msg, msg
+    (tmp_path / "cg.ini").write_text("\n".join(
+        lines + ["meshType = 2", "meshFormat = .cgns", "meshFile = m3"]) + "\n")
+    msg = _init(L, str(tmp_path / "cg.ini"))
+    assert "not supported" in msg, msg

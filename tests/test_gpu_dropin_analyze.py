@@ -1,0 +1,135 @@
+"""The drop-in's analysis consumers on the GPU: `ccfd case.ini` runs with
+calcWing, record points, a source term and exact-solution error norms, and
+its files are compared with the oracle's restatement of the same loop
+(oracle/timedisc.py: analyze.c's file formats over the C oracle).
+
+* NACA O-grid, stationary, calcWing with CL abort keys and two record
+  points: the residual history (CL and CD columns bit for bit as printed,
+  the density residual to 1e-10), both CP files and both record-point files
+  as text;
+* manufactured sine wave (exact function 3, sourceFunction 1, exact-function
+  boundaries), transient, exactSolution: the final CSV to 1e-8 and the
+  printed error norms to their 6 printed digits (the device's sine and the
+  C library's may differ by an ulp, so not bit for bit).
+"""
+import glob
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from ccfd_amd import mesh as M
+from oracle import timedisc as TD
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CCFD = os.path.join(ROOT, "ccfd_amd", "lib", "ccfd")
+
+
+def _write(path, items):
+    with open(path, "w") as f:
+        for k, v in items:
+            f.write("%s = %s\n" % (k, v))
+
+
+def _run(d, ini):
+    r = subprocess.run([CCFD, ini], cwd=d, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:]
+    return r.stdout
+
+
+def test_wing_record_points_and_residual_file():
+    with tempfile.TemporaryDirectory() as d:
+        raw = M.naca_ogrid(ni=64, nj=16, wall_spacing=4e-3, radius=6.0,
+                           wall_bc=101, far_bc=501)
+        # counter-clockwise cells, as gmsh writes them: the reference's
+        # record-point search accepts a point only inside such cells
+        raw.quad[:, :4] = raw.quad[:, [0, 3, 2, 1]]
+        raw.write_gmsh(os.path.join(d, "wing.msh"))
+        ini = os.path.join(d, "wing.ini")
+        _write(ini, [
+            ("meshType", 0), ("meshFormat", ".msh"), ("meshFile", "wing"),
+            ("gamma", 1.4), ("fluxFunction", 3), ("CFL", 0.9),
+            ("timeOrder", 2), ("nRKstages", 3), ("implicit", "F"),
+            ("spatialOrder", 2), ("limiter", 2), ("venk_k", 5.0),
+            ("stationary", "T"), ("abortResidual", "1e-12"),
+            ("icType", 1), ("nDomains", 1), ("domainID", 1), ("rho", 1.0),
+            ("mach", 0.5), ("alpha", 3.0), ("pressure", 1.0),
+            ("nBC", 2), ("BCtype", 101), ("BCtype", 501), ("rho", 1.0),
+            ("mach", 0.5), ("alpha", 3.0), ("pressure", 1.0),
+            ("fileName", "wing"), ("IOtimeInterval", "1e9"),
+            ("IOiterInterval", 10), ("outputFormat", 3), ("tEnd", "1e9"),
+            ("maxIter", 30), ("calcWing", "T"), ("referenceLength", 0.8),
+            ("wall_id", 101), ("cl_abortResidual", "1e-14"),
+            ("nRecordPoints", 2), ("pointCoords", "(/0.5, 0.3/)"),
+            ("pointCoords", "(/-1.2, -0.7/)")])
+        out = _run(d, ini)
+        got = {os.path.basename(p): open(p).read()
+               for p in glob.glob(os.path.join(d, "wing_*"))
+               if not p.endswith(".msh") and "_0000" not in p}
+        od = os.path.join(d, "oracle")
+        os.makedirs(od)
+        TD.run_case(ini, analysis_dir=od)
+        want = {os.path.basename(p): open(p).read()
+                for p in glob.glob(os.path.join(od, "wing_*"))}
+    print(out[-600:])
+    for name in ("wing_CP_pressureSide.csv", "wing_CP_suctionSide.csv",
+                 "wing_recordPoint_0.csv", "wing_recordPoint_1.csv"):
+        assert got[name] == want[name], name
+    g = [l.split(",") for l in got["wing_analysis.csv"].splitlines()]
+    w = [l.split(",") for l in want["wing_analysis.csv"].splitlines()]
+    assert g[0] == w[0] and len(g) == len(w) == 31
+    for a, b in zip(g[1:], w[1:]):
+        assert a[0] == b[0] and a[1] == b[1]          # iteration, time
+        assert abs(float(a[2]) / float(b[2]) - 1) < 1e-10   # residual
+        assert a[3] == b[3] and a[4] == b[4]          # CL, CD
+    assert "Maximum Iteration Number Reached" in out
+
+
+def test_source_term_and_error_norms():
+    with tempfile.TemporaryDirectory() as d:
+        ini = os.path.join(d, "sine.ini")
+        _write(ini, [
+            ("meshType", 1), ("nElemsX", 20), ("nElemsY", 12),
+            ("X0", "(/0.0, 0.0/)"), ("Xmax", "(/2.0, 1.2/)"),
+            ("nBCsegments", "(/1, 1, 1, 1/)"),
+            ("meshBCtype", 601), ("meshBCtype", 601), ("meshBCtype", 601),
+            ("meshBCtype", 601),
+            ("gamma", 1.4), ("fluxFunction", 5), ("calcSource", "T"),
+            ("sourceFunction", 1), ("CFL", 0.5), ("timeOrder", 2),
+            ("nRKstages", 3), ("spatialOrder", 2), ("limiter", 2),
+            ("venk_k", 10.0), ("stationary", "F"),
+            ("icType", 2), ("exactFunc", 3), ("exactSolution", "T"),
+            ("nBC", 1), ("BCtype", 601), ("BCexactFunc", 3),
+            ("fileName", "sine"), ("IOtimeInterval", 0.05),
+            ("IOiterInterval", 100000), ("outputFormat", 3),
+            ("tEnd", 0.1), ("maxIter", 100000)])
+        out = _run(d, ini)
+        final = sorted(glob.glob(os.path.join(d, "sine_*.csv")))[-1]
+        got = np.genfromtxt(final, skip_header=1, delimiter=",")
+        t, it, prim, mesh, c = TD.run_case(ini)
+    _, otext = TD.csv_rows(mesh, prim)
+    want = np.genfromtxt(otext.splitlines()[1:], delimiter=",")
+    print(out[-900:])
+    assert abs(t - 0.1) < 1e-12
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-8)
+    # the last error report against the oracle's norms of its final state
+    import oracle.bindings as B
+    orc = B.Oracle(mesh, B.make_cfg(
+        gamma=c["gam"], flux=c["flux"], order=c["order"], limiter=c["limiter"],
+        venk_k=c["venk_k"], source=c["source"]))
+    orc.set_prim(prim)
+    L1, L2, Li = orc.calc_errors(3, t)
+    blocks = out.split("Error Analysis at t = ")
+    assert len(blocks) >= 3          # the output at t = 0.05 and the end
+    last = blocks[-1]
+    for name, ref in (("L1", L1), ("L2", L2), ("Linf", Li)):
+        m = re.search(r"\| %s\s*:\s*(\S+)\s+(\S+)\s+(\S+)\s+(\S+)" % name, last)
+        vals = np.array([float(x) for x in m.groups()])
+        np.testing.assert_allclose(vals, ref, rtol=1e-5)

@@ -134,7 +134,8 @@ def test_undefined_bc_is_rejected():
 REF_CALC = "/root/reference/calc"
 REF_MESHES = sorted(
     os.path.relpath(os.path.join(dp, f), REF_CALC)
-    for dp, _, fs in os.walk(REF_CALC) for f in fs if f.endswith(".msh"))
+    for dp, _, fs in os.walk(REF_CALC) for f in fs
+    if f.endswith(".msh") or f.endswith(".mesh"))
 
 
 def _bc_for(code):
@@ -154,18 +155,16 @@ def _shoelace(xy, cells):
 
 @pytest.mark.parametrize("name", REF_MESHES or ["none"])
 def test_reference_calc_meshes(name):
-    """the gmsh meshes the reference ships with its cases (calc/), read and
-    preprocessed: closed cells with outward normals, the least-squares
-    identity, partner sides facing back, every boundary edge one ghost side,
-    and the total area of the cells computed independently"""
+    """the meshes the reference ships with its cases (calc/: gmsh 2, gmsh 4
+    and EMC2 files), read and preprocessed: closed cells with outward
+    normals, the least-squares identity, partner sides facing back, every
+    boundary edge one ghost side, and the total area of the cells computed
+    independently"""
     if not REF_MESHES:
         pytest.skip("reference cases not present")
     path = os.path.join(REF_CALC, name)
-    if open(path).read(64).split()[1].startswith("4"):
-        with pytest.raises(OSError, match="format 2"):
-            M.Raw.read_gmsh(path)
-        return
-    raw = M.Raw.read_gmsh(path)
+    raw = M.Raw.read_emc2(path) if name.endswith(".mesh") else \
+        M.Raw.read_gmsh(path)
     codes = sorted(set(raw.bcedge[:, 2].tolist()))
     m = M.Mesh(raw, [_bc_for(c) for c in codes])
     assert m.n_elems == len(raw.tri) + len(raw.quad)

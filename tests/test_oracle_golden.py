@@ -97,7 +97,7 @@ def _calc_meshes():
     for dp, _, fs in os.walk(REF_CALC):
         for f in fs:
             p = os.path.join(dp, f)
-            if f.endswith(".msh") and open(p).read(64).split()[1][0] == "2":
+            if f.endswith(".msh") or f.endswith(".mesh"):
                 out.append(os.path.relpath(p, REF_CALC))
     return sorted(out)
 
@@ -107,15 +107,18 @@ CALC_MESHES = _calc_meshes() if os.path.isdir(REF_CALC) else []
 
 @pytest.mark.parametrize("name", CALC_MESHES or ["none"])
 def test_oracle_matches_reference_on_its_case_meshes(name):
-    """live, on the gmsh meshes of the reference's own cases (curved walls,
-    stretched quads, unstructured triangles) with their boundary codes:
+    """live, on the meshes of the reference's own cases (gmsh 2 and 4 and
+    EMC2 files: curved walls, stretched quads, unstructured triangles) with
+    their boundary codes:
     every fvTimeDerivative output bit for bit, HLLC, 2nd order VK, Euler or
     (where a mesh has no-slip walls) Navier-Stokes"""
     from ccfd_amd import mesh as M
     from tests import cases
     if not CALC_MESHES or not B.ref_available(True):
         pytest.skip("reference cases or harness not present")
-    raw = M.Raw.read_gmsh(os.path.join(REF_CALC, name))
+    path = os.path.join(REF_CALC, name)
+    raw = M.Raw.read_emc2(path) if name.endswith(".mesh") else \
+        M.Raw.read_gmsh(path)
     codes = sorted(set(raw.bcedge[:, 2].tolist()))
     ns = any(c // 100 == M.WALL for c in codes)
     bcs, items = [], [("nBC", len(codes))]
