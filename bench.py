@@ -311,7 +311,8 @@ def _err_summary(errs, both=None):
     return out
 
 
-def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
+def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
+               venk_k=5.0):
     """Served steady solves: Poisson arrivals, continuous batching in an
     FP32-flux arena with local time steps, retire on a 1e-6 density
     residual drop (or the step cap); then the same requests one at a time in
@@ -330,15 +331,15 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True):
                 for sp, a in zip(specs, arrivals)]
 
     def scheme(prec):
-        return G.params(flux=9, order=2, limiter=2, venk_k=5.0, nrk=4,
+        return G.params(flux=9, order=2, limiter=2, venk_k=venk_k, nrk=4,
                         torder=2, precision=prec)
     out = {"requests": n_req, "cells_range": list(cells),
            "arrival_rate_per_s": rate, "max_steps": max_steps,
            "target_drop": 1e-6,
-           "scheme": "AUSMD, VK K=5, RK4 local time steps at CFL 2; served: "
+           "scheme": "AUSMD, VK K=%g, RK4 local time steps at CFL 2; served: "
                      "FP32 flux / FP64 state until a 1e-4 drop or a stall, "
                      "then FP64 to the target; baseline: FP64 one solve at a "
-                     "time"}
+                     "time" % venk_k}
     b = BT.Batch(scheme(G.FP32_FLUX), max_sides=4, local_dt=True,
                  hbm_bytes=(3 * hbm_gb << 30) // 4)
     pol = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
@@ -587,6 +588,8 @@ def main():
                     help="timed steps of the other fluxes / FP32 mode")
     ap.add_argument("--served-requests", type=int, default=12)
     ap.add_argument("--served-rate", type=float, default=4.0)
+    ap.add_argument("--served-venk-k", type=float, default=5.0,
+                    help="Venkatakrishnan constant of the served solves")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--min-cells", type=float, default=None)
@@ -656,7 +659,7 @@ def main():
     if args.workload == "served":
         cells = (args.min_cells or 1e4, args.max_cells or 2e6)
         r = served_run(args.requests, cells, args.rate, args.max_steps,
-                       args.hbm_gb)
+                       args.hbm_gb, venk_k=args.served_venk_k)
         out = {"metric": "served steady solves/s (p50/p99 latency)",
                "value": r["served"]["solves_per_s"], "unit": "solves/s",
                "n_gpus": 1, "steps": None, "warmup": 0,
@@ -860,7 +863,8 @@ def main():
             out["served_sample"] = served_run(
                 args.served_requests, (args.min_cells or 1e4,
                                        args.max_cells or 2e6),
-                args.served_rate, min(args.max_steps, 8000), 32)
+                args.served_rate, min(args.max_steps, 8000), 32,
+                venk_k=args.served_venk_k)
             out["served_sample"]["note"] = (
                 "%d requests (the full stream has 2000) over the full 1e4 - "
                 "2e6 cell range" % args.served_requests)

@@ -22,7 +22,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 
 HOST_SRC = ["mesh.c", "ini.c", "driver.c"]
 GPU_SRC = ["ccfd_gpu.hip", "implicit.hip", "batch.hip", "analyze.hip"]
-HOST_CXX_SRC = ["soa.cpp", "agglomerate.cpp", "exact_host.cpp"]   # host C++ in libccfd_gpu.so
+HOST_CXX_SRC = ["soa.cpp", "agglomerate.cpp", "exact_host.cpp", "serve.cpp"]   # host C++ in libccfd_gpu.so
 FLUX_CODES = list(range(1, 12))
 OBJ = os.path.join(ROOT, "_build")
 

@@ -1,24 +1,31 @@
-"""Served steady solves: a request loop with continuous batching over
-`ccfd_amd.batch.Batch` arenas.
+"""Served steady solves: requests arriving over time, solved with
+continuous batching over `ccfd_amd.batch.Batch` arenas.
 
-Requests arrive over time (the caller gives arrival offsets, e.g. a Poisson
-stream). Each request names how to build its mesh and initial state; a
-thread pool prepares them as they arrive (mesh generation and
-preprocessing run in C / numpy and release the GIL). The loop admits
-prepared requests in arrival order whenever the arena has room (HBM
-budget), steps every admitted solve together, and retires a solve when its
-density residual has dropped by the target factor (relative to its first
-step), when it reaches the step cap, or when its time step turns
-non-finite. Work of requests that arrive while others run overlaps on the
-device: admission happens between step bursts, not after the batch
-drains.
+The serving loop itself (FIFO admission within the arenas' room, step
+bursts, residual polling, retirement, precision promotion) is the C ABI of
+include/ccfd_serve.h (ccfd_amd/csrc/serve.cpp); this module feeds it.
+Requests arrive at given offsets (e.g. a Poisson stream); a thread pool
+prepares each one's mesh and initial state as it arrives (mesh generation
+and preprocessing run in C / numpy and release the GIL); prepared requests
+are submitted in arrival order, and each round of the server admits what
+fits, steps every admitted solve together and retires finished ones. Work
+of requests that arrive while others run overlaps on the device: admission
+happens between step bursts, not after the batch drains.
+
+A solve retires when its density residual has dropped by the target factor
+(relative to its first step), when it reaches the step cap, or when its
+time step turns non-finite.
 
 Precision promotion (optional `polish` arena): FP32 flux arithmetic floors
 the residual near 1e-5 of its initial value, above a 1e-6 target. A solve
 therefore runs in the main (FP32-flux) arena until its drop reaches
 `switch_drop` (or stops improving within 10x of it), and then continues
-from its FP64 conservative state in the polish (FP64) arena to the target. Both arenas
-step concurrently on their own streams.
+from its conservative state (consPrim) in the polish (FP64) arena to the
+target.
+
+Arenas are either `Batch` objects (the server drives them from C) or any
+object with the same methods (fits, add, remove, step, poll, cons, prim),
+reached through callbacks: the CPU tests use stand-ins.
 
 Reported per request: arrival, admission and completion times, steps (and
 how many ran in the polish arena), residual drop, and whether it converged.
@@ -26,9 +33,72 @@ how many ran in the polish arena), residual drop, and whether it converged.
 """
 import collections
 import concurrent.futures as cf
+import ctypes as C
 import time
 
 import numpy as np
+
+from . import _native as N
+from . import batch as BT
+
+CCFD_ENOMEM = -2
+
+_CLOCK = C.CFUNCTYPE(C.c_double, C.c_void_p)
+_FITS = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int))
+_ADD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
+                   C.c_double, C.POINTER(C.c_int))
+_REMOVE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int)
+_STEP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int)
+_POLL = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                    C.POINTER(BT.BatchInfo), C.POINTER(C.c_int))
+_GET = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double))
+
+
+class Backend(C.Structure):
+    """struct ccfd_serve_backend"""
+    _fields_ = [("arena", C.c_void_p), ("fits", _FITS), ("add", _ADD),
+                ("remove", _REMOVE), ("step", _STEP), ("poll", _POLL),
+                ("get_cons", _GET), ("get_prim", _GET)]
+
+
+class ServeCfg(C.Structure):
+    """struct ccfd_serve_cfg"""
+    _fields_ = [("targetDrop", C.c_double), ("maxSteps", C.c_long),
+                ("stepsPerRound", C.c_int), ("maxActive", C.c_int),
+                ("switchDrop", C.c_double), ("stallRounds", C.c_int),
+                ("gamma", C.c_double), ("keepState", C.c_int),
+                ("clock", _CLOCK), ("clockArg", C.c_void_p)]
+
+
+class ServeResult(C.Structure):
+    """struct ccfd_serve_result"""
+    _fields_ = [("id", C.c_long), ("nElems", C.c_long), ("steps", C.c_long),
+                ("polishSteps", C.c_long), ("drop", C.c_double),
+                ("converged", C.c_int), ("nonfinite", C.c_int),
+                ("admitted", C.c_double), ("done", C.c_double)]
+
+
+def _lib():
+    L = BT.lib()
+    if getattr(L, "_serve_typed", False):
+        return L
+    vp = C.c_void_p
+    L.ccfd_serve_backend_batch.argtypes = [vp, C.POINTER(Backend)]
+    L.ccfd_serve_create.argtypes = [C.POINTER(vp), C.POINTER(Backend),
+                                    C.POINTER(Backend), C.POINTER(ServeCfg)]
+    L.ccfd_serve_destroy.argtypes = [vp]
+    L.ccfd_serve_submit.argtypes = [vp, C.c_long, vp, C.c_long,
+                                    C.POINTER(C.c_double), C.c_double]
+    L.ccfd_serve_round.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]
+    L.ccfd_serve_results.argtypes = [vp, C.c_int, C.POINTER(ServeResult),
+                                     C.POINTER(C.c_int)]
+    L.ccfd_serve_take_state.argtypes = [vp, C.c_long, C.POINTER(C.c_double)]
+    for f in ("backend_batch", "create", "destroy", "submit", "round",
+              "results", "take_state"):
+        getattr(L, "ccfd_serve_" + f).restype = C.c_int
+    L._serve_typed = True
+    return L
 
 
 class Request:
@@ -45,13 +115,73 @@ class Request:
 def cons_to_prim(cons, gamma=1.4):
     """consPrim (src/equationOfState.c:37-52) on host arrays"""
     p = np.empty_like(cons)
-    rho = np.where(cons[:, 0] < 1e-19, 1e-10, cons[:, 0])
-    p[:, 0] = rho
-    p[:, 1] = cons[:, 1] / rho
-    p[:, 2] = cons[:, 2] / rho
-    pr = (gamma - 1.0) * (cons[:, 3] - 0.5 * rho * (p[:, 1] ** 2 + p[:, 2] ** 2))
-    p[:, 3] = np.where(pr < 1e-19, 1e-10, pr)
+    p[:, 0] = cons[:, 0]
+    p[:, 1] = cons[:, 1] / cons[:, 0]
+    p[:, 2] = cons[:, 2] / cons[:, 0]
+    p[:, 3] = (gamma - 1.0) * (cons[:, 3] - 0.5 * (cons[:, 1] * p[:, 1] +
+                                                    cons[:, 2] * p[:, 2]))
+    p[:, 0] = np.where(p[:, 0] < 1e-19, 1e-10, p[:, 0])
+    p[:, 3] = np.where(p[:, 3] < 1e-19, 1e-10, p[:, 3])
     return p
+
+
+class _PyArena:
+    """callbacks of a Python arena object (a stand-in in tests); meshes are
+    referred to by integer handles the server passes back"""
+
+    def __init__(self, obj, objs):
+        self.obj, self.objs = obj, objs
+
+        def fits(_, h, f):
+            f[0] = int(bool(obj.fits(objs[h][0])))
+            return 0
+
+        def add(_, h, prim, cfl, k):
+            # the submitted state object, or (on a hand-over to the polish
+            # arena) the state the server converted
+            mesh, state = objs[h]
+            if prim:
+                state = np.ctypeslib.as_array(
+                    prim, shape=(mesh.n_elems, 4)).copy()
+            k[0] = int(obj.add(mesh, state, cfl))
+            return 0
+
+        def remove(_, k):
+            obj.remove(k)
+            return 0
+
+        def step(_, n):
+            obj.step(n)
+            return 0
+
+        def poll(_, cap, ids, info, n):
+            j = 0
+            for k, inf in obj.poll().items():
+                if j >= cap:
+                    break
+                ids[j] = k
+                i = info[j]
+                i.nElems = int(inf.get("n", 0))
+                i.steps = int(inf["steps"])
+                i.res[:] = [float(x) for x in inf["res"][:4]]
+                i.res0[:] = [float(x) for x in inf["res0"][:4]]
+                i.nonfinite = int(bool(inf["nonfinite"]))
+                j += 1
+            n[0] = j
+            return 0
+
+        def getter(name):
+            def get(_, k, out):
+                a = getattr(obj, name)(k)
+                if a is not None:
+                    a = np.asarray(a, dtype=np.float64).ravel()
+                    C.memmove(out, a.ctypes.data, 8 * len(a))
+                return 0
+            return get
+
+        self.fns = (_FITS(fits), _ADD(add), _REMOVE(remove), _STEP(step),
+                    _POLL(poll), _GET(getter("cons")), _GET(getter("prim")))
+        self.be = Backend(None, *self.fns)
 
 
 class Server:
@@ -62,150 +192,137 @@ class Server:
                  sleep=time.sleep, log=None, log_every=10.0):
         self.batch = batch
         self.polish = polish           # FP64 arena for the final drop
-        self.switch_drop = switch_drop
-        self.stall_polls = stall_polls
-        self.gamma = gamma
-        self.max_active = max_active   # 1: one solve at a time
-        self.target_drop = target_drop
-        self.max_steps = max_steps
-        self.steps_per_poll = steps_per_poll
         self.prep_workers = prep_workers
         self.keep_state = keep_state
         self.clock = clock
         self.sleep = sleep
         self.log = log                 # progress lines, every log_every s
         self.log_every = log_every
+        self.L = _lib()
+        self._objs = {}                # handle -> (mesh, prim) for Python arenas
+        self._next = 0
+        self._arenas = []
+        self.native = isinstance(getattr(batch, "h", None), C.c_void_p)
+        main = self._backend(batch)
+        pol = self._backend(polish) if polish is not None else None
+        self._clock = _CLOCK(lambda _: float(self.clock()))
+        cfg = ServeCfg(target_drop, max_steps, steps_per_poll,
+                       int(max_active or 0), switch_drop, stall_polls, gamma,
+                       int(keep_state), self._clock, None)
+        h = C.c_void_p()
+        self._check(self.L.ccfd_serve_create(
+            C.byref(h), C.byref(main), C.byref(pol) if pol else None,
+            C.byref(cfg)))
+        self.h = h
 
-    def _finish(self, arena, k, rec, inf, drop, conv, t0, results):
-        if self.keep_state:
-            rec["prim"] = arena.prim(k)
-        arena.remove(k)
-        rec.update(done=self.clock() - t0, steps=rec["steps0"] + inf["steps"],
-                   drop=drop, converged=bool(conv),
-                   nonfinite=inf["nonfinite"])
-        for key in ("steps0", "res0", "hist", "mesh", "state", "cfl"):
-            rec.pop(key, None)
-        rec["latency"] = rec["done"] - rec["arrival"]
-        results.append(rec)
+    def _backend(self, arena):
+        # a device arena (ccfd_amd.batch.Batch) is driven from C; anything
+        # else with the same methods through callbacks
+        native = isinstance(getattr(arena, "h", None), C.c_void_p)
+        if native != self.native:
+            raise ValueError("both arenas must be device batches or both "
+                             "stand-ins")
+        if native:
+            be = Backend()
+            self._check(self.L.ccfd_serve_backend_batch(arena.h, C.byref(be)))
+            return be
+        a = _PyArena(arena, self._objs)
+        self._arenas.append(a)
+        return a.be
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.L.ccfd_gpu_last_error()
+            msg = msg.decode() if msg else ""
+            raise (ValueError if rc == CCFD_ENOMEM else RuntimeError)(msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ccfd_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _submit(self, r, mesh, prim, cfl, inflight):
+        if self.native:
+            prim = np.ascontiguousarray(prim, dtype=np.float64)
+            assert prim.shape == (mesh.n_elems, 4)
+            handle = C.cast(mesh.ptr, C.c_void_p).value
+            pp = N.dptr(prim)
+        else:
+            self._next += 1
+            handle = self._next
+            self._objs[handle] = (mesh, prim)
+            pp = None
+        rec = dict(r.meta)
+        rec.update(id=r.id, n=mesh.n_elems, arrival=r.arrival)
+        # the mesh and state stay referenced until the result is back
+        inflight[r.id] = (rec, mesh, prim, handle)
+        self._check(self.L.ccfd_serve_submit(self.h, r.id, handle,
+                                             mesh.n_elems, pp, cfl))
+
+    def _collect(self, inflight, t0, results):
+        buf = (ServeResult * 64)()
+        while True:
+            n = C.c_int()
+            self._check(self.L.ccfd_serve_results(self.h, 64, buf, C.byref(n)))
+            for j in range(n.value):
+                x = buf[j]
+                rec, mesh, prim, handle = inflight.pop(x.id)
+                rec.update(admitted=x.admitted - t0, done=x.done - t0,
+                           steps=int(x.steps), polish_steps=int(x.polishSteps),
+                           drop=float(x.drop), converged=bool(x.converged),
+                           nonfinite=bool(x.nonfinite))
+                rec["latency"] = rec["done"] - rec["arrival"]
+                if self.keep_state:
+                    out = np.empty((mesh.n_elems, 4))
+                    self._check(self.L.ccfd_serve_take_state(
+                        self.h, x.id, N.dptr(out)))
+                    rec["prim"] = out
+                self._objs.pop(handle, None)
+                results.append(rec)
+            if n.value < 64:
+                return
 
     def run(self, requests):
         reqs = sorted(requests, key=lambda r: (r.arrival, r.id))
         pending = collections.deque(reqs)     # not yet arrived
         prepping = collections.deque()        # (request, future), FIFO
-        active = {}                           # main arena id -> record
-        polishing = {}                        # polish arena id -> record
-        moving = collections.deque()          # records waiting for polish room
+        inflight = {}                         # submitted, not yet retired
         results = []
         t0 = self.clock()
         next_log = self.log_every
+        busy = False
         with cf.ThreadPoolExecutor(self.prep_workers) as pool:
-            while pending or prepping or active or polishing or moving:
+            while pending or prepping or busy:
                 now = self.clock() - t0
-                if self.log and now >= next_log:
-                    next_log = now + self.log_every
-                    self.log("serve: %.1f s, %d done, %d active (%d cells), "
-                             "%d polishing, %d waiting" % (
-                                 now, len(results), len(active),
-                                 sum(r["n"] for r in active.values()),
-                                 len(polishing) + len(moving),
-                                 len(prepping) + len(pending)))
                 while pending and pending[0].arrival <= now:
                     r = pending.popleft()
                     prepping.append((r, pool.submit(r.make)))
-                # solves handed over to the polish arena, in order
-                if moving and not polishing and \
-                        not self.polish.fits(moving[0]["mesh"]):
-                    raise ValueError("request %d does not fit the empty "
-                                     "polish arena" % moving[0]["id"])
-                while moving and self.polish.fits(moving[0]["mesh"]):
-                    rec = moving.popleft()
-                    k = self.polish.add(rec["mesh"], rec.pop("state"),
-                                        cfl=rec["cfl"])
-                    polishing[k] = rec
-                # admit prepared requests in arrival order while they fit
-                while prepping and prepping[0][1].done() and (
-                        self.max_active is None or
-                        len(active) + len(polishing) + len(moving) <
-                        self.max_active):
-                    r, fut = prepping[0]
+                # prepared requests join the server's queue in arrival order
+                while prepping and prepping[0][1].done():
+                    r, fut = prepping.popleft()
                     mesh, prim, cfl = fut.result()
-                    if not self.batch.fits(mesh):
-                        if not active:
-                            raise ValueError(
-                                "request %d (%d cells) does not fit the empty "
-                                "arena" % (r.id, mesh.n_elems))
-                        break
-                    prepping.popleft()
-                    k = self.batch.add(mesh, prim, cfl=cfl)
-                    rec = dict(r.meta)
-                    rec.update(id=r.id, n=mesh.n_elems, arrival=r.arrival,
-                               admitted=self.clock() - t0, cfl=cfl, steps0=0,
-                               res0=None, hist=[], polish_steps=0,
-                               mesh=mesh if self.polish else None)
-                    active[k] = rec
-                    del mesh, prim
-                if not (active or polishing):
-                    # idle: wait for the next arrival, preparation or room
+                    self._submit(r, mesh, prim, cfl, inflight)
+                act, pol, que = C.c_int(), C.c_int(), C.c_int()
+                self._check(self.L.ccfd_serve_round(
+                    self.h, C.byref(act), C.byref(pol), C.byref(que)))
+                self._collect(inflight, t0, results)
+                busy = act.value + pol.value + que.value > 0
+                if self.log and now >= next_log:
+                    next_log = now + self.log_every
+                    self.log("serve: %.1f s, %d done, %d active, %d polishing, "
+                             "%d waiting" % (now, len(results), act.value,
+                                             pol.value, que.value + len(prepping)
+                                             + len(pending)))
+                if act.value == 0 and pol.value == 0:
+                    # idle: wait for the next arrival or preparation
                     wait = 0.002
-                    if pending and not prepping and not moving:
+                    if pending and not prepping:
                         wait = max(0.0, min(0.05, pending[0].arrival - now))
                     self.sleep(wait)
-                    continue
-                if active:
-                    self.batch.step(self.steps_per_poll)
-                if polishing:
-                    self.polish.step(self.steps_per_poll)
-                if active:
-                    self._poll_main(active, moving, t0, results)
-                if polishing:
-                    self._poll_polish(polishing, t0, results)
         return sorted(results, key=lambda x: x["id"])
-
-    def _poll_main(self, active, moving, t0, results):
-        for k, inf in self.batch.poll().items():
-            if k not in active:
-                continue
-            rec = active[k]
-            if rec["res0"] is None:
-                rec["res0"] = inf["res0"][0]
-            drop = inf["res"][0] / rec["res0"] if rec["res0"] > 0 else 0.0
-            conv = drop <= self.target_drop
-            capped = inf["steps"] >= self.max_steps
-            if conv or capped or inf["nonfinite"]:
-                active.pop(k)
-                self._finish(self.batch, k, rec, inf, drop, conv, t0, results)
-                continue
-            if self.polish is None:
-                continue
-            rec["hist"].append(drop)
-            h = rec["hist"]
-            # stalled near the FP32 floor; a solve that stalls far above it
-            # (unsteady flow, limiter cycling) would not converge in FP64
-            # either and stays on the cheaper arithmetic until the cap
-            stalled = (len(h) > self.stall_polls and
-                       drop <= 10.0 * self.switch_drop and
-                       min(h[-self.stall_polls:]) >
-                       0.5 * min(h[:-self.stall_polls]))
-            if drop <= self.switch_drop or stalled:
-                # continue from the FP64 conservative state in the polish arena
-                rec["state"] = cons_to_prim(self.batch.cons(k), self.gamma)
-                rec["steps0"] = inf["steps"]
-                self.batch.remove(k)
-                active.pop(k)
-                moving.append(rec)
-
-    def _poll_polish(self, polishing, t0, results):
-        for k, inf in self.polish.poll().items():
-            if k not in polishing:
-                continue
-            rec = polishing[k]
-            drop = inf["res"][0] / rec["res0"] if rec["res0"] > 0 else 0.0
-            conv = drop <= self.target_drop
-            rec["polish_steps"] = inf["steps"]
-            if conv or rec["steps0"] + inf["steps"] >= self.max_steps or \
-                    inf["nonfinite"]:
-                polishing.pop(k)
-                self._finish(self.polish, k, rec, inf, drop, conv, t0, results)
 
 
 def summary(results, makespan=None):

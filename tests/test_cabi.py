@@ -23,6 +23,7 @@ def declared(header):
 @pytest.mark.parametrize("header,lib", [
     ("ccfd_gpu.h", "libccfd_gpu.so"),
     ("ccfd_batch.h", "libccfd_gpu.so"),
+    ("ccfd_serve.h", "libccfd_gpu.so"),
     ("ccfd_mesh.h", "libccfd_host.so"),
     ("ccfd_dropin.h", "libccfd_host.so"),
 ])
@@ -42,7 +43,7 @@ def test_every_header_is_checked():
     hdrs = sorted(f for f in os.listdir(os.path.join(ROOT, "include"))
                   if f.endswith(".h"))
     assert hdrs == ["ccfd_batch.h", "ccfd_dropin.h", "ccfd_gpu.h",
-                    "ccfd_mesh.h"]
+                    "ccfd_mesh.h", "ccfd_serve.h"]
 
 
 def _entry_points(header):
@@ -71,14 +72,16 @@ def _null_args(params):
     return vals
 
 
-@pytest.mark.parametrize("header", ["ccfd_gpu.h", "ccfd_batch.h"])
+@pytest.mark.parametrize("header", ["ccfd_gpu.h", "ccfd_batch.h",
+                                    "ccfd_serve.h"])
 def test_null_handles_are_rejected(header):
     L = C.CDLL(os.path.join(LIB, "libccfd_gpu.so"))
     L.ccfd_gpu_last_error.restype = C.c_char_p
     checked = 0
     for name, params in _entry_points(header):
         first = params[0] if params else ""
-        if not ("ccfd_gpu_ctx" in first or "ccfd_batch" in first):
+        if not ("ccfd_gpu_ctx" in first or "ccfd_batch" in first or
+                "ccfd_server" in first):
             continue
         f = getattr(L, name)
         f.restype = C.c_int
@@ -89,7 +92,7 @@ def test_null_handles_are_rejected(header):
             assert rc == -1, (name, rc)  # CCFD_EINVAL
             assert L.ccfd_gpu_last_error(), name
         checked += 1
-    assert checked >= 15
+    assert checked >= (5 if header == "ccfd_serve.h" else 15)
 
 
 def test_context_free_entry_points_check_arguments_first():

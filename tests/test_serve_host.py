@@ -1,7 +1,9 @@
-"""Host logic of the serving loop (ccfd_amd/serve.py) with a stand-in batch
-and a simulated clock: FIFO admission within the HBM budget, continuous
-batching (admission while others run), retirement on convergence or the
-step cap, latency accounting. Runs without a GPU."""
+"""Host logic of the serving loop (the C ABI of include/ccfd_serve.h,
+ccfd_amd/csrc/serve.cpp, driven by ccfd_amd/serve.py) with stand-in arenas
+reached through callbacks and a simulated clock: FIFO admission within the
+HBM budget, continuous batching (admission while others run), retirement
+on convergence or the step cap, precision promotion, latency accounting,
+argument checks. Runs without a GPU."""
 import math
 
 import numpy as np
@@ -226,3 +228,42 @@ def test_without_polish_the_floor_caps_the_solves():
                    sleep=clock.sleep)
     res = srv.run(_requests(6, rate=200.0, seed=12))
     assert all(not r["converged"] and r["steps"] >= 3000 for r in res)
+
+
+def test_c_abi_checks_its_arguments():
+    """ccfd_serve_create refuses incomplete backends and bad settings; an
+    empty server's round does nothing"""
+    import ctypes as C
+    L = S._lib()
+    clock = Clock()
+    fb = FakeBatch(cap=100, clock=clock)
+    a = S._PyArena(fb, {})
+    cfg = S.ServeCfg(1e-6, 100, 10, 0, 1e-4, 4, 1.4, 0, S._CLOCK(lambda _: 0.0),
+                     None)
+    h = C.c_void_p()
+    empty = S.Backend()
+    assert L.ccfd_serve_create(C.byref(h), C.byref(empty), None,
+                               C.byref(cfg)) != 0
+    bad = S.ServeCfg(1e-6, 100, 0, 0, 1e-4, 4, 1.4, 0, cfg.clock, None)
+    assert L.ccfd_serve_create(C.byref(h), C.byref(a.be), None,
+                               C.byref(bad)) != 0
+    assert L.ccfd_serve_create(C.byref(h), C.byref(a.be), None,
+                               C.byref(cfg)) == 0
+    n = [C.c_int(-1) for _ in range(3)]
+    assert L.ccfd_serve_round(h, *[C.byref(x) for x in n]) == 0
+    assert [x.value for x in n] == [0, 0, 0]
+    assert L.ccfd_serve_submit(h, 1, None, 10, None, 0.9) != 0
+    out = np.zeros(4)
+    assert L.ccfd_serve_take_state(h, 1, out.ctypes.data_as(
+        C.POINTER(C.c_double))) != 0
+    L.ccfd_serve_destroy(h)
+
+
+def test_request_larger_than_an_empty_arena_is_an_error():
+    clock = Clock()
+    fb = FakeBatch(cap=1000, clock=clock)
+    srv = S.Server(fb, steps_per_poll=20, prep_workers=1, clock=clock,
+                   sleep=clock.sleep)
+    import pytest
+    with pytest.raises(ValueError, match="does not fit the empty arena"):
+        srv.run([S.Request(0, 0.0, lambda: (FakeMesh(4000), 0, 0.9))])
