@@ -372,6 +372,8 @@ static OpArgs<T> arena_args(ccfd_batch* b) {
   a.eq.gam1q = T(qd.gam1q);
   a.eq.mu = T(0.0);
   a.eq.Pr = T(qd.Pr);
+  a.eqD = qd;
+  a.eqD.mu = 0.0;
   a.blockMap = b->blockMap;
   a.blockInst = b->blockInst;
   a.tInst = b->tInst;

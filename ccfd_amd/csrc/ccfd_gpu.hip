@@ -866,6 +866,7 @@ OpArgs<T> ccfd::op_args(ccfd_gpu_ctx* c) {
   Eq<double> qd = eqd(c);
   a.eq.gam = T(qd.gam); a.eq.gam1 = T(qd.gam1); a.eq.gam1q = T(qd.gam1q);
   a.eq.mu = T(qd.mu); a.eq.Pr = T(qd.Pr);
+  a.eqD = qd;
   a.ex = c->ex;
   a.bct = c->bct;
   return a;

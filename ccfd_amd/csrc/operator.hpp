@@ -85,6 +85,8 @@ struct OpArgs {
   // residual partial sums [4][nblocks] of area * ut^2
   double* res_part;
   Eq<T> eq;
+  Eq<double> eqD;   // the same constants in FP64: the FP64 state's consPrim
+                    // and the time-step limits
   ExactCfg ex;
   BCTable bct;
   // batched arena (all null for one mesh): per launched block,
@@ -393,9 +395,7 @@ __device__ inline void stage_epilogue(const OpArgs<T>& a, int i,
 #pragma unroll
       for (int v = 0; v < 4; ++v) a.U[v * Np + i] = Uc[v];
     }
-    Eq<double> qd = {double(a.eq.gam), double(a.eq.gam1), double(a.eq.gam1q),
-                     double(a.eq.mu), double(a.eq.Pr)};
-    cons_prim<double>(qd, Uc, pn);
+    cons_prim<double>(a.eqD, Uc, pn);
 #pragma unroll
     for (int v = 0; v < 4; ++v) a.prim_out[v * Np + i] = T(pn[v]);
   }

@@ -291,6 +291,56 @@ def test_advance_device_dt_matches_host_loop(variant):
     b.close()
 
 
+@pytest.mark.parametrize("case", ["fp64_tile512", "fp32_face", "ns_split",
+                                  "ts1d_quads"])
+def test_device_time_steps_equal_the_host_loop(case):
+    """ccfd_gpu_advance (calcTimeStep and the steps on the device, several
+    steps per call) equals the host loop that calls calcTimeStep before
+    every step, bit for bit: 512-element tiles, FP32 flux, the viscous
+    limit (Navier-Stokes, split operator) and the 1-D time step on
+    quadrilaterals"""
+    kw = dict(flux=5, order=2, limiter=2, venk_k=5.0, nrk=4, torder=2)
+    tile = 0
+    if case in ("fp64_tile512", "fp32_face"):
+        mesh = cases.delaunay_mesh(120, seed=5, periodic=True)
+        prim = M.smooth_state(mesh.bary, seed=3)
+        if case == "fp32_face":
+            kw["precision"] = G.FP32_FLUX
+        else:
+            tile = 512
+    elif case == "ns_split":
+        raw = M.naca_ogrid(ni=64, nj=16, wall_spacing=4e-3, radius=6.0,
+                           wall_bc=201, far_bc=501)
+        mesh = M.Mesh(raw, [M.BC(201), M.BC(501, rho=1.0, mach=0.4, alpha=2.0,
+                                              pressure=1.0)])
+        prim = np.tile([1.0, 0.4 * np.sqrt(1.4), 0.0, 1.0], (mesh.n_elems, 1))
+        kw.update(flux=2, navierstokes=1, mu=2e-3)
+    else:
+        raw = M.cartesian(40, 3, (0.0, 0.0), (1.0, 0.05), (101, 401, 101, 401))
+        mesh = M.Mesh(raw, [M.BC(101), M.BC(401)])
+        prim = cases.sod_state(mesh)
+        kw.update(ts1d=1, nrk=3)
+    a = G.Solver(mesh, G.params(**kw))
+    b = G.Solver(mesh, G.params(**kw))
+    if tile:
+        b.set_variant(G.KV_FUSED_FACE)
+        b.set_tile(tile)
+        assert b.tile() == tile
+    a.set_prim(prim)
+    b.set_prim(prim)
+    t = 0.0
+    for _ in range(4):
+        dt, _ = a.calc_dt(t, 1e9, 1e9)
+        a.explicit_step(t, dt)
+        t += dt
+    t2, _ = b.advance(4, 0.0, 1e9, 1e9)
+    print(case, t, t2, flush=True)
+    assert t2 == t
+    assert np.array_equal(a.cons(), b.cons())
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("n,periodic", [(120, True), (300, True),
                                         (200, False)])
 def test_fused_equals_split_large_delaunay(n, periodic):
