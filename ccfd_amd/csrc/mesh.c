@@ -37,6 +37,52 @@ static int pair_cmp(const void *a, const void *b)
 	return (A->create > B->create) - (A->create < B->create);
 }
 
+/* The side list in pair_cmp order without a global comparison sort: a
+ * stable counting sort of the entries (given in creation order) by n0,
+ * then each n0 bucket ordered by pair_cmp - insertion sort for the few
+ * entries a node usually has, qsort for a large bucket. pair_cmp is a total
+ * order (creation breaks every tie), so the result is the one a single
+ * qsort gives, in time linear in the entries for bounded node degree.
+ * Returns the sorted copy (the input is left as is), NULL when out of
+ * memory. */
+static pair_entry *sort_pairs(const pair_entry *pe, long n, long nVert)
+{
+	long *start = calloc((size_t)nVert + 1, sizeof(long));
+	pair_entry *out = malloc(sizeof(pair_entry) * (n ? n : 1));
+	if (!start || !out) {
+		free(start);
+		free(out);
+		return NULL;
+	}
+	for (long i = 0; i < n; ++i)
+		start[pe[i].n0 + 1]++;
+	for (long v = 0; v < nVert; ++v)
+		start[v + 1] += start[v];
+	for (long i = 0; i < n; ++i)
+		out[start[pe[i].n0]++] = pe[i];
+	/* start[v] is now the end of bucket v */
+	long lo = 0;
+	for (long v = 0; v < nVert; ++v) {
+		long hi = start[v];
+		if (hi - lo > 32) {
+			qsort(out + lo, hi - lo, sizeof(pair_entry), pair_cmp);
+		} else {
+			for (long i = lo + 1; i < hi; ++i) {
+				pair_entry t = out[i];
+				long j = i - 1;
+				while (j >= lo && pair_cmp(out + j, &t) > 0) {
+					out[j + 1] = out[j];
+					--j;
+				}
+				out[j + 1] = t;
+			}
+		}
+		lo = hi;
+	}
+	free(start);
+	return out;
+}
+
 static void *xcalloc(size_t n, size_t s)
 {
 	void *p = calloc(n ? n : 1, s);
@@ -128,29 +174,15 @@ static void elem_info(ccfd_mesh *m, long e)
 	m->sy[e] = sy;
 }
 
-/* element owning slot s (binary search over slotStart) */
-static long slot_elem(const ccfd_mesh *m, long s)
-{
-	long lo = 0, hi = m->nElems - 1;
-	while (lo < hi) {
-		long mid = (lo + hi + 1) / 2;
-		if (m->slotStart[mid] <= s)
-			lo = mid;
-		else
-			hi = mid - 1;
-	}
-	return lo;
-}
 
 /*
  * createSideInfo (src/mesh.c:250-289) for the list side in slot s. "conn" is
  * either another slot (>= 0) or a ghost -(g + 1). Writes this side's n, len,
  * GP and the connection side's GP, n, len (and the ghost barycentre).
  */
-static void side_info(ccfd_mesh *m, long s, double *connN, double *connLen,
-		double *connGP)
+static void side_info(ccfd_mesh *m, long s, long e, double *connN,
+		double *connLen, double *connGP)
 {
-	long e = slot_elem(m, s);
 	long a = m->slotNode[2 * s], b = m->slotNode[2 * s + 1];
 	double x0 = m->xy[2 * a], y0 = m->xy[2 * a + 1];
 	double x1 = m->xy[2 * b], y1 = m->xy[2 * b + 1];
@@ -190,10 +222,12 @@ static void side_info(ccfd_mesh *m, long s, double *connN, double *connLen,
 	*connLen = len;
 }
 
-ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
+/* slotElem: the element of every slot (3 nTri + 4 nQuad entries), filled
+ * here */
+static ccfd_mesh *mesh_build(long nVert, const double *xy,
 		long nTri, const long *tri, long nQuad, const long *quad,
 		long nBCedge, const long *bcEdge,
-		int nBC, const ccfd_bc *bc, char *err)
+		int nBC, const ccfd_bc *bc, char *err, long *slotElem)
 {
 	ccfd_mesh *m = xcalloc(1, sizeof(*m));
 	if (!m) {
@@ -255,6 +289,8 @@ ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
 			}
 		}
 		m->slotStart[e] = nSlots;
+		for (int i = 0; i < type; ++i)
+			slotElem[nSlots + i] = e;
 		nSlots += type;
 	}
 	m->slotStart[nE] = nSlots;
@@ -346,7 +382,15 @@ ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
 		pe[create].slot = -(k + 1);
 		create++;
 	}
-	qsort(pe, nEntries, sizeof(pair_entry), pair_cmp);
+	{
+		pair_entry *sorted = sort_pairs(pe, nEntries, nVert);
+		free(pe);
+		pe = sorted;
+		if (!pe) {
+			free(edgeBC);
+			goto oom;
+		}
+	}
 
 	/* pairing (src/mesh.c:1713-1755); count ghosts first */
 	long nPairs = nEntries / 2, nBCpairs = 0;
@@ -401,8 +445,8 @@ ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
 			kb++;
 		} else {
 			long sa = A->slot, sb = B->slot;
-			m->nbr[sa] = slot_elem(m, sb);
-			m->nbr[sb] = slot_elem(m, sa);
+			m->nbr[sa] = slotElem[sb];
+			m->nbr[sb] = slotElem[sa];
 			m->partner[sa] = sb;
 			m->partner[sb] = sa;
 			m->owner[sa] = 1;
@@ -425,7 +469,7 @@ ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
 		if (!m->owner[s])
 			continue;
 		double cn[2], cl, cgp[2];
-		side_info(m, s, cn, &cl, cgp);
+		side_info(m, s, slotElem[s], cn, &cl, cgp);
 		long c = m->nbr[s];
 		if (c < 0) {
 			long g = -c - 1;
@@ -504,8 +548,8 @@ ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
 				return NULL;
 			}
 			long ur = bcPairSlot[k], tg = bcPairSlot[hit];
-			m->nbr[ur] = slot_elem(m, tg);
-			m->nbr[tg] = slot_elem(m, ur);
+			m->nbr[ur] = slotElem[tg];
+			m->nbr[tg] = slotElem[ur];
 			m->partner[ur] = tg;
 			m->partner[tg] = ur;
 			m->owner[ur] = 1;
@@ -615,6 +659,26 @@ oom:
 	SETERR("out of memory");
 	ccfd_mesh_free(m);
 	return NULL;
+}
+
+ccfd_mesh *ccfd_mesh_build(long nVert, const double *xy,
+		long nTri, const long *tri, long nQuad, const long *quad,
+		long nBCedge, const long *bcEdge,
+		int nBC, const ccfd_bc *bc, char *err)
+{
+	if (nTri < 0 || nQuad < 0) {
+		SETERR("negative element count");
+		return NULL;
+	}
+	long *slotElem = malloc(sizeof(long) * (3 * nTri + 4 * nQuad + 1));
+	if (!slotElem) {
+		SETERR("out of memory");
+		return NULL;
+	}
+	ccfd_mesh *m = mesh_build(nVert, xy, nTri, tri, nQuad, quad, nBCedge,
+			bcEdge, nBC, bc, err, slotElem);
+	free(slotElem);
+	return m;
 }
 
 void ccfd_mesh_venk_eps(const ccfd_mesh *m, double venk_k, double *eps_sq)

@@ -77,6 +77,38 @@ def test_face_ownership_follows_creation_order():
     assert m.n_faces == (m.owner == 1).sum()
 
 
+@pytest.mark.parametrize("hub", [0, 40])
+def test_pairing_order_on_a_high_degree_node(hub):
+    """a fan of 40 triangles around one node: with the hub numbered first its
+    side-list bucket holds 80 entries (the sort's large-bucket branch), last
+    it holds none. Either way every face pairs, the earlier-created side
+    owns it and the boundary sides become ghosts in edge order."""
+    k = 40
+    ang = 2 * np.pi * np.arange(k) / k
+    rim = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    ids = [i for i in range(k + 1) if i != hub]   # rim node j -> ids[j]
+    xy = np.zeros((k + 1, 2))
+    xy[ids] = rim
+    tri = np.array([[hub, ids[j], ids[(j + 1) % k], 0] for j in range(k)])
+    bce = np.array([[ids[j], ids[(j + 1) % k], 201] for j in range(k)])
+    m = M.Mesh(M.Raw(xy, tri=tri, bcedge=bce), [M.BC(201)])
+    se = _per_slot_elem(m)
+    inner = np.nonzero(m.nbr >= 0)[0]
+    assert len(inner) == 2 * k
+    assert (m.partner[m.partner[inner]] == inner).all()
+    own = inner[m.owner[inner] == 1]
+    assert len(own) == k and (se[own] < se[m.partner[own]]).all()
+    # spoke j is shared by triangles j - 1 and j; triangle 0 owns spoke 0
+    # (its neighbour is triangle k - 1, created later)
+    assert sorted(se[own].tolist()) == sorted([0] + list(range(k - 1)))
+    # the ghosts follow the sorted (node0, node1) order of the rim edges
+    assert m.n_ghosts == k
+    keys = [tuple(sorted((ids[j], ids[(j + 1) % k]))) for j in range(k)]
+    want = [e for _, e in sorted(zip(keys, range(k)))]
+    assert se[m.ghost_slot].tolist() == want
+    assert abs(m.area.sum() - 0.5 * k * np.sin(2 * np.pi / k)) < 1e-13
+
+
 def test_periodic_partners_translate():
     m = cases.delaunay_mesh(20, seed=1, periodic=True)
     se = _per_slot_elem(m)
