@@ -23,6 +23,17 @@ class CcfdBC(C.Structure):
                 ("pVar", C.c_double * 4), ("connection", C.c_double * 2)]
 
 
+class CcfdRawMesh(C.Structure):
+    """struct ccfd_raw_mesh (include/ccfd_mesh.h)"""
+    _fields_ = [
+        ("nVert", C.c_long), ("xy", c_double_p),
+        ("nTri", C.c_long), ("tri", c_long_p),
+        ("nQuad", C.c_long), ("quad", c_long_p),
+        ("nBCedge", C.c_long), ("bcEdge", c_long_p),
+        ("nBC", C.c_int), ("bc", C.POINTER(CcfdBC)),
+    ]
+
+
 class CcfdMesh(C.Structure):
     """struct ccfd_mesh (include/ccfd_mesh.h)"""
     _fields_ = [

@@ -51,6 +51,9 @@ def lib():
         "ccfd_batch_bytes": (C.c_size_t, [vp, vp]),
         "ccfd_batch_fits": (C.c_int, [vp, vp, ip]),
         "ccfd_batch_add": (C.c_int, [vp, vp, C.POINTER(BatchCase), dp, ip]),
+        "ccfd_batch_add_raw": (C.c_int, [vp, vp, C.POINTER(BatchCase), dp,
+                                         ip]),
+        "ccfd_batch_fits_raw": (C.c_int, [vp, vp, ip]),
         "ccfd_batch_remove": (C.c_int, [vp, C.c_int]),
         "ccfd_batch_step": (C.c_int, [vp, C.c_int]),
         "ccfd_batch_sync": (C.c_int, [vp]),
@@ -130,6 +133,24 @@ class Batch:
                                      C.byref(cs), N.dptr(prim), C.byref(k)),
                self.L)
         self.n_elems[k.value] = mesh.n_elems
+        return k.value
+
+    def fits_raw(self, raw_mesh):
+        f = C.c_int()
+        _check(self.L.ccfd_batch_fits_raw(
+            self.h, C.cast(raw_mesh.ptr, C.c_void_p), C.byref(f)), self.L)
+        return bool(f.value)
+
+    def add_raw(self, raw_mesh, prim, cfl=0.9, ptime=1e300, stop_time=1e300):
+        """admit a mesh.RawMesh, preprocessed on the device"""
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (raw_mesh.n_elems, 4)
+        cs = BatchCase(cfl, ptime, stop_time)
+        k = C.c_int()
+        _check(self.L.ccfd_batch_add_raw(
+            self.h, C.cast(raw_mesh.ptr, C.c_void_p), C.byref(cs),
+            N.dptr(prim), C.byref(k)), self.L)
+        self.n_elems[k.value] = raw_mesh.n_elems
         return k.value
 
     def remove(self, k):

@@ -21,7 +21,8 @@ INC = os.path.join(ROOT, "include")
 ORACLE = os.path.join(ROOT, "oracle")
 
 HOST_SRC = ["mesh.c", "ini.c", "driver.c"]
-GPU_SRC = ["ccfd_gpu.hip", "implicit.hip", "batch.hip", "analyze.hip"]
+GPU_SRC = ["ccfd_gpu.hip", "implicit.hip", "batch.hip", "analyze.hip",
+           "devmesh.hip"]
 HOST_CXX_SRC = ["soa.cpp", "agglomerate.cpp", "exact_host.cpp", "serve.cpp"]   # host C++ in libccfd_gpu.so
 FLUX_CODES = list(range(1, 12))
 OBJ = os.path.join(ROOT, "_build")

@@ -13,6 +13,7 @@
 #include "ctx.hpp"
 #include "range_alloc.hpp"
 #include "soa.hpp"
+#include "devmesh.hpp"
 
 using namespace ccfd;
 
@@ -659,6 +660,66 @@ extern "C" int ccfd_batch_fits(ccfd_batch* b, const ccfd_mesh* m, int* fits) {
   return CCFD_OK;
 }
 
+// the rest of an admission, once the instance's image is in the arena
+// (instance k, perm: image position -> reference element): boundary
+// conditions, per-instance scalars, the initial state, the launch tables.
+// A failure leaves the slot as ccfd_batch_remove would.
+static int admit_tail(ccfd_batch* b, int k, int nBC, const ccfd_bc* bc,
+                      const ccfd_batch_case* cs, const double* prim,
+                      const std::vector<int>& perm, int* id) {
+  const Inst& I = b->inst[k];
+  const int N = I.N, np = I.nbk * kBlock, E = b->capElems, e0 = I.b0 * kBlock;
+  // boundary conditions
+  if (nBC > 0) {
+    std::vector<BCDev> bcs(nBC);
+    for (int q = 0; q < nBC; ++q) {
+      memset(&bcs[q], 0, sizeof(BCDev));
+      bcs[q].type = bc[q].type;
+      bcs[q].exactFunc = bc[q].exactFunc;
+      for (int v = 0; v < 4; ++v) bcs[q].pVar[v] = bc[q].pVar[v];
+    }
+    hipMemcpyAsync(b->bcp + I.bc0, bcs.data(), bcs.size() * sizeof(BCDev),
+                   hipMemcpyHostToDevice, b->st);
+  }
+  // per-instance scalars
+  set_dbl(b, b->tInst + k, 0.0);
+  set_dbl(b, b->dtInst + k, 0.0);
+  set_dbl(b, b->cflInst + k, cs->cfl);
+  set_dbl(b, b->pTimeInst + k, cs->pTime);
+  set_dbl(b, b->stopInst + k, cs->stopTime);
+  set_dbl(b, b->areaQInst + k, I.totalArea_q);
+  set_dbl(b, b->stepsInst + k, 0.0);
+  set_dbl(b, b->nfInst + k, 0.0);
+  // a failure from here on leaves the slot as ccfd_batch_remove would:
+  // ranges released, tables rebuilt without it, the first error reported
+  auto undo = [&](int code) {
+    char msg[512];
+    snprintf(msg, sizeof(msg), "%s", ccfd_gpu_last_error());
+    ccfd_batch_remove(b, k);
+    return fail(code, "%s", msg);
+  };
+  // initial state (reference order -> renumbered [4][np] staging)
+  int rc = ensure_stage(b, 4 * (size_t)np * 8);
+  if (rc) return undo(rc);
+  std::vector<double> st(4 * (size_t)np, 1.0);
+  for (int i = 0; i < N; ++i)
+    for (int v = 0; v < 4; ++v) st[(size_t)v * np + i] = prim[4 * (size_t)perm[i] + v];
+  hipMemcpyAsync(b->stage, st.data(), st.size() * 8, hipMemcpyHostToDevice, b->st);
+  const Eq<double> q = beq(b);
+  if (b->ts == 8)
+    hipLaunchKernelGGL(k_bset_prim<double>, dim3((np + 255) / 256), dim3(256), 0, b->st,
+                       np, e0, E, b->stage, (double*)b->prim[b->cur], b->Ubuf[b->ucur], q);
+  else
+    hipLaunchKernelGGL(k_bset_prim<float>, dim3((np + 255) / 256), dim3(256), 0, b->st,
+                       np, e0, E, b->stage, (float*)b->prim[b->cur], b->Ubuf[b->ucur], q);
+  if (hipStreamSynchronize(b->st) != hipSuccess) {
+    fail(CCFD_EHIP, "instance upload failed");
+    return undo(CCFD_EHIP);
+  }
+  *id = k;
+  return rebuild_tables(b);
+}
+
 extern "C" int ccfd_batch_add(ccfd_batch* b, const ccfd_mesh* m,
                               const ccfd_batch_case* cs, const double* prim,
                               int* id) {
@@ -766,55 +827,153 @@ extern "C" int ccfd_batch_add(ccfd_batch* b, const ccfd_mesh* m,
   upload_ints(b, b->blockInst, bi, 1, nbk, b0, 0);
   if (!h.haloCells.empty())
     upload_ints(b, b->haloCells, h.haloCells, 1, (int)h.haloCells.size(), h0, 0);
-  // boundary conditions
-  if (m->nBC > 0) {
-    std::vector<BCDev> bcs(m->nBC);
-    for (int q = 0; q < m->nBC; ++q) {
-      memset(&bcs[q], 0, sizeof(BCDev));
-      bcs[q].type = m->bc[q].type;
-      bcs[q].exactFunc = m->bc[q].exactFunc;
-      for (int v = 0; v < 4; ++v) bcs[q].pVar[v] = m->bc[q].pVar[v];
-    }
-    hipMemcpyAsync(b->bcp + bc0, bcs.data(), bcs.size() * sizeof(BCDev),
-                   hipMemcpyHostToDevice, b->st);
+  return admit_tail(b, k, m->nBC, m->bc, cs, prim, h.perm, id);
+}
+
+static int raw_checks(ccfd_batch* b, const ccfd_raw_mesh* m) {
+  if (m->nTri < 0 || m->nQuad < 0 || m->nTri + m->nQuad <= 0)
+    return fail(CCFD_EINVAL, "empty mesh");
+  if (m->nBC < 0 || m->nBC > kMaxBC || (m->nBC && !m->bc))
+    return fail(CCFD_ENOTSUP, "at most %d boundary conditions", kMaxBC);
+  if ((m->nQuad > 0 ? 4 : 3) > b->S)
+    return fail(CCFD_ENOTSUP, "mesh has 4-sided elements, batch holds %d", b->S);
+  for (int k = 0; k < m->nBC; ++k) {
+    if (m->bc[k].exactFunc > 0 && (m->bc[k].type == 5 || m->bc[k].type == 6))
+      return fail(CCFD_ENOTSUP, "exact-function boundaries are not batched");
+    if (m->bc[k].type == CCFD_BC_PERIODIC)
+      return fail(CCFD_ENOTSUP, "periodic boundaries are paired on the host: "
+                                "use ccfd_batch_add");
   }
-  // per-instance scalars
-  set_dbl(b, b->tInst + k, 0.0);
-  set_dbl(b, b->dtInst + k, 0.0);
-  set_dbl(b, b->cflInst + k, cs->cfl);
-  set_dbl(b, b->pTimeInst + k, cs->pTime);
-  set_dbl(b, b->stopInst + k, cs->stopTime);
-  set_dbl(b, b->areaQInst + k, m->totalArea_q);
-  set_dbl(b, b->stepsInst + k, 0.0);
-  set_dbl(b, b->nfInst + k, 0.0);
-  // a failure from here on leaves the slot as ccfd_batch_remove would:
-  // ranges released, tables rebuilt without it, the first error reported
-  auto undo = [&](int code) {
-    char msg[512];
-    snprintf(msg, sizeof(msg), "%s", ccfd_gpu_last_error());
-    ccfd_batch_remove(b, k);
-    return fail(code, "%s", msg);
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_batch_fits_raw(ccfd_batch* b, const ccfd_raw_mesh* m, int* fits) {
+  if (!b || !m || !fits) return fail(CCFD_EINVAL, "null argument");
+  const long N = m->nTri + m->nQuad;
+  const int nbk = (int)((N + kBlock - 1) / kBlock);
+  bool slot = false;
+  for (auto& I : b->inst)
+    if (!I.used) { slot = true; break; }
+  *fits = slot && N > 0 && b->blocks.can(nbk) && b->ghosts.can((int)m->nBCedge) &&
+          b->halos.can((int)(N / 2 + 1)) && b->bcs.can(m->nBC);
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_batch_add_raw(ccfd_batch* b, const ccfd_raw_mesh* m,
+                                  const ccfd_batch_case* cs, const double* prim,
+                                  int* id) {
+  if (!b || !m || !cs || !prim || !id) return fail(CCFD_EINVAL, "null argument");
+  int rc = raw_checks(b, m);
+  if (rc) return rc;
+  hipSetDevice(b->device);
+  int k = -1;
+  for (int q = 0; q < b->maxInst; ++q)
+    if (!b->inst[q].used) { k = q; break; }
+  if (k < 0) return fail(CCFD_ENOMEM, "all %d instance slots in use", b->maxInst);
+  // the geometry first: it checks the mesh and counts the ghosts
+  DevMesh dm;
+  char err[256] = {0};
+  hipStreamSynchronize(b->st);
+  rc = dm.build(*m, b->st, err);
+  if (rc) return fail(rc, "%s", err);
+  const int N = dm.N, nbk = (N + kBlock - 1) / kBlock, G = dm.G;
+  const int b0 = b->blocks.alloc(nbk);
+  if (b0 < 0) return fail(CCFD_ENOMEM, "no room for %d blocks", nbk);
+  const int g0 = b->ghosts.alloc(G);
+  if (g0 < 0) {
+    b->blocks.release(b0, nbk);
+    return fail(CCFD_ENOMEM, "no room for %d ghosts", G);
+  }
+  const int bc0 = b->bcs.alloc(m->nBC);
+  if (bc0 < 0) {
+    b->blocks.release(b0, nbk);
+    b->ghosts.release(g0, G);
+    return fail(CCFD_ENOMEM, "no room for boundary conditions");
+  }
+  DevArenaDst d;
+  d.S = b->S;
+  d.E = b->capElems;
+  d.e0 = b0 * kBlock;
+  d.capG = b->capGhosts;
+  d.g0 = g0;
+  d.bcBase = bc0;
+  d.ts = (int)b->ts;
+  d.venkK = b->cfg.scheme.venk_k;
+  d.link = b->link;
+  d.w = b->w;
+  d.gp = b->gp;
+  d.fn = b->fn;
+  d.flen = b->flen;
+  d.areaq = b->areaq;
+  d.venk = b->venk;
+  d.area = b->area;
+  d.sx = b->sx;
+  d.sy = b->sy;
+  d.bary = b->bary;
+  d.ghostBC = b->ghostBC;
+  d.ghostCell = b->ghostCell;
+  d.ghostK = b->ghostK;
+  d.ghostBary = b->ghostBary;
+  std::vector<int> counts;
+  auto release = [&](int code) {
+    b->blocks.release(b0, nbk);
+    b->ghosts.release(g0, G);
+    b->bcs.release(bc0, m->nBC);
+    // the image written so far is unreferenced: no launch table lists it
+    // and the ghost entries are marked unused as ccfd_batch_remove does
+    if (G > 0)
+      hipLaunchKernelGGL(k_bfill_int, dim3((G + 255) / 256), dim3(256), 0, b->st,
+                         b->ghostBC + g0, G, -1);
+    hipStreamSynchronize(b->st);
+    return fail(code, "%s", err);
   };
-  // initial state (reference order -> renumbered [4][np] staging)
-  int rc = ensure_stage(b, 4 * (size_t)np * 8);
-  if (rc) return undo(rc);
-  std::vector<double> st(4 * (size_t)np, 1.0);
-  for (int i = 0; i < N; ++i)
-    for (int v = 0; v < 4; ++v) st[(size_t)v * np + i] = prim[4 * (size_t)h.perm[i] + v];
-  hipMemcpyAsync(b->stage, st.data(), st.size() * 8, hipMemcpyHostToDevice, b->st);
-  const Eq<double> q = beq(b);
-  if (b->ts == 8)
-    hipLaunchKernelGGL(k_bset_prim<double>, dim3((np + 255) / 256), dim3(256), 0, b->st,
-                       np, e0, E, b->stage, (double*)b->prim[b->cur], b->Ubuf[b->ucur], q);
-  else
-    hipLaunchKernelGGL(k_bset_prim<float>, dim3((np + 255) / 256), dim3(256), 0, b->st,
-                       np, e0, E, b->stage, (float*)b->prim[b->cur], b->Ubuf[b->ucur], q);
-  if (hipStreamSynchronize(b->st) != hipSuccess) {
-    fail(CCFD_EHIP, "instance upload failed");
-    return undo(CCFD_EHIP);
+  rc = dm.write(d, err);
+  if (!rc) rc = dm.halo_counts(d, counts, err);
+  if (rc) return release(rc);
+  std::vector<int> start(nbk + 1, 0);
+  int maxHalo = 0;
+  for (int j = 0; j < nbk; ++j) {
+    start[j + 1] = start[j] + counts[j];
+    maxHalo = std::max(maxHalo, counts[j]);
   }
-  *id = k;
-  return rebuild_tables(b);
+  const int nh = start[nbk];
+  const int h0 = b->halos.alloc(nh);
+  if (h0 < 0) {
+    snprintf(err, sizeof(err), "no room for the halo lists");
+    return release(CCFD_ENOMEM);
+  }
+  rc = dm.halo_write(d, start, b->haloCells + h0, err);
+  if (rc) {
+    b->halos.release(h0, nh);
+    return release(rc);
+  }
+  Inst& I = b->inst[k];
+  I.used = true;
+  I.wide = kBlock + maxHalo > b->tile;
+  if (I.wide) {
+    ++b->wideHalo;
+    free_graphs(b);
+  }
+  I.N = N;
+  I.nbk = nbk;
+  I.b0 = b0;
+  I.g0 = g0;
+  I.ng = G;
+  I.h0 = h0;
+  I.nh = nh;
+  I.bc0 = bc0;
+  I.nbc = m->nBC;
+  I.perm = dm.perm;
+  I.totalArea_q = dm.totalArea_q;
+  std::vector<int> hs(nbk), he(nbk), bi(nbk, k);
+  for (int j = 0; j < nbk; ++j) {
+    hs[j] = start[j] + h0;
+    he[j] = start[j + 1] + h0;
+  }
+  upload_ints(b, b->haloStart, hs, 1, nbk, b0, 0);
+  upload_ints(b, b->haloEnd, he, 1, nbk, b0, 0);
+  upload_ints(b, b->blockInst, bi, 1, nbk, b0, 0);
+  return admit_tail(b, k, m->nBC, m->bc, cs, prim, dm.perm, id);
 }
 
 extern "C" int ccfd_batch_remove(ccfd_batch* b, int id) {

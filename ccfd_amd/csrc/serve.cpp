@@ -314,6 +314,13 @@ int b_add(void* a, const void* m, const double* prim, double cfl, int* id) {
   ccfd_batch_case cs = {cfl, 1e300, 1e300};
   return ccfd_batch_add(B(a), (const ccfd_mesh*)m, &cs, prim, id);
 }
+int b_fits_raw(void* a, const void* m, int* f) {
+  return ccfd_batch_fits_raw(B(a), (const ccfd_raw_mesh*)m, f);
+}
+int b_add_raw(void* a, const void* m, const double* prim, double cfl, int* id) {
+  ccfd_batch_case cs = {cfl, 1e300, 1e300};
+  return ccfd_batch_add_raw(B(a), (const ccfd_raw_mesh*)m, &cs, prim, id);
+}
 int b_remove(void* a, int id) { return ccfd_batch_remove(B(a), id); }
 int b_step(void* a, int n) { return ccfd_batch_step(B(a), n); }
 int b_poll(void* a, int maxIds, int* ids, ccfd_batch_info* info, int* n) {
@@ -333,5 +340,13 @@ extern "C" int ccfd_serve_backend_batch(ccfd_batch* b, ccfd_serve_backend* out) 
   out->poll = b_poll;
   out->get_cons = b_cons;
   out->get_prim = b_prim;
+  return CCFD_OK;
+}
+
+extern "C" int ccfd_serve_backend_batch_raw(ccfd_batch* b, ccfd_serve_backend* out) {
+  int rc = ccfd_serve_backend_batch(b, out);
+  if (rc) return rc;
+  out->fits = b_fits_raw;
+  out->add = b_add_raw;
   return CCFD_OK;
 }

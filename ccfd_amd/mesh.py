@@ -116,6 +116,26 @@ class Raw:
         return r
 
 
+class RawMesh:
+    """Raw connectivity and boundary conditions handed to the device as they
+    are (struct ccfd_raw_mesh): the batched arena preprocesses it on the GPU
+    at admission (ccfd_batch_add_raw). Keeps the arrays alive while the
+    struct points at them."""
+
+    def __init__(self, raw, bcs):
+        self.raw = raw
+        self.bcs = list(bcs)
+        self.n_elems = len(raw.tri) + len(raw.quad)
+        self._bc = (N.CcfdBC * max(1, len(self.bcs)))()
+        for i, b in enumerate(self.bcs):
+            self._bc[i] = b.to_c()
+        self._s = N.CcfdRawMesh(
+            len(raw.xy), N.dptr(raw.xy), len(raw.tri), N.lptr(raw.tri),
+            len(raw.quad), N.lptr(raw.quad), len(raw.bcedge),
+            N.lptr(raw.bcedge), len(self.bcs), self._bc)
+        self.ptr = C.pointer(self._s)
+
+
 class Mesh:
     """Preprocessed mesh in the reference's element / slot order."""
 

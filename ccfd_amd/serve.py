@@ -84,6 +84,7 @@ def _lib():
         return L
     vp = C.c_void_p
     L.ccfd_serve_backend_batch.argtypes = [vp, C.POINTER(Backend)]
+    L.ccfd_serve_backend_batch_raw.argtypes = [vp, C.POINTER(Backend)]
     L.ccfd_serve_create.argtypes = [C.POINTER(vp), C.POINTER(Backend),
                                     C.POINTER(Backend), C.POINTER(ServeCfg)]
     L.ccfd_serve_destroy.argtypes = [vp]
@@ -189,7 +190,10 @@ class Server:
                  steps_per_poll=50, prep_workers=8, keep_state=False,
                  max_active=None, polish=None, switch_drop=1e-4,
                  stall_polls=4, gamma=1.4, clock=time.perf_counter,
-                 sleep=time.sleep, log=None, log_every=10.0):
+                 sleep=time.sleep, log=None, log_every=10.0, raw=False):
+        # raw: requests carry mesh.RawMesh objects, preprocessed on the
+        # device at admission (device arenas only)
+        self.raw = raw
         self.batch = batch
         self.polish = polish           # FP64 arena for the final drop
         self.prep_workers = prep_workers
@@ -224,8 +228,12 @@ class Server:
                              "stand-ins")
         if native:
             be = Backend()
-            self._check(self.L.ccfd_serve_backend_batch(arena.h, C.byref(be)))
+            fn = (self.L.ccfd_serve_backend_batch_raw if self.raw
+                  else self.L.ccfd_serve_backend_batch)
+            self._check(fn(arena.h, C.byref(be)))
             return be
+        if self.raw:
+            raise ValueError("raw meshes need device arenas")
         a = _PyArena(arena, self._objs)
         self._arenas.append(a)
         return a.be

@@ -69,23 +69,26 @@ def served_specs(n_req=2000, seed=5, cells=(1e4, 2e6)):
     return out
 
 
-def served_case(spec, gamma=1.4):
-    """(mesh, free-stream primitive state, CFL) of one steady solve"""
+def served_case(spec, gamma=1.4, raw=False):
+    """(mesh, free-stream primitive state, CFL) of one steady solve; with raw
+    the mesh is a mesh.RawMesh (preprocessed on the device at admission)
+    instead of a host-preprocessed mesh.Mesh"""
     n, mach, aoa = spec["cells"], spec["mach"], spec["aoa"]
     if spec["kind"] == "naca":
         nj = max(8, int(round(math.sqrt(n / 2.0))))
         ni = max(16, 2 * nj)
-        raw = M.naca_ogrid(ni, nj, wall_bc=101, far_bc=501)
+        raw_mesh = M.naca_ogrid(ni, nj, wall_bc=101, far_bc=501)
         bcs = [M.BC(101), M.BC(501, rho=1.0, mach=mach, alpha=aoa,
                                 pressure=1.0, gamma=gamma)]
     else:
         nj = max(4, int(round(math.sqrt(n / 3.0))))
         ni = 3 * nj
-        raw = M.channel_bump(ni, nj, wall_bc=101, in_bc=501, out_bc=801)
+        raw_mesh = M.channel_bump(ni, nj, wall_bc=101, in_bc=501,
+                                  out_bc=801)
         bcs = [M.BC(101), M.BC(501, rho=1.0, mach=mach, alpha=0.0,
                                 pressure=1.0, gamma=gamma),
                M.BC(801, pressure=1.0)]
-    mesh = M.Mesh(raw, bcs)
+    mesh = M.RawMesh(raw_mesh, bcs) if raw else M.Mesh(raw_mesh, bcs)
     c = math.sqrt(gamma)
     a = math.radians(aoa)
     prim = np.tile([1.0, mach * c * math.cos(a), mach * c * math.sin(a), 1.0],

@@ -98,6 +98,25 @@ typedef struct ccfd_mesh {
 } ccfd_mesh;
 
 /*
+ * Raw connectivity as a generator or a mesh file gives it (the arguments of
+ * ccfd_mesh_build in one struct; the arrays belong to the caller). The
+ * batched arena can build a request's device image from it on the GPU
+ * (ccfd_batch_add_raw) instead of from a host-preprocessed ccfd_mesh.
+ */
+typedef struct ccfd_raw_mesh {
+	long nVert;
+	const double *xy;      /* 2 * nVert */
+	long nTri;
+	const long *tri;       /* 4 * nTri: 3 node ids + domain */
+	long nQuad;
+	const long *quad;      /* 5 * nQuad: 4 node ids + domain */
+	long nBCedge;
+	const long *bcEdge;    /* 3 * nBCedge: 2 node ids + code 100*type+id */
+	int nBC;
+	const ccfd_bc *bc;
+} ccfd_raw_mesh;
+
+/*
  * Build a mesh from raw connectivity.
  *   xy      : 2 * nVert vertex coordinates
  *   tri     : 4 * nTri  (3 zero-based node ids + domain)

@@ -49,6 +49,9 @@ typedef struct ccfd_serve_backend {
 /* the backend of a ccfd_batch (meshes are const ccfd_mesh *; steps run
  * without an output or end time) */
 int ccfd_serve_backend_batch(ccfd_batch *b, ccfd_serve_backend *out);
+/* the same with meshes given as const ccfd_raw_mesh *, each preprocessed on
+ * the device at admission (ccfd_batch_add_raw) */
+int ccfd_serve_backend_batch_raw(ccfd_batch *b, ccfd_serve_backend *out);
 
 typedef struct ccfd_serve_cfg {
 	double targetDrop;      /* density residual drop of a solve (1e-6) */

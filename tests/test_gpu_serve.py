@@ -28,12 +28,13 @@ def _scheme(prec):
     return G.params(flux=9, order=1, nrk=4, torder=2, precision=prec)
 
 
-def _requests(n, rate, seed):
+def _requests(n, rate, seed, raw=False):
     specs = W.served_specs(n, seed=seed, cells=(800, 3000))
     for sp in specs:
         sp["mach"] = 0.3 + 0.3 * (sp["id"] % 3) / 2.0   # subsonic: converges
     arr = SV.poisson_arrivals(n, rate, seed)
-    return [SV.Request(sp["id"], a, lambda sp=sp: W.served_case(sp), sp)
+    return [SV.Request(sp["id"], a,
+                       lambda sp=sp: W.served_case(sp, raw=raw), sp)
             for sp, a in zip(specs, arr)]
 
 
@@ -91,3 +92,25 @@ def test_promotion_to_the_fp64_arena():
     assert any(q["converged"] for q in res64)
     for a in (main, pol, b64):
         a.close()
+
+
+def test_device_preprocessed_requests_equal_host_preprocessed():
+    """requests handed over as raw connectivity and preprocessed on the
+    device at admission (ccfd_serve_backend_batch_raw) get the same steps,
+    drops and states as requests preprocessed on the host"""
+    out = []
+    for raw in (False, True):
+        b = BT.Batch(_scheme(G.FP64), max_sides=4, local_dt=True,
+                     hbm_bytes=1 << 30)
+        srv = SV.Server(b, target_drop=1e-5, max_steps=3000,
+                        steps_per_poll=100, keep_state=True, prep_workers=4,
+                        raw=raw)
+        out.append(srv.run(_requests(6, rate=50.0, seed=21, raw=raw)))
+        srv.close()
+        b.close()
+    print([(r["id"], r["steps"], "%.2e" % r["drop"]) for r in out[1]],
+          flush=True)
+    for r, q in zip(out[0], out[1]):
+        assert r["id"] == q["id"] and r["n"] == q["n"]
+        assert r["steps"] == q["steps"] and r["drop"] == q["drop"]
+        assert np.array_equal(r["prim"], q["prim"])
