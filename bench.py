@@ -312,7 +312,7 @@ def _err_summary(errs, both=None):
 
 
 def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
-               venk_k=5.0):
+               venk_k=5.0, raw=True):
     """Served steady solves: Poisson arrivals, continuous batching in an
     FP32-flux arena with local time steps, retire on a 1e-6 density
     residual drop (or the step cap); then the same requests one at a time in
@@ -327,7 +327,8 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
     arr = SV.poisson_arrivals(n_req, rate, seed)
 
     def requests(arrivals):
-        return [SV.Request(sp["id"], a, (lambda sp=sp: W.served_case(sp)), sp)
+        return [SV.Request(sp["id"], a,
+                           (lambda sp=sp: W.served_case(sp, raw=raw)), sp)
                 for sp, a in zip(specs, arrivals)]
 
     def scheme(prec):
@@ -339,14 +340,15 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
            "scheme": "AUSMD, VK K=%g, RK4 local time steps at CFL 2; served: "
                      "FP32 flux / FP64 state until a 1e-4 drop or a stall, "
                      "then FP64 to the target; baseline: FP64 one solve at a "
-                     "time" % venk_k}
+                     "time; meshes %s" % (venk_k, "preprocessed on the device "
+                     "at admission" if raw else "preprocessed on the host")}
     b = BT.Batch(scheme(G.FP32_FLUX), max_sides=4, local_dt=True,
                  hbm_bytes=(3 * hbm_gb << 30) // 4)
     pol = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
                    hbm_bytes=(hbm_gb << 30) // 4)
     srv = SV.Server(b, target_drop=1e-6, max_steps=max_steps,
                     steps_per_poll=100, keep_state=baseline, polish=pol,
-                    switch_drop=1e-4, log=log)
+                    switch_drop=1e-4, log=log, raw=raw)
     res = srv.run(requests(arr))
     b.close()
     pol.close()
@@ -357,6 +359,7 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
                        hbm_bytes=hbm_gb << 30)
         srv64 = SV.Server(b64, target_drop=1e-6, max_steps=max_steps,
                           steps_per_poll=100, keep_state=True, max_active=1,
+                          raw=raw,
                           log=log)
         res64 = srv64.run(requests(np.zeros(n_req)))
         b64.close()
@@ -471,6 +474,48 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
     return out
 
 
+def prep_run(sizes=(2e5, 2e6), kinds=("naca", "bump"), repeats=3):
+    """Request-time preparation of one served request, host against device:
+    host = ccfd_mesh_build (mesh.Mesh) + ccfd_batch_add (build_soa and
+    build_halo on the host, uploads, initial state); device = mesh.RawMesh +
+    ccfd_batch_add_raw (pairing, geometry, order and halos on the GPU, same
+    initial state). Both paths admit the same instance bit for bit. Best of
+    `repeats` after one warm-up admission."""
+    import numpy as np
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+    from ccfd_amd import mesh as M
+    from ccfd_amd import workloads as W
+    sch = G.params(flux=9, order=2, limiter=2, venk_k=5.0, nrk=4, torder=2)
+    b = BT.Batch(sch, max_sides=4, local_dt=True, hbm_bytes=16 << 30)
+    out = {}
+    for kind in kinds:
+        for n in sizes:
+            spec = {"cells": n, "kind": kind, "mach": 0.7, "aoa": 2.0}
+            rec = {}
+            for path in ("host", "device"):
+                best = None
+                for r in range(repeats + 1):
+                    t0 = time.perf_counter()
+                    mesh, prim, cfl = W.served_case(spec, raw=path == "device")
+                    t1 = time.perf_counter()
+                    k = (b.add_raw if path == "device" else b.add)(mesh, prim,
+                                                                   cfl=cfl)
+                    b.sync()
+                    t2 = time.perf_counter()
+                    b.remove(k)
+                    if r and (best is None or t2 - t0 < best[0] + best[1]):
+                        best = (t1 - t0, t2 - t1)
+                rec[path] = {"prepare_s": best[0], "admit_s": best[1],
+                             "total_s": best[0] + best[1]}
+            rec["cells"] = int(mesh.n_elems)
+            rec["speedup"] = rec["host"]["total_s"] / rec["device"]["total_s"]
+            out["%s_%g" % (kind, n)] = rec
+            log("prep %s %g: %s" % (kind, n, json.dumps(rec)))
+    b.close()
+    return out
+
+
 def implicit_run(steps, cfl, ni=512, nj=256, venk_k=1000.0, chunk=500,
                  modes=("fp64", "fp32_flux"), save_dir=None):
     """Implicit Navier-Stokes around a NACA 2412 (m=0.02, p=0.4, t=0.12) on
@@ -575,7 +620,7 @@ def main():
                     help="also time a 1 GiB streaming copy (reachable HBM "
                          "bandwidth; counter calibration under rocprofv3)")
     ap.add_argument("--workload", choices=["operator", "batched", "served",
-                                           "implicit"],
+                                           "implicit", "prep"],
                     default="operator",
                     help="operator: the 4M-triangle stage benchmark (plus a "
                          "short served sample unless --no-served); batched: "
@@ -654,6 +699,19 @@ def main():
                "config": {"workload": "512x256 O-grid NS Re 5000, to a 1e-8 "
                                       "drop or %d steps" % r["max_steps"]},
                "implicit": r}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "prep":
+        r = prep_run()
+        out = {"metric": "request preparation s (host vs device)",
+               "value": r["naca_2e+06"]["device"]["total_s"], "unit": "s",
+               "n_gpus": 1, "steps": None, "warmup": 1,
+               "higher_is_better": False, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic (NACA O-grid and channel-bump requests)",
+               "config": {"workload": "one served request's mesh set-up and "
+                                      "admission, 2e5 and 2e6 cells"},
+               "prep": r}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "served":

@@ -191,8 +191,9 @@ class Server:
                  max_active=None, polish=None, switch_drop=1e-4,
                  stall_polls=4, gamma=1.4, clock=time.perf_counter,
                  sleep=time.sleep, log=None, log_every=10.0, raw=False):
-        # raw: requests carry mesh.RawMesh objects, preprocessed on the
-        # device at admission (device arenas only)
+        # raw: requests carry mesh.RawMesh objects, which a device arena
+        # preprocesses on the GPU at admission (a stand-in arena gets the
+        # object as it is)
         self.raw = raw
         self.batch = batch
         self.polish = polish           # FP64 arena for the final drop
@@ -232,8 +233,6 @@ class Server:
                   else self.L.ccfd_serve_backend_batch)
             self._check(fn(arena.h, C.byref(be)))
             return be
-        if self.raw:
-            raise ValueError("raw meshes need device arenas")
         a = _PyArena(arena, self._objs)
         self._arenas.append(a)
         return a.be
