@@ -58,12 +58,45 @@ def build_host():
            "-o", out] + srcs + ["-lm"]
     if not os.path.exists(gpu):
         raise RuntimeError("build the GPU library first (libccfd_gpu.so)")
-    cmd += ["-L" + LIB, "-lccfd_gpu", "-Wl,-rpath,$ORIGIN"]
+    cmd += ["-L" + LIB, "-lccfd_gpu", "-Wl,-rpath,$ORIGIN", "-ldl"]
     _run(cmd)
+    build_cgns()
     # `ccfd case.ini`: the reference's command line on the GPU path
     _run(["gcc", "-std=c99", "-O2", "-I" + INC, "-o", os.path.join(LIB, "ccfd"),
           os.path.join(CSRC, "ccfd_main.c"), "-L" + LIB, "-lccfd_host",
           "-lccfd_gpu", "-Wl,-rpath,$ORIGIN", "-lm"])
+    return out
+
+
+def hdf5_prefix():
+    """an HDF5 installation with the C headers and library, or None"""
+    import glob
+    for pre in (os.environ.get("HDF5_DIR"), "/opt/conda", "/usr",
+                "/usr/lib/x86_64-linux-gnu/hdf5/serial"):
+        if not pre:
+            continue
+        inc = os.path.join(pre, "include", "hdf5.h")
+        libs = glob.glob(os.path.join(pre, "lib", "libhdf5.so*"))
+        if os.path.exists(inc) and libs:
+            return pre
+    return None
+
+
+def build_cgns():
+    """libccfd_cgns.so: the CGNS output and restart files of the drop-in,
+    written with HDF5 (loaded by the drop-in only when a case uses CGNS).
+    Without HDF5 it is not built, and such cases stop with a message."""
+    out = os.path.join(LIB, "libccfd_cgns.so")
+    pre = hdf5_prefix()
+    if pre is None:
+        if os.path.exists(out):
+            os.remove(out)
+        return None
+    _run(["gcc", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-O2", "-fPIC",
+          "-shared", "-Wall", "-I" + INC, "-I" + os.path.join(pre, "include"),
+          "-o", out, os.path.join(CSRC, "cgns_hdf5.c"),
+          "-L" + os.path.join(pre, "lib"), "-lhdf5",
+          "-Wl,-rpath," + os.path.join(pre, "lib")])
     return out
 
 

@@ -6,16 +6,20 @@
  * initBoundary, initMesh, initInitialCondition, initFV, initTimeDisc,
  * initLinearSolver), the mesh is preprocessed on the host (mesh.c) and the
  * state lives on the device; timeDisc reproduces the reference's loop
- * (src/timeDiscretization.c:559-777) with CSV output and the analysis
- * consumers of src/analyze.c (residual file, CL / CD / CP, record points,
- * error norms).
+ * (src/timeDiscretization.c:559-777) with the reference's output formats
+ * (CGNS through libccfd_cgns.so, curve, CSV), restarts from a CGNS flow
+ * solution, and the analysis consumers of src/analyze.c (residual file,
+ * CL / CD / CP, record points, error norms).
  */
+#define _GNU_SOURCE   /* dladdr */
+#include <dlfcn.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
 
+#include "ccfd_cgns.h"
 #include "ccfd_dropin.h"
 
 #define FAILX(...) do { fprintf(stdout, "| ERROR: "); fprintf(stdout, __VA_ARGS__); fprintf(stdout, "\n"); exit(1); } while (0)
@@ -57,7 +61,64 @@ static struct {
 	long *recElem;
 	FILE **recFile;
 	FILE *resFile;
+	/* restart (src/main.c:64-94) and output bookkeeping (src/output.c) */
+	int restart;
+	long iniIter;
+	double timeOverall;
+	char gridFile[600];
+	long nOut, capOut;
+	double *outTime;
+	long *outIter;
+	long nBCedge;
+	long *bcEdge;
+	double printTime0;
 } C;
+
+/* libccfd_cgns.so, loaded on the first CGNS file */
+static struct {
+	void *h;
+	int (*write_mesh)(const char *, const ccfd_mesh *, long, const long *, char *);
+	int (*write_solution)(const char *, const char *, const ccfd_mesh *,
+			const double *, double, double, char *);
+	int (*write_master)(const char *, const char *, const ccfd_mesh *, int,
+			const double *, const long *, const char *const *, char *);
+	int (*read_solution)(const char *, long, double *, double *, double *, char *);
+} CG;
+
+static int cgns_load(char *e)
+{
+	if (CG.h)
+		return 0;
+	Dl_info di;
+	char path[1024];
+	const char *dir = ".";
+	int n = 1;
+	if (dladdr((void *)cgns_load, &di) && di.dli_fname) {
+		const char *sl = strrchr(di.dli_fname, '/');
+		if (sl) {
+			dir = di.dli_fname;
+			n = (int)(sl - di.dli_fname);
+		}
+	}
+	snprintf(path, sizeof(path), "%.*s/libccfd_cgns.so", n, dir);
+	void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+	if (!h) {
+		snprintf(e, 256, "CGNS files need libccfd_cgns.so (HDF5): %s", dlerror());
+		return -1;
+	}
+	*(void **)&CG.write_mesh = dlsym(h, "ccfd_cgns_write_mesh");
+	*(void **)&CG.write_solution = dlsym(h, "ccfd_cgns_write_solution");
+	*(void **)&CG.write_master = dlsym(h, "ccfd_cgns_write_master");
+	*(void **)&CG.read_solution = dlsym(h, "ccfd_cgns_read_solution");
+	if (!CG.write_mesh || !CG.write_solution || !CG.write_master ||
+			!CG.read_solution) {
+		snprintf(e, 256, "libccfd_cgns.so lacks the CGNS entry points");
+		dlclose(h);
+		return -1;
+	}
+	CG.h = h;
+	return 0;
+}
 
 static double wall(void)
 {
@@ -78,12 +139,18 @@ static int open_analysis_files(char *e)
 	char name[600];
 	if (C.calcWing || C.stationary) {
 		snprintf(name, sizeof(name), "%s_analysis.csv", C.fileName);
-		C.resFile = fopen(name, "w");
+		/* a restart appends to an existing file (src/analyze.c:209-234) */
+		FILE *old = C.restart ? fopen(name, "r") : NULL;
+		if (old)
+			fclose(old);
+		C.resFile = fopen(name, old ? "a" : "w");
 		if (!C.resFile) {
 			snprintf(e, 256, "cannot open %s", name);
 			return -1;
 		}
-		if (C.calcWing)
+		if (old)
+			;
+		else if (C.calcWing)
 			fprintf(C.resFile, "Iter, Time, Residual(%s), CL, CD\n", C.abortName);
 		else
 			fprintf(C.resFile, "Iter, Time, Residual(RHO), Residual(VX), Residual(VY), Residual(E)\n");
@@ -145,12 +212,19 @@ static void close_analysis_files(void)
 
 int ccfd_init(const char *iniFile, int device, char *err)
 {
+	return ccfd_init_restart(iniFile, NULL, device, err);
+}
+
+int ccfd_init_restart(const char *iniFile, const char *restartFile, int device,
+		char *err)
+{
 	char e[256] = "";
 	ccfd_ini *ini = NULL;
 	double *xy = NULL;
 	long *tri = NULL, *quad = NULL, *bce = NULL;
 	ccfd_bc bcs[32];
 	double *prim = NULL;
+	double restartTime = 0.0;
 	ccfd_finalize();
 	memset(&C, 0, sizeof(C));
 	ini = ccfd_ini_read(iniFile, e);
@@ -158,6 +232,29 @@ int ccfd_init(const char *iniFile, int device, char *err)
 		goto fail;
 	/* main: stationary flag (src/main.c:54) */
 	NEED(ccfd_ini_bool(ini, "stationary", "T", &C.stationary), "stationary");
+	if (restartFile) {
+		/* src/main.c:64-94: the time stamp after the file name's last
+		 * '_' is the first iteration (stationary) or the start time */
+		FILE *rf = fopen(restartFile, "r");
+		if (!rf) {
+			snprintf(e, 256, "No restart file found");
+			goto fail;
+		}
+		fclose(rf);
+		const char *u = strrchr(restartFile, '_');
+		if (!u) {
+			snprintf(e, 256, "restart file name %s has no time stamp", restartFile);
+			goto fail;
+		}
+		double stamp = strtod(u + 1, NULL);
+		C.restart = 1;
+		if (C.stationary) {
+			C.iniIter = (long)(int)stamp;
+		} else {
+			C.iniIter = 0;
+			restartTime = stamp;
+		}
+	}
 	/* initOutput (src/output.c:36-48) */
 	NEED(ccfd_ini_str(ini, "fileName", NULL, C.fileName, sizeof(C.fileName)), "fileName");
 	NEED(ccfd_ini_dbl(ini, "IOtimeInterval", NULL, &C.IOtimeInterval), "IOtimeInterval");
@@ -306,6 +403,17 @@ int ccfd_init(const char *iniFile, int device, char *err)
 	C.mesh = ccfd_mesh_build(nv, xy, nt, tri, nq, quad, nb, bce, (int)nBC, bcs, e);
 	if (!C.mesh)
 		goto fail;
+	/* src/mesh.c:1918-1922: the grid file of CGNS output, written unless
+	 * restarting */
+	snprintf(C.gridFile, sizeof(C.gridFile), "%s_mesh.cgns", C.fileName);
+	C.nBCedge = nb;
+	C.bcEdge = bce;
+	bce = NULL;
+	if (C.outputFormat == 1 && !C.restart) {
+		if (cgns_load(e) || CG.write_mesh(C.gridFile, C.mesh, C.nBCedge,
+				C.bcEdge, e))
+			goto fail;
+	}
 	/* initInitialCondition (src/initialCondition.c:36-90) */
 	long icType;
 	NEED(ccfd_ini_long(ini, "icType", NULL, &icType), "icType");
@@ -501,9 +609,24 @@ int ccfd_init(const char *iniFile, int device, char *err)
 			goto fail;
 		}
 	}
+	/* initTimeDisc (src/timeDiscretization.c:226-239), then
+	 * setInitialCondition: a restart reads the state, the time and the
+	 * overall time from the CGNS file (src/initialCondition.c:94-158) */
+	C.t = restartTime;
+	C.printTime0 = (floor(restartTime / C.IOtimeInterval) + 1) * C.IOtimeInterval;
+	if (C.restart) {
+		double tr = 0.0, to = 0.0;
+		if (cgns_load(e) || CG.read_solution(restartFile, N, prim, &tr, &to, e))
+			goto fail;
+		if (ccfd_gpu_set_prim(C.ctx, prim)) {
+			snprintf(e, 256, "%s", ccfd_gpu_last_error());
+			goto fail;
+		}
+		C.t = tr;
+		C.timeOverall = to;
+	}
 	if (open_analysis_files(e))
 		goto fail;
-	C.t = 0.0;
 	C.ready = 1;
 	free(prim); free(xy); free(tri); free(quad); free(bce);
 	ccfd_ini_free(ini);
@@ -520,6 +643,13 @@ fail:
 void ccfd_finalize(void)
 {
 	close_analysis_files();
+	free(C.outTime);
+	free(C.outIter);
+	free(C.bcEdge);
+	C.outTime = NULL;
+	C.outIter = NULL;
+	C.bcEdge = NULL;
+	C.nOut = C.capOut = 0;
 	free(C.recElem);
 	free(C.recFile);
 	free(C.wingXY);
@@ -609,27 +739,20 @@ void GMRES_M(double time, double dt, double alpha, double **B, double normB,
 	free(x);
 }
 
-/* csvOutput (src/output.c:56-113): x-sorted rows (stable) of the element
- * barycentre x, density, x-velocity, pressure */
-int ccfd_write_csv(const char *path)
+/* the rows (x, rho, u, p) of every element sorted by x with the
+ * reference's bubble sort (stable, adjacent swaps; src/output.c:84-101) */
+static double *sorted_rows(const double *prim)
 {
-	if (!C.ready)
-		return -1;
 	long N = C.mesh->nElems;
-	double *prim = malloc(sizeof(double) * 4 * N);
-	double *rows = malloc(sizeof(double) * 4 * N);
-	if (!prim || !rows || ccfd_gpu_get_prim(C.ctx, prim)) {
-		free(prim);
-		free(rows);
-		return -1;
-	}
+	double *rows = malloc(sizeof(double) * 4 * (N ? N : 1));
+	if (!rows)
+		return NULL;
 	for (long i = 0; i < N; ++i) {
 		rows[4 * i] = C.mesh->bary[2 * i];
 		rows[4 * i + 1] = prim[4 * i];
 		rows[4 * i + 2] = prim[4 * i + 1];
 		rows[4 * i + 3] = prim[4 * i + 3];
 	}
-	/* bubble sort by x, as the reference (stable, adjacent swaps) */
 	long n = N;
 	int swapped = 1;
 	while (swapped && n > 0) {
@@ -645,40 +768,152 @@ int ccfd_write_csv(const char *path)
 		}
 		n--;
 	}
-	FILE *f = fopen(path, "w");
+	return rows;
+}
+
+/* csvOutput (src/output.c:56-113) */
+static int write_csv_prim(const char *path, const double *prim)
+{
+	double *rows = sorted_rows(prim);
+	FILE *f = rows ? fopen(path, "w") : NULL;
 	if (!f) {
-		free(prim);
 		free(rows);
 		return -1;
 	}
 	fprintf(f, "CoordinateX, Density, Velocity, Pressure\n");
-	for (long i = 0; i < N; ++i)
+	for (long i = 0; i < C.mesh->nElems; ++i)
 		fprintf(f, "%15.9f,%15.9f,%15.9f,%15.9f\n", rows[4 * i],
 				rows[4 * i + 1], rows[4 * i + 2], rows[4 * i + 3]);
 	fclose(f);
-	free(prim);
 	free(rows);
 	return 0;
 }
 
-/* dataOutput (src/output.c:333-354): <fileName>_%015.7f.csv (transient) or
- * <fileName>_%09ld.csv (stationary); CSV only on this path */
+/* curveOutput (src/output.c:253-327): density, velocity and pressure
+ * against x, one block each */
+static int write_curve_prim(const char *path, const double *prim)
+{
+	double *rows = sorted_rows(prim);
+	FILE *f = rows ? fopen(path, "w") : NULL;
+	if (!f) {
+		free(rows);
+		return -1;
+	}
+	static const char *head[3] = {"#Density\n", "\n#Velocity\n", "\n#Pressure\n"};
+	for (int c = 0; c < 3; ++c) {
+		fprintf(f, "%s", head[c]);
+		for (long i = 0; i < C.mesh->nElems; ++i)
+			fprintf(f, "%15.9f,%15.9f\n", rows[4 * i], rows[4 * i + 1 + c]);
+	}
+	fclose(f);
+	free(rows);
+	return 0;
+}
+
+int ccfd_write_csv(const char *path)
+{
+	if (!C.ready)
+		return -1;
+	long N = C.mesh->nElems;
+	double *prim = malloc(sizeof(double) * 4 * N);
+	int rc = (!prim || ccfd_gpu_get_prim(C.ctx, prim)) ? -1 : write_csv_prim(path, prim);
+	free(prim);
+	return rc;
+}
+
+/* one file of dataOutput in the case's format: the flow (or, with exact,
+ * the exact solution) at time / iteration iter (src/output.c:333-395) */
+static void write_output(double time, long iter, const double *prim, int exact)
+{
+	char name[640], e[256] = "";
+	const char *ex = exact ? "_ex" : "";
+	if (C.stationary)
+		snprintf(name, sizeof(name), "%s%s_%09ld", C.fileName, ex, iter);
+	else
+		snprintf(name, sizeof(name), "%s%s_%015.7f", C.fileName, ex, time);
+	int rc;
+	switch (C.outputFormat) {
+	case 1:
+		strcat(name, ".cgns");
+		rc = cgns_load(e) || CG.write_solution(name, C.gridFile, C.mesh, prim,
+				time, C.timeOverall, e);
+		break;
+	case 2:
+		strcat(name, ".curve");
+		rc = write_curve_prim(name, prim);
+		break;
+	case 3:
+		strcat(name, ".csv");
+		rc = write_csv_prim(name, prim);
+		break;
+	default:
+		FAILX("Output Format unknown");
+	}
+	if (rc)
+		FAILX("could not write %s %s", name, e);
+}
+
+/* dataOutput (src/output.c:333-395): the flow state, the exact solution
+ * when the case has one, and the output time for the CGNS master file */
 static void data_output(double time, long iter)
 {
-	char name[600];
-	if (C.stationary)
-		snprintf(name, sizeof(name), "%s_%09ld.csv", C.fileName, iter);
-	else
-		snprintf(name, sizeof(name), "%s_%015.7f.csv", C.fileName, time);
-	if (C.outputFormat != 3) {
-		static int warned = 0;
-		if (!warned)
-			printf("| output format %d not available, writing CSV\n",
-					C.outputFormat);
-		warned = 1;
+	if (C.nOut == C.capOut) {
+		long cap = C.capOut ? 2 * C.capOut : 16;
+		double *t = realloc(C.outTime, sizeof(double) * cap);
+		if (t)
+			C.outTime = t;
+		long *it = realloc(C.outIter, sizeof(long) * cap);
+		if (it)
+			C.outIter = it;
+		if (!t || !it)
+			FAILX("out of memory");
+		C.capOut = cap;
 	}
-	if (ccfd_write_csv(name))
-		FAILX("could not write %s", name);
+	C.outTime[C.nOut] = time;
+	C.outIter[C.nOut] = iter;
+	C.nOut++;
+	long N = C.mesh->nElems;
+	double *prim = malloc(sizeof(double) * 4 * (N ? N : 1));
+	if (!prim || ccfd_gpu_get_prim(C.ctx, prim))
+		FAILX("could not read the state: %s", ccfd_gpu_last_error());
+	write_output(time, iter, prim, 0);
+	if (C.hasExactSolution) {
+		for (long i = 0; i < N; ++i)
+			if (ccfd_exact_func(C.exactFunc, &C.prm, C.mesh, C.mesh->bary + 2 * i,
+					time, prim + 4 * i))
+				FAILX("exact function %d failed", C.exactFunc);
+		write_output(time, iter, prim, 1);
+	}
+	free(prim);
+}
+
+/* finalizeDataOutput (src/output.c:407-525): the CGNS master file linking
+ * every flow solution written so far */
+static void finalize_output(void)
+{
+	if (C.outputFormat != 1)
+		return;
+	char name[600], e[256] = "";
+	snprintf(name, sizeof(name), "%s_Master.cgns", C.fileName);
+	char **files = calloc((size_t)(C.nOut ? C.nOut : 1), sizeof(char *));
+	if (!files)
+		FAILX("out of memory");
+	for (long k = 0; k < C.nOut; ++k) {
+		files[k] = malloc(640);
+		if (!files[k])
+			FAILX("out of memory");
+		if (C.stationary)
+			snprintf(files[k], 640, "%s_%09ld.cgns", C.fileName, C.outIter[k]);
+		else
+			snprintf(files[k], 640, "%s_%015.7f.cgns", C.fileName, C.outTime[k]);
+	}
+	int rc = cgns_load(e) || CG.write_master(name, C.gridFile, C.mesh, (int)C.nOut,
+			C.outTime, C.outIter, (const char *const *)files, e);
+	for (long k = 0; k < C.nOut; ++k)
+		free(files[k]);
+	free(files);
+	if (rc)
+		FAILX("could not write %s %s", name, e);
 }
 
 /* ---- analysis consumers, src/analyze.c ---- */
@@ -812,10 +1047,12 @@ void timeDisc(void)
 {
 	need_ready("timeDisc");
 	double t = C.t;
-	double printTime = (floor(t / C.IOtimeInterval) + 1) * C.IOtimeInterval;
-	long printIter = (0 / C.IOiterInterval + 1) * C.IOiterInterval;
+	/* src/timeDiscretization.c:238-239: from the restart file name's time
+	 * stamp (0 without a restart) and the first iteration */
+	double printTime = C.printTime0;
+	long printIter = (C.iniIter / C.IOiterInterval + 1) * C.IOiterInterval;
 	int hasConverged = C.stationary ? 0 : 1;
-	data_output(t, 0);
+	data_output(t, C.iniIter);
 	double dt;
 	int vd;
 	double t0 = wall(), tIO = t0;
@@ -824,7 +1061,7 @@ void timeDisc(void)
 	if (vd)
 		printf("| Viscous Time Step Dominates!\n");
 	long iter;
-	for (iter = 1; iter <= C.maxIter; ++iter) {
+	for (iter = C.iniIter + 1; iter <= C.maxIter; ++iter) {
 		calc_dt(t, printTime, &dt, &vd);
 		C.dt = dt;
 		double res[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -849,6 +1086,7 @@ void timeDisc(void)
 			printf("| Iteration Number: %ld\n", iter);
 			printf("| Writing Final State to Disk\n");
 			data_output(t, iter);
+			finalize_output();
 			if (C.hasExactSolution)
 				calcErrors(t);
 			break;
@@ -881,6 +1119,7 @@ void timeDisc(void)
 		}
 		if (C.stationary && hasConverged) {
 			data_output(t, iter);
+			finalize_output();
 			if (C.hasExactSolution)
 				calcErrors(t);
 			break;
@@ -912,6 +1151,7 @@ void timeDisc(void)
 			if (C.hasExactSolution)
 				calcErrors(t);
 			data_output(t, iter);
+			finalize_output();
 			if (printTime - t <= 1e-15)
 				printTime += C.IOtimeInterval;
 			if (iter == printIter)
@@ -926,6 +1166,7 @@ void timeDisc(void)
 		printf("| Current Time: %.10g\n", t);
 		printf("| Final State will be written to disk\n");
 		data_output(t, iter - 1);
+		finalize_output();
 		if (C.hasExactSolution)
 			calcErrors(t);
 	}
@@ -939,8 +1180,13 @@ void timeDisc(void)
 
 int ccfd_main(const char *iniFile, int device)
 {
+	return ccfd_main_restart(iniFile, NULL, device);
+}
+
+int ccfd_main_restart(const char *iniFile, const char *restartFile, int device)
+{
 	char err[256];
-	if (ccfd_init(iniFile, device, err)) {
+	if (ccfd_init_restart(iniFile, restartFile, device, err)) {
 		printf("| ERROR: %s\n", err);
 		return 1;
 	}

@@ -186,6 +186,7 @@ class Mesh:
         "w": ("w", "s", 2, np.float64),
         "bbv": ("bbv", "s", 2, np.float64),
         "bbd": ("bbd", "s", 1, np.float64),
+        "slot_node": ("slotNode", "s", 2, np.int64),
         "ghost_slot": ("ghostSlot", "g", 1, np.int64),
         "ghost_bc": ("ghostBC", "g", 1, np.int32),
         "ghost_bary": ("ghostBary", "g", 2, np.float64),

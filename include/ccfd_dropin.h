@@ -55,8 +55,18 @@ int ccfd_ini_count(ccfd_ini *ini, const char *key);
  * and the device context; returns 0 or a negative status (message in err,
  * >= 256 bytes, may be NULL) */
 int ccfd_init(const char *iniFile, int device, char *err);
+/* ccfd_init for a restart from a CGNS flow solution (src/main.c:64-94,
+ * src/initialCondition.c:94-158): the state, time and overall time come
+ * from the file, the first iteration (stationary) or the print schedule
+ * (transient) from the time stamp after the file name's last '_'; no grid
+ * file is written and the analysis file is appended to. restartFile NULL
+ * is ccfd_init. CGNS files need libccfd_cgns.so (include/ccfd_cgns.h). */
+int ccfd_init_restart(const char *iniFile, const char *restartFile,
+		int device, char *err);
 /* full run: ccfd_init + timeDisc + ccfd_finalize; returns 0 on success */
 int ccfd_main(const char *iniFile, int device);
+/* the same from a restart file (`ccfd case.ini restart.cgns`) */
+int ccfd_main_restart(const char *iniFile, const char *restartFile, int device);
 void ccfd_finalize(void);
 /* the global case */
 ccfd_gpu_ctx *ccfd_case_context(void);

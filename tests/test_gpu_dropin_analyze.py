@@ -111,19 +111,33 @@ def test_source_term_and_error_norms():
             ("IOiterInterval", 100000), ("outputFormat", 3),
             ("tEnd", 0.1), ("maxIter", 100000)])
         out = _run(d, ini)
-        final = sorted(glob.glob(os.path.join(d, "sine_*.csv")))[-1]
-        got = np.genfromtxt(final, skip_header=1, delimiter=",")
+        files = sorted(glob.glob(os.path.join(d, "sine_*.csv")))
+        flow = [p for p in files if "_ex_" not in os.path.basename(p)]
+        exact = [p for p in files if "_ex_" in os.path.basename(p)]
+        # dataOutput writes the exact solution beside every flow output
+        # (src/output.c:373-395): t = 0, 0.05 and the end
+        assert len(flow) == len(exact) == 3
+        got = np.genfromtxt(flow[-1], skip_header=1, delimiter=",")
+        got_ex = np.genfromtxt(exact[-1], skip_header=1, delimiter=",")
         t, it, prim, mesh, c = TD.run_case(ini)
     _, otext = TD.csv_rows(mesh, prim)
     want = np.genfromtxt(otext.splitlines()[1:], delimiter=",")
     print(out[-900:])
     assert abs(t - 0.1) < 1e-12
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-8)
-    # the last error report against the oracle's norms of its final state
     import oracle.bindings as B
-    orc = B.Oracle(mesh, B.make_cfg(
-        gamma=c["gam"], flux=c["flux"], order=c["order"], limiter=c["limiter"],
-        venk_k=c["venk_k"], source=c["source"]))
+    cfg = B.make_cfg(gamma=c["gam"], flux=c["flux"], order=c["order"],
+                     limiter=c["limiter"], venk_k=c["venk_k"],
+                     source=c["source"])
+    # the exact-solution file: the exact function at the barycentres at the
+    # final time, rows as csvOutput sorts them
+    ex = np.array([B.exact_func(cfg, mesh, 3, mesh.bary[i], t)
+                   for i in range(mesh.n_elems)])
+    _, etext = TD.csv_rows(mesh, ex)
+    want_ex = np.genfromtxt(etext.splitlines()[1:], delimiter=",")
+    np.testing.assert_allclose(got_ex, want_ex, rtol=0, atol=2e-9)
+    # the last error report against the oracle's norms of its final state
+    orc = B.Oracle(mesh, cfg)
     orc.set_prim(prim)
     L1, L2, Li = orc.calc_errors(3, t)
     blocks = out.split("Error Analysis at t = ")
