@@ -312,7 +312,7 @@ def _err_summary(errs, both=None):
 
 
 def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
-               venk_k=5.0, raw=True):
+               venk_k=5.0, raw=True, variant=None):
     """Served steady solves: Poisson arrivals, continuous batching in an
     FP32-flux arena with local time steps, retire on a 1e-6 density
     residual drop (or the step cap); then the same requests one at a time in
@@ -346,24 +346,41 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
                  hbm_bytes=(3 * hbm_gb << 30) // 4)
     pol = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
                    hbm_bytes=(hbm_gb << 30) // 4)
+    if variant is not None:
+        b.set_variant(variant)
+        pol.set_variant(variant)
+        out["variant"] = variant
     srv = SV.Server(b, target_drop=1e-6, max_steps=max_steps,
                     steps_per_poll=100, keep_state=baseline, polish=pol,
-                    switch_drop=1e-4, log=log, raw=raw)
+                    switch_drop=1e-4, log=log, raw=raw,
+                    autotune_steps=0 if variant is not None else 4)
     res = srv.run(requests(arr))
     b.close()
     pol.close()
     out["served"] = SV.summary(res)
+    names = {G.KV_AUTO: "auto", G.KV_SPLIT: "split", G.KV_FUSED: "fused",
+             G.KV_FUSED_FACE: "fused_face"}
+
+    def tuned(server):
+        cnt = {}
+        for arena, v in server.tuned:
+            key = "%s_%s" % (arena, names.get(v, str(v)))
+            cnt[key] = cnt.get(key, 0) + 1
+        return cnt
+    out["served"]["operator_choices"] = tuned(srv)
     log("served: %s" % json.dumps(out["served"]))
     if baseline:
         b64 = BT.Batch(scheme(G.FP64), max_sides=4, local_dt=True,
                        hbm_bytes=hbm_gb << 30)
         srv64 = SV.Server(b64, target_drop=1e-6, max_steps=max_steps,
                           steps_per_poll=100, keep_state=True, max_active=1,
-                          raw=raw,
+                          raw=raw, autotune_steps=0 if variant is not None else 4,
                           log=log)
         res64 = srv64.run(requests(np.zeros(n_req)))
+        one_choices = tuned(srv64)
         b64.close()
         out["one_at_a_time_fp64"] = SV.summary(res64)
+        out["one_at_a_time_fp64"]["operator_choices"] = one_choices
         both = [r["converged"] and q["converged"] for r, q in zip(res, res64)]
         out["density_err_vs_fp64"] = _err_summary(
             [_density_err(r["prim"], q["prim"]) for r, q in zip(res, res64)],
@@ -438,9 +455,15 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
         import numpy as np
         tot, errs, eerrs = 0.0, [], []
         nan32 = nan64 = 0
+        choices = {}
         for k, (m, p) in zip(ids, cases):
             s = G.Solver(m, sch64)
             s.set_prim(p)
+            # each mesh on its fastest operator, picked before the timed
+            # steps as the arena's is (the state is left as it was)
+            v, _, _ = s.autotune(3)
+            name = {1: "split", 2: "fused", 3: "fused_face"}.get(v, str(v))
+            choices[name] = choices.get(name, 0) + 1
             s.sync()
             w0 = time.perf_counter()
             s.advance(steps, 0.0, 1e300, 1e300, residual=True)
@@ -453,6 +476,7 @@ def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
             eerrs.append(_energy_err(q32, q64))
             s.close()
         out["one_at_a_time_fp64"] = {"seconds": tot,
+                                     "operator_choices": choices,
                                      "solves_per_s": len(ids) / tot,
                                      "cell_stage_updates_per_s":
                                          ncell * 3 * steps / tot}
@@ -633,6 +657,10 @@ def main():
                     help="timed steps of the other fluxes / FP32 mode")
     ap.add_argument("--served-requests", type=int, default=12)
     ap.add_argument("--served-rate", type=float, default=4.0)
+    ap.add_argument("--served-variant", default="auto",
+                    choices=["auto", "split", "fused", "fused_face"],
+                    help="served workload: operator of both arenas (auto: "
+                         "the arena's default)")
     ap.add_argument("--served-venk-k", type=float, default=5.0,
                     help="Venkatakrishnan constant of the served solves")
     ap.add_argument("--requests", type=int, default=2000)
@@ -716,8 +744,12 @@ def main():
         return
     if args.workload == "served":
         cells = (args.min_cells or 1e4, args.max_cells or 2e6)
+        from ccfd_amd import gpu as G
+        var = {"auto": None, "split": G.KV_SPLIT, "fused": G.KV_FUSED,
+               "fused_face": G.KV_FUSED_FACE}[args.served_variant]
         r = served_run(args.requests, cells, args.rate, args.max_steps,
-                       args.hbm_gb, venk_k=args.served_venk_k)
+                       args.hbm_gb, venk_k=args.served_venk_k,
+                       baseline=not args.no_baseline, variant=var)
         out = {"metric": "served steady solves/s (p50/p99 latency)",
                "value": r["served"]["solves_per_s"], "unit": "solves/s",
                "n_gpus": 1, "steps": None, "warmup": 0,

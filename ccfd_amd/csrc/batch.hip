@@ -23,6 +23,7 @@ struct Inst {
   bool used = false;
   bool wide = false;   // block halo larger than the fused tile
   int N = 0, nbk = 0, b0 = 0, g0 = 0, ng = 0, h0 = 0, nh = 0, bc0 = 0, nbc = 0;
+  int maxHalo = 0;     // largest halo list of the instance's blocks
   std::vector<int> perm;
   double totalArea_q = 0.0;
 };
@@ -39,6 +40,10 @@ struct ccfd_batch {
   size_t ts = 8;
   int capBlocks = 0, capElems = 0, capGhosts = 0, capHalo = 0, capBC = 0;
   int maxInst = 0, tile = 0;
+  // LDS columns the fused kernels use now: kBlock + the largest halo of the
+  // admitted instances (rounded up to 16), at most tile. Only the LDS
+  // layout depends on it, not any result.
+  int tileAct = 0;
   // arena arrays
   int* link = nullptr;
   void *w = nullptr, *gp = nullptr, *fn = nullptr, *flen = nullptr,
@@ -314,6 +319,10 @@ static int rebuild_tables(ccfd_batch* b) {
       map.push_back(((I.b0 + j) << 9) | std::min(kBlock, I.N - j * kBlock));
   }
   lb.push_back((int)map.size());
+  int halo = 0;
+  for (int k : order)
+    if (kBlock + b->inst[k].maxHalo <= b->tile) halo = std::max(halo, b->inst[k].maxHalo);
+  b->tileAct = std::min(b->tile, kBlock + (halo + 15) / 16 * 16);
   b->nLaunch = (int)map.size();
   b->nAct = (int)act.size();
   b->ghostHigh = b->ghosts.high();
@@ -425,10 +434,10 @@ static void launch_step(ccfd_batch* b) {
             else hipLaunchKernelGGL((bc_side_recon_kernel<T, 2, 4>), dim3(gBlocks), dim3(128), 0, b->st, a);
           }
         }
-        FusedArgs f{b->haloStart, b->haloEnd, b->haloCells, b->tile};
+        FusedArgs f{b->haloStart, b->haloEnd, b->haloCells, b->tileAct};
         launch_fused_code<T>(b->cfg.scheme.iFlux, a, f, lim,
                              kv == CCFD_KV_FUSED_FACE ? 1 : 0, b->nLaunch,
-                             fused_lds_bytes(sizeof(T), b->S, b->tile), b->st);
+                             fused_lds_bytes(sizeof(T), b->S, b->tileAct), b->st);
       } else {
         if (b->S == 3) {
           if (lim == 1) hipLaunchKernelGGL((recon_kernel<T, 1, 3>), grid, blk, 0, b->st, a);
@@ -766,6 +775,7 @@ extern "C" int ccfd_batch_add(ccfd_batch* b, const ccfd_mesh* m,
   Inst& I = b->inst[k];
   I.used = true;
   I.wide = kBlock + h.maxHalo > b->tile;
+  I.maxHalo = h.maxHalo;
   if (I.wide) {
     ++b->wideHalo;
     free_graphs(b);
@@ -950,6 +960,7 @@ extern "C" int ccfd_batch_add_raw(ccfd_batch* b, const ccfd_raw_mesh* m,
   Inst& I = b->inst[k];
   I.used = true;
   I.wide = kBlock + maxHalo > b->tile;
+  I.maxHalo = maxHalo;
   if (I.wide) {
     ++b->wideHalo;
     free_graphs(b);

@@ -190,11 +190,17 @@ class Server:
                  steps_per_poll=50, prep_workers=8, keep_state=False,
                  max_active=None, polish=None, switch_drop=1e-4,
                  stall_polls=4, gamma=1.4, clock=time.perf_counter,
-                 sleep=time.sleep, log=None, log_every=10.0, raw=False):
+                 sleep=time.sleep, log=None, log_every=10.0, raw=False,
+                 autotune_steps=0):
         # raw: requests carry mesh.RawMesh objects, which a device arena
         # preprocesses on the GPU at admission (a stand-in arena gets the
         # object as it is)
         self.raw = raw
+        # autotune_steps > 0: after a round that admitted solves to a device
+        # arena, time its operator implementations on the new mix
+        # (ccfd_batch_autotune; the state is restored) and keep the fastest
+        self.autotune_steps = autotune_steps
+        self.tuned = []                # (arena, variant) per tuning
         self.batch = batch
         self.polish = polish           # FP64 arena for the final drop
         self.prep_workers = prep_workers
@@ -301,6 +307,7 @@ class Server:
         t0 = self.clock()
         next_log = self.log_every
         busy = False
+        prev = {}                             # solves per arena, last round
         with cf.ThreadPoolExecutor(self.prep_workers) as pool:
             while pending or prepping or busy:
                 now = self.clock() - t0
@@ -316,6 +323,13 @@ class Server:
                 self._check(self.L.ccfd_serve_round(
                     self.h, C.byref(act), C.byref(pol), C.byref(que)))
                 self._collect(inflight, t0, results)
+                if self.native and self.autotune_steps > 0:
+                    for name, arena, n in (("main", self.batch, act.value),
+                                           ("polish", self.polish, pol.value)):
+                        if arena is not None and n > prev.get(name, 0):
+                            v, _ = arena.autotune(self.autotune_steps)
+                            self.tuned.append((name, v))
+                    prev = {"main": act.value, "polish": pol.value}
                 busy = act.value + pol.value + que.value > 0
                 if self.log and now >= next_log:
                     next_log = now + self.log_every
