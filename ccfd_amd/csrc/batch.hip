@@ -850,9 +850,6 @@ static int raw_checks(ccfd_batch* b, const ccfd_raw_mesh* m) {
   for (int k = 0; k < m->nBC; ++k) {
     if (m->bc[k].exactFunc > 0 && (m->bc[k].type == 5 || m->bc[k].type == 6))
       return fail(CCFD_ENOTSUP, "exact-function boundaries are not batched");
-    if (m->bc[k].type == CCFD_BC_PERIODIC)
-      return fail(CCFD_ENOTSUP, "periodic boundaries are paired on the host: "
-                                "use ccfd_batch_add");
   }
   return CCFD_OK;
 }

@@ -222,6 +222,102 @@ __global__ void k_side_info(int nSlots, int nTri, const double* __restrict__ xy,
   }
 }
 
+// connectPeriodicBC (src/mesh.c:172-244): per boundary condition its
+// periodic class (1: the side to connect, 2: a candidate, 0: neither), its
+// group (id / 10) and connection vector
+struct PerBC {
+  int cls[kMaxBC], group[kMaxBC];
+  double conn[kMaxBC][2];
+};
+
+__global__ void k_per_flag(int G, const int* __restrict__ gBC, PerBC pb,
+                           int* __restrict__ isCand) {
+  const int g = blockIdx.x * kT + threadIdx.x;
+  if (g >= G) return;
+  const int b = gBC[g];
+  isCand[g] = (b >= 0 && pb.cls[b] == 2) ? 1 : 0;
+}
+
+__global__ void k_per_gather(int G, const int* __restrict__ isCand,
+                             const int* __restrict__ pos, int* __restrict__ cand) {
+  const int g = blockIdx.x * kT + threadIdx.x;
+  if (g < G && isCand[g]) cand[pos[g]] = g;
+}
+
+// each side to connect takes the first candidate of its group (boundary-
+// side order) within 1e-7 (L1) of its GP position plus the connection;
+// hit[g] = the candidate's ghost index, -1 if none (err[0] = 3, err[1] = g)
+__global__ void k_per_match(int G, int nCand, const int* __restrict__ gBC,
+                            PerBC pb, const int* __restrict__ cand,
+                            const double* __restrict__ gGP,
+                            const double* __restrict__ gBary,
+                            int* __restrict__ hit, int* __restrict__ err) {
+  const int g = blockIdx.x * kT + threadIdx.x;
+  if (g >= G) return;
+  const int b = gBC[g];
+  hit[g] = -2;   // not a side to connect
+  if (b < 0 || pb.cls[b] != 1) return;
+  const double ax = gGP[2 * g] + gBary[2 * g];
+  const double ay = gGP[2 * g + 1] + gBary[2 * g + 1];
+  int h = -1;
+  for (int j = 0; j < nCand && h < 0; ++j) {
+    const int kk = cand[j];
+    if (pb.group[gBC[kk]] != pb.group[b]) continue;
+    const double sxp = gGP[2 * kk] + gBary[2 * kk];
+    const double syp = gGP[2 * kk + 1] + gBary[2 * kk + 1];
+    if ((fabs(ax + pb.conn[b][0] - sxp) + fabs(ay + pb.conn[b][1] - syp)) <= 1e-7)
+      h = kk;
+  }
+  hit[g] = h;
+  if (h < 0) {
+    atomicMax(err, 3);
+    atomicMin(err + 1, g);
+  }
+}
+
+// connect each matched pair (the side to connect owns the face) and mark
+// both ghosts as gone
+__global__ void k_per_link(int G, int nTri, const int* __restrict__ hit,
+                           const int* __restrict__ gSlot, int* __restrict__ nbr,
+                           int* __restrict__ partner,
+                           unsigned char* __restrict__ owner,
+                           int* __restrict__ keep) {
+  const int g = blockIdx.x * kT + threadIdx.x;
+  if (g >= G || hit[g] < 0) return;
+  const int h = hit[g];
+  const int ur = gSlot[g], tg = gSlot[h];
+  nbr[ur] = slot_elem(tg, nTri);
+  nbr[tg] = slot_elem(ur, nTri);
+  partner[ur] = tg;
+  partner[tg] = ur;
+  owner[ur] = 1;
+  owner[tg] = 0;
+  keep[g] = 0;
+  keep[h] = 0;
+}
+
+// the remaining ghosts, renumbered in boundary-side order
+__global__ void k_ghost_compact(int G, const int* __restrict__ keep,
+                                const int* __restrict__ pos,
+                                const int* __restrict__ gSlot,
+                                const int* __restrict__ gBC,
+                                const double* __restrict__ gBary,
+                                const double* __restrict__ gGP,
+                                int* __restrict__ nbr, int* __restrict__ oSlot,
+                                int* __restrict__ oBC, double* __restrict__ oBary,
+                                double* __restrict__ oGP) {
+  const int g = blockIdx.x * kT + threadIdx.x;
+  if (g >= G || !keep[g]) return;
+  const int n = pos[g], s = gSlot[g];
+  oSlot[n] = s;
+  oBC[n] = gBC[g];
+  oBary[2 * n] = gBary[2 * g];
+  oBary[2 * n + 1] = gBary[2 * g + 1];
+  oGP[2 * n] = gGP[2 * g];
+  oGP[2 * n + 1] = gGP[2 * g + 1];
+  nbr[s] = -(n + 1);
+}
+
 // connection GP and the face's normal and length on every slot
 __global__ void k_slot_conn(int nSlots, const int* __restrict__ nbr,
                             const int* __restrict__ partner,
@@ -344,6 +440,11 @@ __global__ void k_sfc_key(int N, const double* __restrict__ bary,
   const uint32_t iy = (uint32_t)((fy < 65535.0) ? fy : 65535.0);
   key[e] = hilbert_index(65536u, ix, iy);
   idx[e] = e;
+}
+
+__global__ void k_fill_one(int n, int* __restrict__ p) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i < n) p[i] = 1;
 }
 
 __global__ void k_inverse(int N, const int* __restrict__ perm,
@@ -549,9 +650,6 @@ int DevMesh::build(const ccfd_raw_mesh& m, hipStream_t st, char* err) {
       }
     if (type != 0 && edgeBC[k] < 0)
       DM_ERR(CCFD_EINVAL, "BC %ld from mesh file not defined in parameter file", code);
-    if (edgeBC[k] >= 0 && m.bc[edgeBC[k]].type == CCFD_BC_PERIODIC)
-      DM_ERR(CCFD_ENOTSUP, "periodic boundaries are paired on the host: "
-                           "use ccfd_batch_add");
     if (a < 0 || a >= m.nVert || b < 0 || b >= m.nVert)
       DM_ERR(CCFD_EINVAL, "BC edge %d references a missing node", k);
     bce[2 * (size_t)k] = (int)a;
@@ -643,11 +741,6 @@ int DevMesh::build(const ccfd_raw_mesh& m, hipStream_t st, char* err) {
   }
   tbs = tb2;
   hipcub::DeviceScan::ExclusiveSum(tmp, tbs, isBC, kb, nPairs, st);
-  int lastKb = 0, lastBC = 0;
-  if (nPairs > 0) {
-    hipMemcpyAsync(&lastKb, kb + nPairs - 1, 4, hipMemcpyDeviceToHost, st);
-    hipMemcpyAsync(&lastBC, isBC + nPairs - 1, 4, hipMemcpyDeviceToHost, st);
-  }
   hipLaunchKernelGGL(k_pair_link, dim3(nblk(nPairs)), dim3(kT), 0, st, nPairs, nSlots,
                      nTri, valB, isBC, kb, ebcD, nbr_, partner_, owner_, gSlot_, gBC_);
 
@@ -657,6 +750,85 @@ int DevMesh::build(const ccfd_raw_mesh& m, hipStream_t st, char* err) {
   hipLaunchKernelGGL(k_side_info, dim3(nblk(nSlots)), dim3(kT), 0, st, nSlots, nTri,
                      xy_, en_, bary_, nbr_, partner_, owner_, n_, len_, gp_, gBary_,
                      gGP_);
+  // periodic hookup and ghost compaction
+  int G0 = nPairs > 0 ? -1 : 0;
+  {
+    PerBC pb;
+    memset(&pb, 0, sizeof(pb));
+    bool periodic = false;
+    for (int q = 0; q < m.nBC && q < kMaxBC; ++q) {
+      if (m.bc[q].type != CCFD_BC_PERIODIC) continue;
+      pb.cls[q] = m.bc[q].id % 10 == 1 ? 1 : m.bc[q].id % 10 == 2 ? 2 : 0;
+      pb.group[q] = m.bc[q].id / 10;
+      pb.conn[q][0] = m.bc[q].connection[0];
+      pb.conn[q][1] = m.bc[q].connection[1];
+      periodic = periodic || pb.cls[q] != 0;
+    }
+    // the boundary pairs are known once the pairing's scan has run
+    if (nPairs > 0) {
+      int lk = 0, lb = 0;
+      hipMemcpyAsync(&lk, kb + nPairs - 1, 4, hipMemcpyDeviceToHost, st);
+      hipMemcpyAsync(&lb, isBC + nPairs - 1, 4, hipMemcpyDeviceToHost, st);
+      if (hipStreamSynchronize(st) != hipSuccess)
+        DM_ERR(CCFD_EHIP, "side pairing on the device failed");
+      G0 = lk + lb;
+    }
+    if (G0 != nB) DM_ERR(CCFD_EINVAL, "%d boundary edges but %d boundary sides", nB, G0);
+    G = G0;
+    if (periodic && G0 > 0) {
+      int* isCand = (int*)alloc(4 * (size_t)G0);
+      int* pos = (int*)alloc(4 * (size_t)G0);
+      int* cand = (int*)alloc(4 * (size_t)G0);
+      int* hit = (int*)alloc(4 * (size_t)G0);
+      int* keep = (int*)alloc(4 * (size_t)G0);
+      int* oSlot = (int*)alloc(4 * (size_t)G0);
+      int* oBC = (int*)alloc(4 * (size_t)G0);
+      double* oBary = (double*)alloc(16 * (size_t)G0);
+      double* oGP = (double*)alloc(16 * (size_t)G0);
+      size_t tb4 = 0;
+      hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, isCand, pos, G0, st);
+      void* tmp4 = alloc(tb4);
+      if (oom_) DM_ERR(CCFD_ENOMEM, "device allocation for the periodic hookup failed");
+      hipLaunchKernelGGL(k_per_flag, dim3(nblk(G0)), dim3(kT), 0, st, G0, gBC_, pb, isCand);
+      hipcub::DeviceScan::ExclusiveSum(tmp4, tb4, isCand, pos, G0, st);
+      int nCand = 0, lastC = 0;
+      hipMemcpyAsync(&nCand, pos + G0 - 1, 4, hipMemcpyDeviceToHost, st);
+      hipMemcpyAsync(&lastC, isCand + G0 - 1, 4, hipMemcpyDeviceToHost, st);
+      hipStreamSynchronize(st);
+      nCand += lastC;
+      hipLaunchKernelGGL(k_per_gather, dim3(nblk(G0)), dim3(kT), 0, st, G0, isCand, pos, cand);
+      hipLaunchKernelGGL(k_per_match, dim3(nblk(G0)), dim3(kT), 0, st, G0, nCand, gBC_, pb,
+                         cand, gGP_, gBary_, hit, errD);
+      int perr[2] = {0, 0};
+      hipMemcpyAsync(perr, errD, 8, hipMemcpyDeviceToHost, st);
+      if (hipStreamSynchronize(st) != hipSuccess)
+        DM_ERR(CCFD_EHIP, "periodic hookup on the device failed");
+      if (perr[0] == 3) {
+        double gp[2] = {0, 0}, gb[2] = {0, 0};
+        hipMemcpy(gp, gGP_ + 2 * (size_t)perr[1], 16, hipMemcpyDeviceToHost);
+        hipMemcpy(gb, gBary_ + 2 * (size_t)perr[1], 16, hipMemcpyDeviceToHost);
+        DM_ERR(CCFD_EINVAL, "connectPeriodicBC: no connection for side GP %g %g",
+               gp[0] + gb[0], gp[1] + gb[1]);
+      }
+      // keep = 1 everywhere, then the matched pairs leave
+      hipLaunchKernelGGL(k_fill_one, dim3(nblk(G0)), dim3(kT), 0, st, G0, keep);
+      hipLaunchKernelGGL(k_per_link, dim3(nblk(G0)), dim3(kT), 0, st, G0, nTri, hit, gSlot_,
+                         nbr_, partner_, owner_, keep);
+      hipcub::DeviceScan::ExclusiveSum(tmp4, tb4, keep, pos, G0, st);
+      int nk = 0, lastK = 0;
+      hipMemcpyAsync(&nk, pos + G0 - 1, 4, hipMemcpyDeviceToHost, st);
+      hipMemcpyAsync(&lastK, keep + G0 - 1, 4, hipMemcpyDeviceToHost, st);
+      hipLaunchKernelGGL(k_ghost_compact, dim3(nblk(G0)), dim3(kT), 0, st, G0, keep, pos,
+                         gSlot_, gBC_, gBary_, gGP_, nbr_, oSlot, oBC, oBary, oGP);
+      if (hipStreamSynchronize(st) != hipSuccess)
+        DM_ERR(CCFD_EHIP, "ghost compaction on the device failed");
+      G = nk + lastK;
+      gSlot_ = oSlot;
+      gBC_ = oBC;
+      gBary_ = oBary;
+      gGP_ = oGP;
+    }
+  }
   hipLaunchKernelGGL(k_slot_conn, dim3(nblk(nSlots)), dim3(kT), 0, st, nSlots, nbr_,
                      partner_, owner_, n_, len_, gp_, gGP_, gpc_, fn_, flen_);
   hipLaunchKernelGGL(k_recon_w, dim3(nblk(N)), dim3(kT), 0, st, N, nTri, gp_, gpc_, w_);
@@ -676,8 +848,6 @@ int DevMesh::build(const ccfd_raw_mesh& m, hipStream_t st, char* err) {
   hipMemcpyAsync(area.data(), area_, 8 * (size_t)N, hipMemcpyDeviceToHost, st);
   if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
     DM_ERR(CCFD_EHIP, "mesh geometry on the device failed");
-  G = nPairs > 0 ? lastKb + lastBC : 0;
-  if (G != nB) DM_ERR(CCFD_EINVAL, "%d boundary edges but %d boundary sides", nB, G);
   double tq = 0.0;
   for (int e = 0; e < N; ++e) {
     if (tq != 0.0) tq = 1.0 / (1.0 / tq + area[e]);

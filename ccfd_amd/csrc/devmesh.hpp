@@ -12,13 +12,15 @@
 //    each consecutive pair is one face, the earlier-created side owns it;
 //  * createElemInfo / createSideInfo (src/mesh.c:250-341) per element and
 //    per owning side, the ghost barycentre and GP of boundary sides;
+//  * connectPeriodicBC (src/mesh.c:172-244): each side of an x1 periodic
+//    boundary takes the first x2 side of its group, in boundary-side
+//    order, within 1e-7 of its position plus the connection vector; the
+//    remaining boundary sides are renumbered as ghosts in that order;
 //  * createReconstructionInfo (src/mesh.c:80-114) per element;
 //  * Hilbert order of the barycentres (stable radix sort of the curve
 //    index) and the arena scatter with renumbered links;
 //  * per 256-element block the ascending distinct ids of outside
 //    neighbours (one workgroup per block: LDS gather and bitonic sort).
-// Periodic boundaries (connectPeriodicBC, src/mesh.c:172-244) stay on the
-// host path.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -488,8 +488,9 @@ static ccfd_mesh *mesh_build(long nVert, const double *xy,
 	}
 
 	/* connectPeriodicBC (src/mesh.c:172-244). GP position of a BC side is
-	 * its (ghost) GP plus the ghost barycentre. Matching is by bucketed
-	 * search instead of the reference's list walk; a match is unique for
+	 * its (ghost) GP plus the ghost barycentre; each x1 side takes the
+	 * first x2 side of its group, in boundary-side order, within 1e-7 (L1)
+	 * of its position plus the connection vector. A match is unique for
 	 * any mesh whose periodic sides are more than 1e-7 apart. */
 	long nPer2 = 0;
 	for (long k = 0; k < nBCpairs; ++k) {

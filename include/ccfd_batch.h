@@ -85,16 +85,16 @@ int ccfd_batch_add(ccfd_batch *b, const ccfd_mesh *m,
  * side pairing (a stable radix sort of the side keys in the side-list
  * order of src/mesh.c:1532-1566, 1713-1755), element and side geometry
  * (createElemInfo / createSideInfo, src/mesh.c:250-341), ghost states'
- * geometry, least-squares weights (createReconstructionInfo,
- * src/mesh.c:80-114), the space-filling-curve order and the block halo
- * lists. Every array equals, bit for bit, the one ccfd_batch_add writes
- * for ccfd_mesh_build(m), and the instance behaves identically. Periodic
- * boundaries are not supported here (CCFD_ENOTSUP: use ccfd_batch_add).
- * The host only checks the input and resolves boundary codes.
+ * geometry, periodic hookup (connectPeriodicBC, src/mesh.c:172-244),
+ * least-squares weights (createReconstructionInfo, src/mesh.c:80-114), the
+ * space-filling-curve order and the block halo lists. Every array equals,
+ * bit for bit, the one ccfd_batch_add writes for ccfd_mesh_build(m), and
+ * the instance behaves identically. The host only checks the input and
+ * resolves boundary codes.
  */
 int ccfd_batch_add_raw(ccfd_batch *b, const ccfd_raw_mesh *m,
 		const ccfd_batch_case *cs, const double *prim, int *id);
-/* ccfd_batch_fits for a raw mesh (no periodic boundaries) */
+/* ccfd_batch_fits for a raw mesh */
 int ccfd_batch_fits_raw(ccfd_batch *b, const ccfd_raw_mesh *m, int *fits);
 
 /* retire an instance and free its arena space */

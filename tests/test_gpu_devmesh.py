@@ -6,8 +6,8 @@ must behave bit for bit as the instance admitted from the host builder's
 image (ccfd_mesh_build + build_soa, i.e. src/mesh.c's construction): the
 same time step, residuals and states after several steps, in FP64 and in
 FP32-flux arenas, for triangle, quadrilateral and mixed meshes, next to other
-instances. Malformed meshes are refused with the host builder's messages,
-and periodic meshes are left to the host path."""
+instances, periodic meshes included. Malformed meshes are refused with the
+host builder's messages."""
 import math
 
 import numpy as np
@@ -32,6 +32,10 @@ def _case(name):
     """(raw, bcs, max sides, local time steps)"""
     if name == "delaunay":
         raw, bcs = M.jittered_delaunay(40, 0.3, 7, periodic=False, bc_code=401)
+        return raw, bcs, 3, False
+    if name == "periodic":
+        # connectPeriodicBC on the device: both directions periodic
+        raw, bcs = M.jittered_delaunay(48, 0.3, 9, periodic=True)
         return raw, bcs, 3, False
     if name == "naca":
         raw = M.naca_ogrid(96, 48, wall_bc=101, far_bc=501)
@@ -66,7 +70,8 @@ def _state(mesh, seed):
 
 
 @pytest.mark.parametrize("prec", [G.FP64, G.FP32_FLUX], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("name", ["delaunay", "naca", "bump", "mixed", "large"])
+@pytest.mark.parametrize("name", ["delaunay", "periodic", "naca", "bump", "mixed",
+                                  "large"])
 def test_device_preprocessing_equals_the_host_image(prec, name):
     raw, bcs, sides, lts = _case(name)
     host = M.Mesh(raw, bcs)
@@ -122,11 +127,16 @@ def test_device_preprocessing_refusals():
     refused(M.Raw(raw.xy, quad=bad, bcedge=raw.bcedge), bcs, "references node")
     refused(M.Raw(raw.xy, quad=raw.quad, bcedge=raw.bcedge[1:]), bcs,
             "open mesh")
+    # a periodic pair whose connection vector misses its partner
     per, pbcs = M.jittered_delaunay(8, 0.3, 1, periodic=True)
+    pbcs = [M.BC(c.code, connection=(0.0, 1.5)) if c.code == 701 else c
+            for c in pbcs]
     b3 = BT.Batch(sch, max_sides=3, hbm_bytes=32 * MB)
-    with pytest.raises(BT.GpuError, match="periodic"):
+    with pytest.raises(BT.GpuError, match="no connection"):
         b3.add_raw(M.RawMesh(per, pbcs),
                    np.tile([1.0, 0.5, 0.0, 1.0], (len(per.tri), 1)))
+    with pytest.raises(ValueError, match="no connection"):
+        M.Mesh(per, pbcs)
     # the refusals left the arena as it was
     b.step(2)
     b.sync()
