@@ -540,6 +540,39 @@ def prep_run(sizes=(2e5, 2e6), kinds=("naca", "bump"), repeats=3):
     return out
 
 
+def arena_ops_run(cells=2e6, steps=40):
+    """One served-size request alone in an arena: ms per step (four stages,
+    local time steps, AUSMD + VK K=5) of each operator implementation, FP64
+    and FP32 flux, on a NACA O-grid and a channel bump. Host clock around
+    `steps` steps after 4 warm-up steps, synchronised on both sides."""
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+    from ccfd_amd import workloads as W
+    out = {}
+    for kind in ("naca", "bump"):
+        spec = {"cells": cells, "kind": kind, "mach": 0.7, "aoa": 2.0}
+        for prec, pname in ((G.FP64, "fp64"), (G.FP32_FLUX, "fp32_flux")):
+            sch = G.params(flux=9, order=2, limiter=2, venk_k=5.0, nrk=4,
+                           torder=2, precision=prec)
+            b = BT.Batch(sch, max_sides=4, local_dt=True, hbm_bytes=8 << 30)
+            mesh, prim, cfl = W.served_case(spec, raw=True)
+            b.add_raw(mesh, prim, cfl=cfl)
+            res = {}
+            for var, vn in ((G.KV_SPLIT, "split"), (G.KV_FUSED, "fused"),
+                            (G.KV_FUSED_FACE, "fused_face")):
+                b.set_variant(var)
+                b.step(4)
+                b.sync()
+                t0 = time.perf_counter()
+                b.step(steps)
+                b.sync()
+                res[vn] = 1e3 * (time.perf_counter() - t0) / steps
+            out["%s_%s" % (kind, pname)] = res
+            log("arena ops %s %s: %s" % (kind, pname, json.dumps(res)))
+            b.close()
+    return out
+
+
 def implicit_run(steps, cfl, ni=512, nj=256, venk_k=1000.0, chunk=500,
                  modes=("fp64", "fp32_flux"), save_dir=None):
     """Implicit Navier-Stokes around a NACA 2412 (m=0.02, p=0.4, t=0.12) on
@@ -644,7 +677,7 @@ def main():
                     help="also time a 1 GiB streaming copy (reachable HBM "
                          "bandwidth; counter calibration under rocprofv3)")
     ap.add_argument("--workload", choices=["operator", "batched", "served",
-                                           "implicit", "prep"],
+                                           "implicit", "prep", "arena-ops"],
                     default="operator",
                     help="operator: the 4M-triangle stage benchmark (plus a "
                          "short served sample unless --no-served); batched: "
@@ -727,6 +760,19 @@ def main():
                "config": {"workload": "512x256 O-grid NS Re 5000, to a 1e-8 "
                                       "drop or %d steps" % r["max_steps"]},
                "implicit": r}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "arena-ops":
+        r = arena_ops_run()
+        out = {"metric": "ms per step of one 2M-cell request in an arena",
+               "value": r["naca_fp32_flux"]["split"], "unit": "ms",
+               "n_gpus": 1, "steps": 40, "warmup": 4,
+               "higher_is_better": False, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64 / f32 flux",
+               "data": "synthetic (NACA O-grid and channel-bump requests)",
+               "config": {"workload": "one 2e6-cell served request, each "
+                                      "operator implementation"},
+               "arena_ops": r}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "prep":
