@@ -813,6 +813,49 @@ __global__ __launch_bounds__(128) void k_gs_cells(
   }
 }
 
+// The same update with four threads per element, one per row a of the
+// 4 x 4 system, for FP32 blocks: row a of b_i - sum B_ij x_j accumulates in
+// the order of k_gs_cells (neighbours in slot order, columns in order) and
+// the rows meet in LDS for the D_i^-1 product, so both kernels give the same
+// bits. With FP32 blocks the one-thread kernel waits on its gathers (13.0
+// against 8.9 us per colour on the 512 x 256 O-grid); with FP64 blocks the
+// four lanes' repeated index and neighbour loads cost more than they hide
+// (16.4 against 14.5 us), so FP64 keeps k_gs_cells.
+constexpr int kGsRowCells = 64;   // elements per block (256 threads)
+
+template <typename M, typename K>
+__global__ __launch_bounds__(4 * kGsRowCells) void k_gs_rows(
+    int n, int start, const int* __restrict__ cells, int N, int Np, int S,
+    const int* __restrict__ nbr, const M* __restrict__ Blk,
+    const M* __restrict__ Dinv, const K* __restrict__ b, K* __restrict__ x) {
+  __shared__ double rs[4 * kGsRowCells];
+  const int tid = threadIdx.x, a = tid & 3;
+  const int t = blockIdx.x * kGsRowCells + (tid >> 2);
+  const bool active = t < n;
+  int p = 0, i = 0;
+  double r = 0.0;
+  if (active) {
+    p = start + t;
+    i = cells[p];
+    r = double(b[a * Np + i]);
+    for (int s = 0; s < S; ++s) {
+      const int j = nbr[(size_t)s * N + p];
+      if (j < 0) continue;
+      for (int c = 0; c < 4; ++c)
+        r -= double(Blk[((size_t)(4 * a + c) * S + s) * N + p]) *
+             double(x[c * Np + j]);
+    }
+  }
+  rs[tid] = r;
+  __syncthreads();
+  if (active) {
+    const double* rr = rs + (tid & ~3);
+    double u = 0.0;
+    for (int c = 0; c < 4; ++c) u += double(Dinv[(size_t)(4 * a + c) * N + p]) * rr[c];
+    x[a * Np + i] = K(u);
+  }
+}
+
 // colour-ordered copies of the side blocks and inverted diagonal blocks
 template <typename M>
 __global__ void k_gs_order(int N, int Np, int S, const int* __restrict__ cells,
@@ -1499,10 +1542,16 @@ static void gs0(ccfd_gpu_ctx* c, int sweeps, bool zero, const K* b, K* x) {
   if (zero) hipMemsetAsync(x, 0, 4 * (size_t)Np * sizeof(K), c->st);
   auto update = [&](int k) {
     const int n = s->mcStart[k + 1] - s->mcStart[k];
-    hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
-                       c->st, n, s->mcStart[k], s->mcCells, c->N, Np, S,
-                       s->mcLink, (const M*)s->mcBlk, (const M*)s->mcDinv, b,
-                       x);
+    if (sizeof(M) == 4)
+      hipLaunchKernelGGL((k_gs_rows<M, K>), dim3((n + kGsRowCells - 1) / kGsRowCells),
+                         dim3(4 * kGsRowCells), 0, c->st, n, s->mcStart[k],
+                         s->mcCells, c->N, Np, S, s->mcLink, (const M*)s->mcBlk,
+                         (const M*)s->mcDinv, b, x);
+    else
+      hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
+                         c->st, n, s->mcStart[k], s->mcCells, c->N, Np, S,
+                         s->mcLink, (const M*)s->mcBlk, (const M*)s->mcDinv, b,
+                         x);
   };
   // a colour updated twice in a row gives the same values: skip repeats
   int last = -1;
