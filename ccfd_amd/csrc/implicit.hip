@@ -820,39 +820,49 @@ __global__ __launch_bounds__(128) void k_gs_cells(
 // bits. With FP32 blocks the one-thread kernel waits on its gathers (13.0
 // against 8.9 us per colour on the 512 x 256 O-grid); with FP64 blocks the
 // four lanes' repeated index and neighbour loads cost more than they hide
-// (16.4 against 14.5 us), so FP64 keeps k_gs_cells.
-constexpr int kGsRowCells = 64;   // elements per block (256 threads)
-
-template <typename M, typename K>
-__global__ __launch_bounds__(4 * kGsRowCells) void k_gs_rows(
+// (16.4 against 14.5 us; two lanes of two rows each: 14.7 us, the blocks'
+// 512 B per element bound it), so FP64 keeps k_gs_cells.
+//
+// L lanes per element (2 or 4), each with 4 / L rows; 256 threads
+template <typename M, typename K, int L>
+__global__ __launch_bounds__(256) void k_gs_rows(
     int n, int start, const int* __restrict__ cells, int N, int Np, int S,
     const int* __restrict__ nbr, const M* __restrict__ Blk,
     const M* __restrict__ Dinv, const K* __restrict__ b, K* __restrict__ x) {
-  __shared__ double rs[4 * kGsRowCells];
-  const int tid = threadIdx.x, a = tid & 3;
-  const int t = blockIdx.x * kGsRowCells + (tid >> 2);
+  constexpr int R = 4 / L, C = 256 / L;   // rows per lane, elements per block
+  __shared__ double rs[4 * C];
+  const int tid = threadIdx.x, lane = tid % L, e = tid / L;
+  const int t = blockIdx.x * C + e;
   const bool active = t < n;
   int p = 0, i = 0;
-  double r = 0.0;
+  double r[R];
+  for (int q = 0; q < R; ++q) r[q] = 0.0;
   if (active) {
     p = start + t;
     i = cells[p];
-    r = double(b[a * Np + i]);
+    for (int q = 0; q < R; ++q) r[q] = double(b[(lane * R + q) * Np + i]);
     for (int s = 0; s < S; ++s) {
       const int j = nbr[(size_t)s * N + p];
       if (j < 0) continue;
-      for (int c = 0; c < 4; ++c)
-        r -= double(Blk[((size_t)(4 * a + c) * S + s) * N + p]) *
-             double(x[c * Np + j]);
+      double xj[4];
+      for (int c = 0; c < 4; ++c) xj[c] = double(x[c * Np + j]);
+      for (int q = 0; q < R; ++q) {
+        const int a = lane * R + q;
+        for (int c = 0; c < 4; ++c)
+          r[q] -= double(Blk[((size_t)(4 * a + c) * S + s) * N + p]) * xj[c];
+      }
     }
   }
-  rs[tid] = r;
+  for (int q = 0; q < R; ++q) rs[4 * e + lane * R + q] = r[q];
   __syncthreads();
   if (active) {
-    const double* rr = rs + (tid & ~3);
-    double u = 0.0;
-    for (int c = 0; c < 4; ++c) u += double(Dinv[(size_t)(4 * a + c) * N + p]) * rr[c];
-    x[a * Np + i] = K(u);
+    const double* rr = rs + 4 * e;
+    for (int q = 0; q < R; ++q) {
+      const int a = lane * R + q;
+      double u = 0.0;
+      for (int c = 0; c < 4; ++c) u += double(Dinv[(size_t)(4 * a + c) * N + p]) * rr[c];
+      x[a * Np + i] = K(u);
+    }
   }
 }
 
@@ -1543,10 +1553,9 @@ static void gs0(ccfd_gpu_ctx* c, int sweeps, bool zero, const K* b, K* x) {
   auto update = [&](int k) {
     const int n = s->mcStart[k + 1] - s->mcStart[k];
     if (sizeof(M) == 4)
-      hipLaunchKernelGGL((k_gs_rows<M, K>), dim3((n + kGsRowCells - 1) / kGsRowCells),
-                         dim3(4 * kGsRowCells), 0, c->st, n, s->mcStart[k],
-                         s->mcCells, c->N, Np, S, s->mcLink, (const M*)s->mcBlk,
-                         (const M*)s->mcDinv, b, x);
+      hipLaunchKernelGGL((k_gs_rows<M, K, 4>), dim3((n + 63) / 64), dim3(256), 0,
+                         c->st, n, s->mcStart[k], s->mcCells, c->N, Np, S,
+                         s->mcLink, (const M*)s->mcBlk, (const M*)s->mcDinv, b, x);
     else
       hipLaunchKernelGGL((k_gs_cells<M, K>), dim3((n + 127) / 128), dim3(128), 0,
                          c->st, n, s->mcStart[k], s->mcCells, c->N, Np, S,
