@@ -402,6 +402,67 @@ def served_run(n_req, cells, rate, max_steps, hbm_gb, seed=5, baseline=True,
     return out
 
 
+def served_convergence_run(n_req, cells, venk_ks, chunks, chunk, hbm_gb,
+                           seed=5):
+    """Residual history of the served requests without the serving loop:
+    every request in one FP64 arena (the served scheme: AUSMD, VK, RK4 local
+    steps at CFL 2, free-stream start), stepped in chunks for each
+    Venkatakrishnan constant; per request the density residual drop after
+    every chunk and the first chunk end at or below 1e-6."""
+    from ccfd_amd import batch as BT
+    from ccfd_amd import gpu as G
+    from ccfd_amd import workloads as W
+    specs = W.served_specs(n_req, seed=seed, cells=cells)
+    out = {"requests": n_req, "cells_range": list(cells), "chunk": chunk,
+           "steps": chunks * chunk, "target_drop": 1e-6,
+           "scheme": "AUSMD, VK, RK4 local time steps at CFL 2, FP64, "
+                     "meshes preprocessed on the device at admission",
+           "runs": {}}
+    for K in venk_ks:
+        sch = G.params(flux=9, order=2, limiter=2, venk_k=K, nrk=4, torder=2,
+                       precision=G.FP64)
+        b = BT.Batch(sch, max_sides=4, local_dt=True, hbm_bytes=hbm_gb << 30)
+        ids = {}
+        for sp in specs:
+            m, p, cfl = W.served_case(sp, raw=True)
+            ids[b.add_raw(m, p, cfl=cfl)] = sp
+        variant, _ = b.autotune(4)
+        hist = {k: [] for k in ids}
+        nonfinite = set()
+        t0 = time.perf_counter()
+        for c in range(chunks):
+            b.step(chunk)
+            b.sync()
+            info = b.poll()
+            for k in ids:
+                i = info[k]
+                hist[k].append(float(i["res"][0] / i["res0"][0])
+                               if i["res0"][0] > 0 else float("nan"))
+                if i["nonfinite"]:
+                    nonfinite.add(k)
+            log("K=%g: %d steps, %.1f s" % (K, (c + 1) * chunk,
+                                             time.perf_counter() - t0))
+        el = time.perf_counter() - t0
+        reqs = []
+        for k in sorted(ids, key=lambda k: ids[k]["id"]):
+            sp, h = ids[k], hist[k]
+            first = next((j for j, d in enumerate(h) if d <= 1e-6), None)
+            reqs.append({"id": sp["id"], "kind": sp["kind"],
+                         "cells": sp["cells"], "mach": round(sp["mach"], 3),
+                         "drop": [float("%.3g" % d) for d in h],
+                         "min_drop": float("%.3g" % min(h)),
+                         "steps_to_target": (None if first is None
+                                             else (first + 1) * chunk),
+                         "nonfinite": k in nonfinite})
+        out["runs"]["%g" % K] = {
+            "variant": {1: "split", 2: "fused", 3: "fused_face"}[variant],
+            "seconds": el, "reached_target": sum(
+                r["steps_to_target"] is not None for r in reqs),
+            "requests": reqs}
+        b.close()
+    return out
+
+
 def batched_run(n_inst, cells, steps, hbm_gb, baseline=True):
     """Batched explicit Euler: n_inst seeded jittered Delaunay meshes (cells
     log-uniform in the range), 4-quadrant Riemann states, HLLC + BJ, RK3
@@ -677,11 +738,18 @@ def main():
                     help="also time a 1 GiB streaming copy (reachable HBM "
                          "bandwidth; counter calibration under rocprofv3)")
     ap.add_argument("--workload", choices=["operator", "batched", "served",
-                                           "implicit", "prep", "arena-ops"],
+                                           "implicit", "prep", "arena-ops",
+                                           "served-convergence"],
                     default="operator",
                     help="operator: the 4M-triangle stage benchmark (plus a "
                          "short served sample unless --no-served); batched: "
-                         "many meshes in one arena; served: the request loop")
+                         "many meshes in one arena; served: the request "
+                         "loop; served-convergence: residual histories of "
+                         "the served requests")
+    ap.add_argument("--venk-list", default="5,100",
+                    help="served-convergence: Venkatakrishnan constants")
+    ap.add_argument("--chunk", type=int, default=1000,
+                    help="served-convergence: steps between residual reads")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flux-variants", action="store_true",
@@ -786,6 +854,27 @@ def main():
                "config": {"workload": "one served request's mesh set-up and "
                                       "admission, 2e5 and 2e6 cells"},
                "prep": r}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "served-convergence":
+        cells = (args.min_cells or 1e4, args.max_cells or 2e6)
+        ks = [float(k) for k in args.venk_list.split(",")]
+        r = served_convergence_run(args.served_requests, cells, ks,
+                                   max(1, args.max_steps // args.chunk),
+                                   args.chunk, args.hbm_gb)
+        first = r["runs"]["%g" % ks[0]]
+        out = {"metric": "served requests reaching a 1e-6 density residual "
+                         "drop (explicit local steps, one FP64 arena)",
+               "value": first["reached_target"], "unit": "requests",
+               "n_gpus": 1, "steps": r["steps"], "warmup": 0,
+               "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic (channel-with-bump and NACA O-grid "
+                       "requests)",
+               "config": {"workload": "%d requests, cells %g..%g, %d steps"
+                          % (args.served_requests, cells[0], cells[1],
+                             r["steps"])},
+               "served_convergence": r}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "served":
