@@ -29,6 +29,7 @@
 #define CG_BAR_2 3
 #define CG_TRI_3 5
 #define CG_QUAD_4 7
+#define CG_MIXED 20
 
 static void seterr(char *err, const char *fmt, ...)
 {
@@ -741,5 +742,354 @@ done:
 		H5Gclose(base);
 	H5Fclose(f);
 	free(a);
+	return rc;
+}
+
+/* ------------------------------------------------------------ mesh input */
+
+/* a fixed-length string attribute; 0 when read */
+static int attr_str(hid_t g, const char *name, char *out, size_t size)
+{
+	out[0] = 0;
+	if (H5Aexists(g, name) <= 0)
+		return -1;
+	hid_t a = H5Aopen(g, name, H5P_DEFAULT);
+	if (a < 0)
+		return -1;
+	hid_t t = H5Tcopy(H5T_C_S1);
+	H5Tset_size(t, size - 1);
+	char buf[NAMELEN + 1];
+	memset(buf, 0, sizeof(buf));
+	int rc = size <= sizeof(buf) && H5Aread(a, t, buf) >= 0 ? 0 : -1;
+	if (!rc)
+		snprintf(out, size, "%s", buf);
+	H5Tclose(t);
+	H5Aclose(a);
+	return rc;
+}
+
+/* open a child node; a link node (type LK) is followed to its target */
+static hid_t open_node(hid_t parent, const char *name)
+{
+	hid_t g = H5Gopen2(parent, name, H5P_DEFAULT);
+	if (g < 0)
+		return -1;
+	char type[4];
+	if (attr_str(g, "type", type, sizeof(type)) == 0 && !strcmp(type, "LK")) {
+		hid_t t = H5Gopen2(g, " link", H5P_DEFAULT);
+		H5Gclose(g);
+		return t;
+	}
+	return g;
+}
+
+#define MAXKIDS 256
+typedef struct {
+	int n;
+	char name[MAXKIDS][NAMELEN + 1];
+} kids_t;
+
+/* the children with a label, in creation order where the file tracks it
+ * (else by name, as the CGNS library lists them); 0 when listed */
+static int kids(hid_t parent, const char *label, kids_t *out)
+{
+	H5G_info_t info;
+	out->n = 0;
+	if (H5Gget_info(parent, &info) < 0)
+		return -1;
+	H5_index_t idx = H5_INDEX_CRT_ORDER;
+	char nm[256];
+	if (info.nlinks > 0 && H5Lget_name_by_idx(parent, ".", idx, H5_ITER_INC, 0,
+			nm, sizeof(nm), H5P_DEFAULT) < 0)
+		idx = H5_INDEX_NAME;
+	for (hsize_t k = 0; k < info.nlinks; ++k) {
+		if (H5Lget_name_by_idx(parent, ".", idx, H5_ITER_INC, k, nm, sizeof(nm),
+				H5P_DEFAULT) < 0)
+			return -1;
+		if (nm[0] == ' ' || strlen(nm) > NAMELEN)
+			continue;
+		hid_t g = open_node(parent, nm);
+		if (g < 0)
+			continue;
+		char lb[NAMELEN + 1];
+		int match = attr_str(g, "label", lb, sizeof(lb)) == 0 && !strcmp(lb, label);
+		H5Gclose(g);
+		if (!match)
+			continue;
+		if (out->n == MAXKIDS)
+			return -1;
+		strcpy(out->name[out->n++], nm);
+	}
+	return 0;
+}
+
+/* number of values of a node's data and its HDF5 dimensions (reversed
+ * from the CGNS order); -1 without data */
+static long data_dims(hid_t g, int *nd, hsize_t dims[8])
+{
+	hid_t d = H5Dopen2(g, " data", H5P_DEFAULT);
+	if (d < 0)
+		return -1;
+	hid_t s = H5Dget_space(d);
+	int n = H5Sget_simple_extent_ndims(s);
+	long count = -1;
+	if (n >= 0 && n <= 8) {
+		H5Sget_simple_extent_dims(s, dims, NULL);
+		count = (long)H5Sget_simple_extent_npoints(s);
+		*nd = n;
+	}
+	H5Sclose(s);
+	H5Dclose(d);
+	return count;
+}
+
+/* a node's data converted to the memory type; count checked */
+static int data_read(hid_t g, hid_t mtype, void *out, long count)
+{
+	int nd;
+	hsize_t dims[8];
+	if (data_dims(g, &nd, dims) != count)
+		return -1;
+	hid_t d = H5Dopen2(g, " data", H5P_DEFAULT);
+	int rc = H5Dread(d, mtype, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) >= 0 ? 0 : -1;
+	H5Dclose(d);
+	return rc;
+}
+
+/* the codes of the zone's BC_t nodes on consecutive boundary edges, as
+ * many per node as its point set counts (cg_boco_info's npnts) */
+static int bc_codes(hid_t z, long *bce, long nb, const char *path, char *err,
+		kids_t *k)
+{
+	if (kids(z, "ZoneBC_t", k)) {
+		seterr(err, "%s: unreadable zone", path);
+		return -1;
+	}
+	if (k->n == 0)
+		return 0;
+	hid_t zbc = open_node(z, k->name[0]);
+	if (zbc < 0 || kids(zbc, "BC_t", k)) {
+		if (zbc >= 0)
+			H5Gclose(zbc);
+		seterr(err, "%s: unreadable ZoneBC", path);
+		return -1;
+	}
+	static const char *sets[4] = {"PointList", "PointRange", "ElementList",
+		"ElementRange"};
+	long e = 0;
+	int rc = 0;
+	for (int c = 0; c < k->n && !rc; ++c) {
+		int code;
+		if (sscanf(k->name[c], "%d", &code) != 1) {
+			seterr(err, "%s: boundary condition %s is not a boundary code",
+					path, k->name[c]);
+			rc = -1;
+			break;
+		}
+		hid_t b = open_node(zbc, k->name[c]);
+		long np = 0;
+		for (int q = 0; q < 4 && b >= 0; ++q) {
+			if (H5Lexists(b, sets[q], H5P_DEFAULT) <= 0)
+				continue;
+			hid_t ps = open_node(b, sets[q]);
+			int nd = 0;
+			hsize_t dims[8];
+			long cnt = ps >= 0 ? data_dims(ps, &nd, dims) : -1;
+			if (ps >= 0)
+				H5Gclose(ps);
+			if (cnt < 0 || nd != 2) {
+				seterr(err, "%s: invalid point set of %s", path, k->name[c]);
+				rc = -1;
+			}
+			np = rc ? 0 : (long)dims[0];
+			break;
+		}
+		if (b >= 0)
+			H5Gclose(b);
+		if (b < 0) {
+			seterr(err, "%s: unreadable boundary condition %s", path, k->name[c]);
+			rc = -1;
+		}
+		for (long p = 0; p < np && !rc; ++p) {
+			if (e >= nb) {
+				seterr(err, "%s: more boundary condition points than boundary "
+						"edges (%ld)", path, nb);
+				rc = -1;
+				break;
+			}
+			bce[3 * e++ + 2] = code;
+		}
+	}
+	H5Gclose(zbc);
+	return rc;
+}
+
+int ccfd_cgns_read_mesh(const char *path, long *nVert, double **xy,
+		long *nTri, long **tri, long *nQuad, long **quad,
+		long *nBCedge, long **bcEdge, char *err)
+{
+	quiet();
+	if (!path || !nVert || !xy || !nTri || !tri || !nQuad || !quad ||
+			!nBCedge || !bcEdge) {
+		seterr(err, "null argument");
+		return -1;
+	}
+	*nVert = *nTri = *nQuad = *nBCedge = 0;
+	*xy = NULL;
+	*tri = *quad = *bcEdge = NULL;
+	hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+	if (f < 0) {
+		seterr(err, "cannot open %s", path);
+		return -1;
+	}
+	int rc = -1;
+	hid_t base = -1, z = -1, gc = -1;
+	long *conn = NULL;
+	double *xs = NULL;
+	kids_t *k = malloc(sizeof(kids_t));
+	kids_t *ks = malloc(sizeof(kids_t));
+	long nv = 0, nt = 0, nq = 0, nb = 0;
+	if (!k || !ks) {
+		seterr(err, "out of memory");
+		goto done;
+	}
+	if (kids(f, "CGNSBase_t", k) || k->n == 0 ||
+			(base = open_node(f, k->name[0])) < 0) {
+		seterr(err, "%s: no CGNS base", path);
+		goto done;
+	}
+	if (kids(base, "Zone_t", k) || k->n == 0) {
+		seterr(err, "%s: no zone", path);
+		goto done;
+	}
+	int zi = 0;
+	for (int i = 1; i < k->n; ++i)
+		if (strcmp(k->name[i], k->name[zi]) < 0)
+			zi = i;
+	long size[3];
+	if ((z = open_node(base, k->name[zi])) < 0 ||
+			data_read(z, H5T_NATIVE_LONG, size, 3)) {
+		seterr(err, "%s: zone %s is not an unstructured zone", path, k->name[zi]);
+		goto done;
+	}
+	nv = size[0];
+	if (nv <= 0 || nv > (1L << 31)) {
+		seterr(err, "%s: %ld nodes", path, nv);
+		goto done;
+	}
+	xs = malloc(sizeof(double) * 2 * (size_t)nv);
+	*xy = malloc(sizeof(double) * 2 * (size_t)nv);
+	if (!xs || !*xy) {
+		seterr(err, "out of memory");
+		goto done;
+	}
+	if ((gc = open_node(z, "GridCoordinates")) < 0 ||
+			read_data(gc, "CoordinateX", H5T_NATIVE_DOUBLE, xs, nv) ||
+			read_data(gc, "CoordinateY", H5T_NATIVE_DOUBLE, xs + nv, nv)) {
+		seterr(err, "%s: cannot read the coordinates of %ld nodes", path, nv);
+		goto done;
+	}
+	for (long i = 0; i < nv; ++i) {
+		(*xy)[2 * i] = xs[i];
+		(*xy)[2 * i + 1] = xs[nv + i];
+	}
+	if (kids(z, "Elements_t", ks)) {
+		seterr(err, "%s: unreadable sections", path);
+		goto done;
+	}
+	for (int s = 0; s < ks->n; ++s) {
+		hid_t g = open_node(z, ks->name[s]);
+		int hdr[2];
+		hid_t ec = -1;
+		int nd;
+		hsize_t dims[8];
+		long cnt = -1;
+		if (g >= 0 && data_read(g, H5T_NATIVE_INT, hdr, 2) == 0 &&
+				(ec = open_node(g, "ElementConnectivity")) >= 0)
+			cnt = data_dims(ec, &nd, dims);
+		free(conn);
+		conn = cnt > 0 ? malloc(sizeof(long) * (size_t)cnt) : NULL;
+		int ok = cnt >= 0 && (cnt == 0 || (conn &&
+				data_read(ec, H5T_NATIVE_LONG, conn, cnt) == 0));
+		if (ec >= 0)
+			H5Gclose(ec);
+		if (g >= 0)
+			H5Gclose(g);
+		if (!ok) {
+			seterr(err, "%s: unreadable section %s", path, ks->name[s]);
+			goto done;
+		}
+		int npe = hdr[0] == CG_BAR_2 ? 2 : hdr[0] == CG_TRI_3 ? 3 :
+			hdr[0] == CG_QUAD_4 ? 4 : 0;
+		if (hdr[0] == CG_MIXED) {
+			seterr(err, "%s: section %s is MIXED; the CGNS library reads "
+					"only fixed-size element sections", path, ks->name[s]);
+			goto done;
+		}
+		if (!npe) {
+			seterr(err, "Only TRI_3 and QUAD_4 Elements allowed");
+			goto done;
+		}
+		long n = cnt / npe, w = npe == 2 ? 3 : npe + 1;
+		long *out = calloc((size_t)(n ? n : 1) * (size_t)w, sizeof(long));
+		if (!out) {
+			seterr(err, "out of memory");
+			goto done;
+		}
+		for (long i = 0; i < n; ++i) {
+			for (int j = 0; j < npe; ++j) {
+				long v = conn[npe * i + j] - 1;
+				if (v < 0 || v >= nv) {
+					seterr(err, "%s: section %s: node %ld out of range",
+							path, ks->name[s], v + 1);
+					free(out);
+					goto done;
+				}
+				out[w * i + j] = v;
+			}
+			if (npe > 2)
+				out[w * i + npe] = 1;
+		}
+		if (npe == 2) {
+			free(*bcEdge);
+			*bcEdge = out;
+			nb = n;
+			if (bc_codes(z, out, n, path, err, k))
+				goto done;
+		} else if (npe == 3) {
+			free(*tri);
+			*tri = out;
+			nt = n;
+		} else {
+			free(*quad);
+			*quad = out;
+			nq = n;
+		}
+	}
+	*nVert = nv;
+	*nTri = nt;
+	*nQuad = nq;
+	*nBCedge = nb;
+	rc = 0;
+done:
+	if (rc) {
+		free(*xy);
+		free(*tri);
+		free(*quad);
+		free(*bcEdge);
+		*xy = NULL;
+		*tri = *quad = *bcEdge = NULL;
+	}
+	if (gc >= 0)
+		H5Gclose(gc);
+	if (z >= 0)
+		H5Gclose(z);
+	if (base >= 0)
+		H5Gclose(base);
+	H5Fclose(f);
+	free(conn);
+	free(xs);
+	free(k);
+	free(ks);
 	return rc;
 }

@@ -83,6 +83,8 @@ static struct {
 	int (*write_master)(const char *, const char *, const ccfd_mesh *, int,
 			const double *, const long *, const char *const *, char *);
 	int (*read_solution)(const char *, long, double *, double *, double *, char *);
+	int (*read_mesh)(const char *, long *, double **, long *, long **, long *,
+			long **, long *, long **, char *);
 } CG;
 
 static int cgns_load(char *e)
@@ -110,8 +112,9 @@ static int cgns_load(char *e)
 	*(void **)&CG.write_solution = dlsym(h, "ccfd_cgns_write_solution");
 	*(void **)&CG.write_master = dlsym(h, "ccfd_cgns_write_master");
 	*(void **)&CG.read_solution = dlsym(h, "ccfd_cgns_read_solution");
+	*(void **)&CG.read_mesh = dlsym(h, "ccfd_cgns_read_mesh");
 	if (!CG.write_mesh || !CG.write_solution || !CG.write_master ||
-			!CG.read_solution) {
+			!CG.read_solution || !CG.read_mesh) {
 		snprintf(e, 256, "libccfd_cgns.so lacks the CGNS entry points");
 		dlclose(h);
 		return -1;
@@ -381,12 +384,13 @@ int ccfd_init_restart(const char *iniFile, const char *restartFile, int device,
 		char fmt[64], file[256], path[600];
 		NEED(ccfd_ini_str(ini, "meshFormat", NULL, fmt, sizeof(fmt)), "meshFormat");
 		NEED(ccfd_ini_str(ini, "meshFile", NULL, file, sizeof(file)), "meshFile");
-		/* src/mesh.c:1419-1440: .msh / .msh2 / .msh4 gmsh, .mesh EMC2;
-		 * CGNS (.cgns) is not read on this path */
+		/* src/mesh.c:1419-1440: .msh / .msh2 / .msh4 gmsh, .mesh EMC2,
+		 * .cgns CGNS (through libccfd_cgns.so) */
 		int gmsh = !strcmp(fmt, ".msh") || !strcmp(fmt, ".msh2") ||
 			!strcmp(fmt, ".msh4");
-		if (!gmsh && strcmp(fmt, ".mesh") != 0) {
-			snprintf(e, 256, "mesh format %s not supported (gmsh or EMC2)", fmt);
+		int cgns = !strcmp(fmt, ".cgns");
+		if (!gmsh && !cgns && strcmp(fmt, ".mesh") != 0) {
+			snprintf(e, 256, "Unknown Mesh Format %s", fmt);
 			goto fail;
 		}
 		/* relative to the ini file's directory */
@@ -396,9 +400,14 @@ int ccfd_init_restart(const char *iniFile, const char *restartFile, int device,
 					iniFile, file, fmt);
 		else
 			snprintf(path, sizeof(path), "%s%s", file, fmt);
-		if ((gmsh ? ccfd_gmsh_read : ccfd_emc2_read)(path, &nv, &xy, &nt,
-				&tri, &nq, &quad, &nb, &bce, e))
+		if (cgns) {
+			if (cgns_load(e) || CG.read_mesh(path, &nv, &xy, &nt, &tri,
+					&nq, &quad, &nb, &bce, e))
+				goto fail;
+		} else if ((gmsh ? ccfd_gmsh_read : ccfd_emc2_read)(path, &nv, &xy,
+				&nt, &tri, &nq, &quad, &nb, &bce, e)) {
 			goto fail;
+		}
 	}
 	C.mesh = ccfd_mesh_build(nv, xy, nt, tri, nq, quad, nb, bce, (int)nBC, bcs, e);
 	if (!C.mesh)

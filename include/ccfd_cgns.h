@@ -64,6 +64,25 @@ int ccfd_cgns_write_master(const char *path, const char *gridFile,
 int ccfd_cgns_read_solution(const char *path, long nElems, double *prim,
 		double *time, double *timeOverall, char *err);
 
+/*
+ * Read an unstructured 2-D mesh (readCGNS, src/mesh.c:1153-1394), with the
+ * outputs of ccfd_gmsh_read (ccfd_mesh.h; arrays malloc'ed, the caller
+ * frees them). The first base and, of its zones, the first by name (the
+ * CGNS library sorts zones); CoordinateX / CoordinateY of the zone's
+ * "GridCoordinates"; sections in file order: the last BAR_2 section gives
+ * the boundary edges, the last TRI_3 and QUAD_4 sections the cells
+ * (domain 1). The ZoneBC_t's BC_t nodes, in file order, hand their names,
+ * read as integer codes, to consecutive boundary edges, as many to each as
+ * its point set counts (a list its length, a range 2), whatever the set
+ * holds; edges left over keep code 0. Refused with a message: MIXED
+ * sections (the CGNS 4.1 library the reference vendors reads only
+ * fixed-size sections), other element types, a BC name that is not a
+ * number, more BC points than boundary edges, node ids out of range.
+ */
+int ccfd_cgns_read_mesh(const char *path, long *nVert, double **xy,
+		long *nTri, long **tri, long *nQuad, long **quad,
+		long *nBCedge, long **bcEdge, char *err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,8 @@ LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                    "ccfd_amd", "lib", "libccfd_cgns.so")
 TREE_SRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native",
                         "cgns_tree.c")
+EDIT_SRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native",
+                        "cgns_edit.c")
 
 
 @pytest.fixture(scope="module")
@@ -43,6 +45,10 @@ def cg():
     L.ccfd_cgns_read_solution.argtypes = [cp, C.c_long, dp,
                                           C.POINTER(C.c_double),
                                           C.POINTER(C.c_double), cp]
+    PL, PD = C.POINTER(C.c_long), C.POINTER(C.POINTER(C.c_double))
+    PPL = C.POINTER(C.POINTER(C.c_long))
+    L.ccfd_cgns_read_mesh.argtypes = [cp, PL, PD, PL, PPL, PL, PPL, PL, PPL,
+                                      cp]
     return L, pre
 
 
@@ -70,6 +76,46 @@ def tree(cg, tmp_path_factory):
             order.append(f[0])
         return nodes, order
     return dump
+
+
+@pytest.fixture(scope="module")
+def edit(cg, tmp_path_factory):
+    _, pre = cg
+    exe = str(tmp_path_factory.mktemp("edit") / "cgns_edit")
+    r = subprocess.run(["gcc", "-std=c99", "-O1", "-I" + os.path.join(pre, "include"),
+                        EDIT_SRC, "-o", exe, "-L" + os.path.join(pre, "lib"),
+                        "-lhdf5", "-Wl,-rpath," + os.path.join(pre, "lib")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+    def run(*args):
+        subprocess.run([exe] + [str(a) for a in args], check=True)
+    return run
+
+
+def _read_mesh(L, path):
+    """ccfd_cgns_read_mesh: (Raw, None) or (None, message)"""
+    n = [C.c_long() for _ in range(4)]
+    xy = C.POINTER(C.c_double)()
+    arrs = [C.POINTER(C.c_long)() for _ in range(3)]
+    err = C.create_string_buffer(256)
+    rc = L.ccfd_cgns_read_mesh(str(path).encode(), C.byref(n[0]), C.byref(xy),
+                               C.byref(n[1]), C.byref(arrs[0]), C.byref(n[2]),
+                               C.byref(arrs[1]), C.byref(n[3]),
+                               C.byref(arrs[2]), err)
+    if rc:
+        return None, err.value.decode()
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+
+    def take(p, count):
+        a = (np.ctypeslib.as_array(p, shape=(count,)).copy() if count
+             else np.zeros(0))
+        libc.free(C.cast(p, C.c_void_p))
+        return a
+    return M.Raw(take(xy, 2 * n[0].value), take(arrs[0], 4 * n[1].value),
+                 take(arrs[1], 5 * n[2].value),
+                 take(arrs[2], 3 * n[3].value)), None
 
 
 def _mesh():
@@ -299,3 +345,81 @@ def test_null_arguments_are_refused(cg):
     t = C.c_double()
     assert L.ccfd_cgns_read_solution(b"x.cgns", 0, None, C.byref(t), C.byref(t),
                                      err) == -1
+
+
+def _written(L, tmp_path, name="m.cgns"):
+    raw, bcs, m = _mesh()
+    path = tmp_path / name
+    err = C.create_string_buffer(256)
+    assert L.ccfd_cgns_write_mesh(str(path).encode(), C.cast(m._p, C.c_void_p),
+                                  len(raw.bcedge), N.lptr(raw.bcedge),
+                                  err) == 0, err.value
+    return raw, bcs, m, path
+
+
+def test_mesh_file_round_trip(cg, tmp_path):
+    """readCGNS (src/mesh.c:1153-1394) of the grid file cgnsWriteMesh
+    writes: the nodes, the cells in element order (domain 1), and the
+    boundary edges of the Boundaries section with the codes of the BC
+    nodes in order; the mesh built from it equals the original"""
+    L, _ = cg
+    raw, bcs, m, path = _written(L, tmp_path)
+    got, msg = _read_mesh(L, path)
+    assert msg is None, msg
+    assert np.array_equal(got.xy, raw.xy)
+    assert np.array_equal(got.tri, raw.tri)
+    assert np.array_equal(got.quad, raw.quad)
+    bars, part = _expected_bars(raw, m)
+    want = []
+    for c, bc in enumerate(bcs):
+        for i in range(part[c], part[c + 1]):
+            want.append([bars[2 * i] - 1, bars[2 * i + 1] - 1, bc.code])
+    assert got.bcedge.tolist() == want
+    m2 = M.Mesh(got, bcs)
+    assert m2.n_elems == m.n_elems and m2.n_ghosts == m.n_ghosts
+    for a in ("elem_node", "bary", "area", "sx", "sy", "slot_start"):
+        assert np.array_equal(getattr(m2, a), getattr(m, a)), a
+
+
+def test_mesh_reader_follows_the_library_and_the_reference(cg, edit, tmp_path):
+    """the reference's reading, with the CGNS 4.1 library it vendors: BC
+    codes go to consecutive boundary edges, as many per BC as its point set
+    counts (a range counts 2), whatever the set holds; zone 1 is the first
+    zone by name; MIXED sections are refused by the library, other element
+    types by the reference; a BC name must be a code"""
+    L, _ = cg
+    raw, bcs, m, path = _written(L, tmp_path)
+    n_side = {bc.code: int((raw.bcedge[:, 2] == bc.code).sum()) for bc in bcs}
+    # PointRange on the first BC (401, six edges)
+    p = tmp_path / "range.cgns"
+    p.write_bytes(path.read_bytes())
+    edit("range", p, "/Base/Zone/ZoneBC/401", 41, 46)
+    got, msg = _read_mesh(L, p)
+    assert msg is None, msg
+    codes = [401] * 2 + [101] * n_side[101] + [301] * n_side[301]
+    codes += [0] * (len(got.bcedge) - len(codes))
+    assert got.bcedge[:, 2].tolist() == codes
+    # a second zone that sorts first, without the quadrilaterals
+    p = tmp_path / "zones.cgns"
+    p.write_bytes(path.read_bytes())
+    edit("copy", p, "/Base/Zone", "/Base/Abc")
+    edit("delete", p, "/Base/Abc/Quadrilaterals")
+    got, msg = _read_mesh(L, p)
+    assert msg is None, msg
+    assert len(got.quad) == 0 and np.array_equal(got.tri, raw.tri)
+    # refusals
+    for name, args, text in [
+            ("mixed", ("settype", "/Base/Zone/Triangles", 20), "MIXED"),
+            ("tri6", ("settype", "/Base/Zone/Triangles", 6),
+             "Only TRI_3 and QUAD_4 Elements allowed"),
+            ("name", ("rename", "/Base/Zone/ZoneBC/101", "wall"),
+             "boundary condition wall is not a boundary code"),
+            ("extra", ("copy", "/Base/Zone/ZoneBC/401", "/Base/Zone/ZoneBC/402"),
+             "more boundary condition points than boundary edges")]:
+        p = tmp_path / (name + ".cgns")
+        p.write_bytes(path.read_bytes())
+        edit(args[0], p, *args[1:])
+        got, msg = _read_mesh(L, p)
+        assert got is None and text in msg, (name, msg)
+    got, msg = _read_mesh(L, tmp_path / "missing.cgns")
+    assert got is None and "cannot open" in msg

@@ -118,7 +118,8 @@ def test_unsupported_settings_are_refused(tmp_path, monkeypatch):
     message, not a wrong run: a source term without its function (the
     reference's mandatory sourceFunction, src/equation.c:121), an
     exact-function initial condition the reference does not know, a gmsh
-    file of another format, and a CGNS mesh"""
+    file of another format, an unknown mesh format and a missing CGNS mesh
+    file"""
     L = _lib()
     monkeypatch.chdir(tmp_path)
     base = open(FILES[0]).read()
@@ -135,7 +136,37 @@ def test_unsupported_settings_are_refused(tmp_path, monkeypatch):
         lines + ["meshType = 2", "meshFormat = .msh", "meshFile = m3"]) + "\n")
     msg = _init(L, str(tmp_path / "m3.ini"))
     assert "format 3 is not supported" in msg, msg
+    (tmp_path / "xy.ini").write_text("\n".join(
+        lines + ["meshType = 2", "meshFormat = .xyz", "meshFile = m3"]) + "\n")
+    msg = _init(L, str(tmp_path / "xy.ini"))
+    assert "Unknown Mesh Format" in msg, msg
     (tmp_path / "cg.ini").write_text("\n".join(
         lines + ["meshType = 2", "meshFormat = .cgns", "meshFile = m3"]) + "\n")
     msg = _init(L, str(tmp_path / "cg.ini"))
-    assert "not supported" in msg, msg
+    assert "cannot open" in msg or "libccfd_cgns.so" in msg, msg
+
+
+def test_cgns_grid_file_is_read_as_a_mesh(tmp_path, monkeypatch):
+    """meshFormat = .cgns (readCGNS, src/mesh.c:1153-1394): set-up writes
+    the grid file of a CGNS-output case before it reaches the device, and a
+    case naming that file as its mesh reads it and gets as far as the
+    device"""
+    from tests.test_gpu_cgns_restart import _stationary
+    L = _lib()
+    if not os.path.exists(os.path.join(os.path.dirname(HOST), "libccfd_cgns.so")):
+        pytest.skip("libccfd_cgns.so not built")
+    monkeypatch.chdir(tmp_path)
+    ini = _stationary(str(tmp_path))
+    assert _device_only(_init(L, ini))
+    assert (tmp_path / "st_mesh.cgns").exists()
+    keys = ("meshtype", "nelemsx", "nelemsy", "x0", "xmax", "nbcsegments",
+            "meshbctype", "filename")
+    lines = [x for x in open(ini).read().splitlines() if _key(x) not in keys]
+    (tmp_path / "rd.ini").write_text("\n".join(lines + [
+        "meshType = 2", "meshFormat = .cgns", "meshFile = st_mesh",
+        "fileName = rd"]) + "\n")
+    msg = _init(L, str(tmp_path / "rd.ini"))
+    assert _device_only(msg), msg
+    (tmp_path / "st_mesh.cgns").write_bytes(b"not hdf5")
+    msg = _init(L, str(tmp_path / "rd.ini"))
+    assert "cannot open" in msg, msg

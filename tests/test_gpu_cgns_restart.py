@@ -11,7 +11,10 @@ src/initialCondition.c:94-158).
   on it, so the runs agree to about 1e-12 rather than bit for bit.
 * Curve (outputFormat 2): the CSV's sorted rows as density, velocity and
   pressure blocks.
-* A restart file of another mesh is refused."""
+* A restart file of another mesh is refused.
+* CGNS mesh input (meshFormat .cgns, readCGNS, src/mesh.c:1153-1394): the
+  grid file of a run, read back as the mesh of the same case, gives the
+  same solution bit for bit."""
 import ctypes as C
 import glob
 import os
@@ -162,3 +165,24 @@ def test_curve_output_has_the_csv_rows():
             rows = [r.split(",") for r in b.splitlines()[1:] if r]
             assert [r[0] for r in rows] == [c[0] for c in csv]
             assert [r[1] for r in rows] == [c[col] for c in csv]
+
+
+def test_cgns_grid_file_as_mesh_input_reproduces_the_run():
+    L = _cgns()
+    n = 20 * 12
+    mesh_keys = ("meshType", "nElemsX", "nElemsY", "X0", "Xmax",
+                 "nBCsegments", "meshBCtype", "fileName")
+    with tempfile.TemporaryDirectory() as d:
+        _run(d, _stationary(d))
+        full, t_full = _read(L, os.path.join(d, "st_000000060.cgns"), n)
+        with open(os.path.join(d, "st.ini")) as f:
+            lines = [x for x in f.read().splitlines()
+                     if x.split("=")[0].strip() not in mesh_keys]
+        lines += ["meshType = 2", "meshFormat = .cgns", "meshFile = st_mesh",
+                  "fileName = rd"]
+        with open(os.path.join(d, "rd.ini"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        _run(d, "rd.ini")
+        got, t_got = _read(L, os.path.join(d, "rd_000000060.cgns"), n)
+        assert t_got == t_full
+        assert np.array_equal(got, full)
