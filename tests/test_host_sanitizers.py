@@ -6,8 +6,9 @@ context, freeing what it built), a missing file, and the gmsh meshes of the
 reference's own cases read, built, refused when opened up, and freed; and
 the host C++ of the device library (tests/native/host_sanitize_cxx.cpp):
 device layout, halo lists and multigrid levels of those meshes, and the
-batched arena's allocator through random sequences. Any sanitizer report
-fails the test. Host only."""
+batched arena's allocator through random sequences; and the CGNS library
+(tests/native/cgns_sanitize.c). Any sanitizer report fails the test. Host
+only."""
 import glob
 import os
 import shutil
