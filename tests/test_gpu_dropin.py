@@ -8,7 +8,11 @@ way ccfd's own C code calls them, against the oracle:
   vectors: the oracle's serial sum to 1e-13;
 * GMRES_M (src/linearSolver.c:385-510) on the implicit Sod case: the
   oracle's single GMRES cycle to 1e-7;
-* the complete C program of INTEGRATION.md, compiled and run.
+* the complete C program of INTEGRATION.md, compiled and run;
+* `ccfd sod_implicit.ini`, the whole implicit run, with and without LU-SGS:
+  the Newton and GMRES totals it prints (src/timeDiscretization.c:753-756)
+  equal those of the reference's own run of the same file (60 / 300 with
+  LU-SGS, 107 / 535 without, in 10 time steps).
 """
 import ctypes as C
 import os
@@ -212,3 +216,23 @@ def test_the_integration_program_runs(tmp_path):
     _, _, prim, mesh, _ = TD.run_case(ini)
     _, otext = TD.csv_rows(mesh, prim)
     assert text == otext
+
+
+@pytest.mark.parametrize("precond,newton,gmres", [("T", 60, 300),
+                                                  ("F", 107, 535)])
+def test_implicit_sod_run_takes_the_reference_iterations(tmp_path, precond,
+                                                         newton, gmres):
+    exe = os.path.join(LIB, "ccfd")
+    lines = []
+    for x in open(SOD_IMPLICIT).read().splitlines():
+        if x.split("!")[0].split("=")[0].strip().lower() == "precond":
+            x = "precond=%s" % precond
+        lines.append(x)
+    (tmp_path / "sod.ini").write_text("\n".join(lines) + "\n")
+    print("run: ccfd sod.ini, precond=%s" % precond, flush=True)
+    r = subprocess.run([exe, "sod.ini"], cwd=tmp_path, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:]
+    print(r.stdout[-400:])
+    assert "| Newton Iterations: %d\n" % newton in r.stdout
+    assert "| GMRES Iterations : %d\n" % gmres in r.stdout

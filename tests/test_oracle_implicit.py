@@ -298,3 +298,32 @@ def test_dot_is_the_sum_of_products():
         scale = math.fsum(np.abs(a * b).ravel().tolist())
         assert abs(B.dot(a, b) - exact) <= 1e-15 * n * scale
     assert B.dot(np.zeros((3, 4)), np.ones((3, 4))) == 0.0
+
+
+@pytest.mark.parametrize("precond,newton,gmres", [(1, 60, 300),
+                                                  (0, 107, 535)])
+def test_sod_implicit_run_takes_the_reference_iterations(precond, newton,
+                                                          gmres):
+    """calc/riemann/sod_implicit.ini run to tEnd: 10 time steps with the
+    Newton and GMRES totals the reference's own run of that file prints
+    (src/timeDiscretization.c:755-756): 60 / 300 with LU-SGS, 107 / 535
+    without. Each Newton count depends on every residual norm of the run,
+    so a change in the step, the Jacobian, the preconditioner or the
+    Eisenstat-Walker tolerance shows here."""
+    import os
+    from oracle import timedisc as TD
+    ini = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                       "calc", "sod_implicit.ini")
+    mesh, cfg, prim, c = TD.setup_case(ini)
+    orc = B.Oracle(mesh, cfg)
+    orc.set_prim(prim)
+    ic = B.ImplicitCfg(c["nKdim"], c["nNewtonIter"], precond, c["epsNewton"],
+                       c["epsGMRES"], c["gammaEW"])
+    t, steps, nn, ng = 0.0, 0, 0, 0
+    while c["tEnd"] - t > 1e-15:
+        dt, _, bad = orc.calc_dt(t, c["IOtimeInterval"], c["tEnd"])
+        assert not bad
+        rc, _, a, b = orc.implicit_step(ic, t, dt)
+        assert rc == 0
+        nn, ng, t, steps = nn + a, ng + b, t + dt, steps + 1
+    assert (steps, nn, ng) == (10, newton, gmres)
