@@ -35,6 +35,14 @@ def _pair(mesh, flux, order, lim, k=5.0, ns=0, mu=0.0, precision=G.FP64,
 VARIANTS = pytest.mark.parametrize(
     "variant", [G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE],
     ids=["split", "fused", "fused_face"])
+# (order, limiter, operator implementation): the fused operators are 2nd
+# order only, so 1st order runs the split pair alone
+ORDER_VARIANTS = pytest.mark.parametrize(
+    "order,lim,variant",
+    [(1, 1, G.KV_SPLIT)] + [(2, lim, v) for lim in (1, 2)
+                            for v in (G.KV_SPLIT, G.KV_FUSED, G.KV_FUSED_FACE)],
+    ids=["o1-split"] + ["o2-l%d-%s" % (lim, n) for lim in (1, 2)
+                        for n in ("split", "fused", "fused_face")])
 
 
 def _check_bitwise(gs, orc, prim, time=0.0, order=2):
@@ -112,11 +120,8 @@ def test_fvtd_periodic_delaunay_bitwise(flux, variant):
     gs.close()
 
 
-@pytest.mark.parametrize("order,lim", [(1, 1), (2, 1), (2, 2)])
-@VARIANTS
+@ORDER_VARIANTS
 def test_fvtd_all_bc_types_bitwise(order, lim, variant):
-    if order == 1 and variant in (G.KV_FUSED, G.KV_FUSED_FACE):
-        pytest.skip("the fused operator is 2nd order only")
     mesh, _ = cases.mixed_bc_mesh()
     gs, orc = _pair(mesh, 5, order, lim, k=5.0, variant=variant)
     prim = cases.random_state(mesh.n_elems, seed=11, mach=0.5)
@@ -124,13 +129,10 @@ def test_fvtd_all_bc_types_bitwise(order, lim, variant):
     gs.close()
 
 
-@pytest.mark.parametrize("order,lim", [(1, 1), (2, 1), (2, 2)])
-@VARIANTS
+@ORDER_VARIANTS
 def test_fvtd_mixed_triangles_and_quads_bitwise(order, lim, variant):
     """elements with 3 and 4 sides in one mesh: the absent fourth slot of
     every triangle is skipped by every kernel"""
-    if order == 1 and variant in (G.KV_FUSED, G.KV_FUSED_FACE):
-        pytest.skip("the fused operator is 2nd order only")
     mesh, _ = cases.mixed_mesh()
     assert len(mesh.raw.tri) > 100 and len(mesh.raw.quad) > 100
     gs, orc = _pair(mesh, 9, order, lim, k=5.0, variant=variant)
