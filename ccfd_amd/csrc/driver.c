@@ -148,7 +148,7 @@ static int open_analysis_files(char *e)
 			fclose(old);
 		C.resFile = fopen(name, old ? "a" : "w");
 		if (!C.resFile) {
-			snprintf(e, 256, "cannot open %s", name);
+			snprintf(e, 256, "cannot open %.200s", name);
 			return -1;
 		}
 		if (old)
@@ -191,7 +191,7 @@ static int open_analysis_files(char *e)
 		snprintf(name, sizeof(name), "%s_recordPoint_%ld.csv", C.fileName, k);
 		C.recFile[k] = fopen(name, "w");
 		if (!C.recFile[k]) {
-			snprintf(e, 256, "cannot open %s", name);
+			snprintf(e, 256, "cannot open %.200s", name);
 			return -1;
 		}
 		fprintf(C.recFile[k], "Time, Density, VelocityX, VelocityY, Pressure\n");
